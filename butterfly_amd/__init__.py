@@ -1,0 +1,424 @@
+"""Python host mirror of the MI355X 2-D Helmholtz butterfly library.
+
+Loads the C-ABI library ``libbutterfly_amd.so`` (built from ``csrc/``) and
+mirrors the reference's Python wrapper for this path
+(wrappers/python/butterfly.pyx): ``Helm2`` (butterfly.pyx:273-328),
+``Quadtree``, ``FacHelm2.make_multilevel`` (butterfly.pyx:211-220) and
+``Mat.__matmul__`` (butterfly.pyx:390-396).  The factors live in GPU
+memory; this module only moves arrays across the C ABI.  There is no CPU
+fallback: without the built library or a GPU the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbutterfly_amd.so")
+
+BF_LAYER_POTENTIAL_SINGLE = 1
+PRECISION = {"fp64": 0, "fp32": 1, "fp32x2": 2}
+
+ERROR_NAMES = {
+    0: "NONE", 1: "INVALID_ARGUMENTS", 2: "RUNTIME_ERROR", 3: "NOT_IMPLEMENTED",
+    4: "MEMORY_ERROR", 5: "OUT_OF_RANGE", 6: "FILE_ERROR", 7: "TYPE_ERROR",
+    8: "INCOMPATIBLE_SHAPES",
+}
+
+
+class BfError(RuntimeError):
+    """Raised when a library call records an error code (error.c:7-24)."""
+
+
+class Helm2Struct(C.Structure):
+    _fields_ = [("k", C.c_double), ("layerPot", C.c_int),
+                ("alpha", C.c_double * 2), ("beta", C.c_double * 2)]
+
+
+class Points2Struct(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_size_t),
+                ("capacity", C.c_size_t), ("isView", C.c_bool)]
+
+
+class PermStruct(C.Structure):
+    _fields_ = [("index", C.POINTER(C.c_size_t)), ("size", C.c_size_t), ("isView", C.c_bool)]
+
+
+class TreeStruct(C.Structure):
+    _fields_ = [("vtable", C.c_void_p), ("root", C.c_void_p), ("perm", C.POINTER(PermStruct))]
+
+
+class QuadtreeStruct(C.Structure):
+    _fields_ = [("super", TreeStruct), ("points", C.c_void_p), ("unitNormals", C.c_void_p),
+                ("nodes", C.c_void_p), ("numNodes", C.c_size_t), ("leafSize", C.c_size_t),
+                ("levelOffset", C.POINTER(C.c_size_t)), ("numLevels", C.c_size_t)]
+
+
+class FacOptionsStruct(C.Structure):
+    _fields_ = [("tol", C.c_double), ("rankConst", C.c_double),
+                ("maxDenseSize", C.c_size_t), ("passThrough", C.c_bool)]
+
+
+class MatDenseComplexStruct(C.Structure):
+    _fields_ = [("vtbl", C.c_void_p), ("props", C.c_int), ("numRows", C.c_size_t),
+                ("numCols", C.c_size_t), ("rowStride", C.c_size_t), ("colStride", C.c_size_t),
+                ("data", C.c_void_p), ("pyArray", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the C-ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BfError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C butterfly_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, dp, i64p = C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    sig = {
+        "bfGetError": (C.c_int, []),
+        "bfSetError": (None, [C.c_int]),
+        "bfSeed": (None, [sz]),
+        "bfRealUniform1": (C.c_double, []),
+        "bfRealUniform": (None, [sz, dp]),
+        "bfRealRandn": (None, [sz, dp]),
+        "bfComplexRandn": (None, [sz, dp]),
+        "bfPoints2NewViewFromPtr": (vp, [dp, sz]),
+        "bfPoints2DeinitAndDealloc": (None, [C.POINTER(vp)]),
+        "bfQuadtreeNew": (vp, []),
+        "bfQuadtreeInitWithLeafSize": (None, [vp, vp, vp, sz]),
+        "bfQuadtreeDeinitAndDealloc": (None, [C.POINTER(vp)]),
+        "bfHelm2RankEstForTwoCircles": (sz, [vp, vp, vp, C.c_double, C.c_double]),
+        "bfHelm2GetKernelMatrix": (vp, [vp, vp, vp, vp, vp]),
+        "bfFacHelm2OptionsInitDefault": (None, [vp]),
+        "bfFacHelm2MakeMultilevel": (vp, [vp, vp, vp]),
+        "bfFacHelm2MakeMultilevelWithOptions": (vp, [vp, vp, vp, vp]),
+        "bfhipFacHelm2Plan": (vp, [vp, vp, vp, vp]),
+        "bfMatGetType": (C.c_int, [vp]),
+        "bfMatGetNumRows": (sz, [vp]),
+        "bfMatGetNumCols": (sz, [vp]),
+        "bfMatNumBytes": (sz, [vp]),
+        "bfMatDelete": (None, [C.POINTER(vp)]),
+        "bfMatMul": (vp, [vp, vp]),
+        "bfMatDenseComplexNew": (vp, []),
+        "bfMatDenseComplexInitViewFromPtr": (None, [vp, sz, sz, vp]),
+        "bfMatProductNumFactors": (sz, [vp, sz]),
+        "bfhipMalloc": (vp, [sz]),
+        "bfhipFree": (None, [vp]),
+        "bfhipMemcpyHtoD": (C.c_int, [vp, vp, sz]),
+        "bfhipMemcpyDtoH": (C.c_int, [vp, vp, sz]),
+        "bfhipSynchronize": (C.c_int, []),
+        "bfhipDeviceCount": (C.c_int, []),
+        "bfhipMatMulDevice": (C.c_int, [vp, vp, vp, sz, C.c_int]),
+        "bfhipFacPrepareMixed": (C.c_int, [vp, C.c_int]),
+        "bfhipFacGetStats": (None, [vp, dp]),
+        "bfhipGetKernelTimes": (None, [dp]),
+        "bfhipGetKernelTimeSums": (None, [dp, dp]),
+        "bfhipResetKernelTimes": (None, []),
+        "bfhipFacPlanSizes": (None, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
+        "bfhipFacPlanExport": (None, [vp, i64p, i64p, i64p]),
+        "bfhipFacBlockData": (C.c_int, [vp, sz, dp]),
+        "bfhipFacDenseData": (C.c_int, [vp, sz, dp]),
+        "bfhipHankel0": (C.c_int, [sz, dp, dp]),
+        "bfhipLstSqBatched": (C.c_int, [sz, sz, sz, dp, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def check():
+    """Raise BfError if the library recorded an error (and clear it)."""
+    e = lib().bfGetError()
+    if e:
+        raise BfError(ERROR_NAMES.get(e, str(e)))
+
+
+def seed(s):
+    lib().bfSeed(s)
+
+
+def complex_randn(n):
+    """bfComplexRandn: n complex standard normals from the seeded stream."""
+    out = np.empty(2 * n)
+    lib().bfComplexRandn(n, _dp(out))
+    return out.view(np.complex128)
+
+
+def real_uniform(n):
+    out = np.empty(n)
+    lib().bfRealUniform(n, _dp(out))
+    return out
+
+
+def sample_box(n, x0, x1, y0, y1):
+    """n points uniform in [x0,x1] x [y0,y1], x then y per point (points.c:25-28)."""
+    u = real_uniform(2 * n).reshape(n, 2)
+    return np.stack([(x1 - x0) * u[:, 0] + x0, (y1 - y0) * u[:, 1] + y0], 1)
+
+
+class Helm2:
+    """Kernel parameters (helm2.h:10-15); single-layer potential."""
+
+    def __init__(self, k, layer_pot=BF_LAYER_POTENTIAL_SINGLE):
+        self.s = Helm2Struct(float(k), layer_pot, (0.0, 0.0), (0.0, 0.0))
+
+    @property
+    def k(self):
+        return self.s.k
+
+    def get_kernel_matrix(self, src, tgt):
+        """bfHelm2GetKernelMatrix: dense (tgt x src) kernel, sampled on the GPU."""
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        tgt = np.ascontiguousarray(tgt, dtype=np.float64)
+        L = lib()
+        ps = L.bfPoints2NewViewFromPtr(_dp(src), len(src))
+        pt = L.bfPoints2NewViewFromPtr(_dp(tgt), len(tgt))
+        m = L.bfHelm2GetKernelMatrix(C.byref(self.s), ps, pt, None, None)
+        L.bfPoints2DeinitAndDealloc(C.byref(C.c_void_p(ps)))
+        L.bfPoints2DeinitAndDealloc(C.byref(C.c_void_p(pt)))
+        check()
+        return Mat(m).to_numpy()
+
+
+class Quadtree:
+    """bfQuadtreeInit over an (N, 2) array; keeps the array alive."""
+
+    def __init__(self, points, leaf_size=1):
+        self.points = np.ascontiguousarray(points, dtype=np.float64)
+        L = lib()
+        self._pts = L.bfPoints2NewViewFromPtr(_dp(self.points), len(self.points))
+        self.ptr = L.bfQuadtreeNew()
+        L.bfQuadtreeInitWithLeafSize(self.ptr, self._pts, None, leaf_size)
+        check()
+
+    @property
+    def perm(self):
+        """Tree permutation: tree position i holds original point perm[i]."""
+        q = QuadtreeStruct.from_address(self.ptr)
+        p = q.super.perm.contents
+        return np.ctypeslib.as_array(p.index, shape=(p.size,)).astype(np.int64)
+
+    @property
+    def num_nodes(self):
+        return QuadtreeStruct.from_address(self.ptr).numNodes
+
+    def __del__(self):
+        try:
+            L = lib()
+            L.bfQuadtreeDeinitAndDealloc(C.byref(C.c_void_p(self.ptr)))
+            L.bfPoints2DeinitAndDealloc(C.byref(C.c_void_p(self._pts)))
+        except Exception:
+            pass
+
+
+def options(tol=1e-15, rank_const=1.0, max_dense=128 * 128, pass_through=False):
+    o = FacOptionsStruct()
+    lib().bfFacHelm2OptionsInitDefault(C.byref(o))
+    o.tol, o.rankConst, o.maxDenseSize, o.passThrough = tol, rank_const, max_dense, pass_through
+    return o
+
+
+class Mat:
+    """A BfMat handle owned by Python (freed with bfMatDelete)."""
+
+    def __init__(self, ptr, keep=None):
+        if not ptr:
+            check()
+            raise BfError("NULL matrix")
+        self.ptr = ptr
+        self._keep = keep
+
+    @property
+    def shape(self):
+        L = lib()
+        return (L.bfMatGetNumRows(self.ptr), L.bfMatGetNumCols(self.ptr))
+
+    def to_numpy(self):
+        d = MatDenseComplexStruct.from_address(self.ptr)
+        n = d.numRows * d.numCols
+        a = np.ctypeslib.as_array(C.cast(d.data, C.POINTER(C.c_double)), shape=(2 * n,))
+        return a.copy().view(np.complex128).reshape(d.numRows, d.numCols)
+
+    def __matmul__(self, x):
+        """bfMatMul(self, X): X (N_src x nrhs) in source-tree order."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        squeeze = x.ndim == 1
+        if squeeze:
+            x = x[:, None]
+        L = lib()
+        xm = L.bfMatDenseComplexNew()
+        L.bfMatDenseComplexInitViewFromPtr(xm, x.shape[0], x.shape[1], x.ctypes.data)
+        y = L.bfMatMul(self.ptr, xm)
+        L.bfMatDelete(C.byref(C.c_void_p(xm)))
+        check()
+        out = Mat(y).to_numpy()
+        return out[:, 0] if squeeze else out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().bfMatDelete(C.byref(C.c_void_p(self.ptr)))
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Factorization(Mat):
+    """The device factorization returned by FacHelm2.make_multilevel."""
+
+    def stats(self):
+        out = np.zeros(16)
+        lib().bfhipFacGetStats(self.ptr, _dp(out))
+        keys = ["factor_s", "plan_s", "kernels_s", "entries", "lstsq_blocks", "kernel_blocks",
+                "identity_blocks", "dense_leaves", "butterfly_leaves", "stages", "h0_evals",
+                "apply_factor_bytes", "apply_vector_bytes", "sample_ms", "compress_ms", "apply_plan_s"]
+        return dict(zip(keys, out.tolist()))
+
+    def plan(self):
+        """(leafTab [L,6], facTab [F,4], blkTab [B,7]) in the oracle's layout."""
+        L = lib()
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        L.bfhipFacPlanSizes(self.ptr, C.byref(a), C.byref(b), C.byref(c))
+        lt = np.zeros((a.value, 6), np.int64)
+        ft = np.zeros((b.value, 4), np.int64)
+        bt = np.zeros((c.value, 7), np.int64)
+        p = C.POINTER(C.c_int64)
+        L.bfhipFacPlanExport(self.ptr, lt.ctypes.data_as(p), ft.ctypes.data_as(p), bt.ctypes.data_as(p))
+        return lt, ft, bt
+
+    def block(self, idx, m, n):
+        out = np.zeros(2 * m * n)
+        if lib().bfhipFacBlockData(self.ptr, idx, _dp(out)):
+            check()
+            raise BfError("block copy failed")
+        return out.view(np.complex128).reshape(m, n)
+
+    def dense_leaf(self, idx, m, n):
+        out = np.zeros(2 * m * n)
+        if lib().bfhipFacDenseData(self.ptr, idx, _dp(out)):
+            check()
+            raise BfError("dense copy failed")
+        return out.view(np.complex128).reshape(m, n)
+
+    def prepare_mixed(self, precision="fp32"):
+        if lib().bfhipFacPrepareMixed(self.ptr, PRECISION[precision]):
+            check()
+            raise BfError("prepare_mixed failed")
+
+
+class FacHelm2:
+    @staticmethod
+    def make_multilevel(helm, src_tree, tgt_tree, opts=None):
+        """bfFacHelm2MakeMultilevel (fac_helm2.c:943-1002); opts selects the
+        config-driven planner (bfFacHelm2MakeMultilevelWithOptions)."""
+        L = lib()
+        if opts is None:
+            p = L.bfFacHelm2MakeMultilevel(C.byref(helm.s), src_tree.ptr, tgt_tree.ptr)
+        else:
+            p = L.bfFacHelm2MakeMultilevelWithOptions(C.byref(helm.s), src_tree.ptr, tgt_tree.ptr, C.byref(opts))
+        check()
+        return Factorization(p, keep=(src_tree, tgt_tree))
+
+    @staticmethod
+    def plan_only(helm, src_tree, tgt_tree, opts=None):
+        """Host planner only (bfhipFacHelm2Plan): no GPU needed."""
+        L = lib()
+        if opts is None:
+            opts = options()
+        p = L.bfhipFacHelm2Plan(C.byref(helm.s), src_tree.ptr, tgt_tree.ptr, C.byref(opts))
+        check()
+        return Factorization(p, keep=(src_tree, tgt_tree))
+
+
+def hankel0(x):
+    """Device H0^(1) at the given arguments: returns complex J0 + i Y0."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(2 * len(x))
+    if lib().bfhipHankel0(len(x), _dp(x), _dp(out)):
+        check()
+        raise BfError("hankel0 failed")
+    return out.view(np.complex128)
+
+
+def lstsq_batched(A, B):
+    """Device pinv(A_b) B_b with the reference's truncation, A: (b,p,p), B: (b,p,n)."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    B = np.ascontiguousarray(B, dtype=np.complex128)
+    b, p, _ = A.shape
+    n = B.shape[2]
+    X = np.zeros((b, p, n), np.complex128)
+    rc = lib().bfhipLstSqBatched(b, p, n, _dp(A.view(np.float64)), _dp(B.view(np.float64)),
+                                 _dp(X.view(np.float64)))
+    if rc:
+        check()
+        raise BfError(f"lstsq_batched failed ({rc})")
+    return X
+
+
+class DeviceArray:
+    """A device allocation filled from / read back to numpy (bfhipMalloc)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.ptr = lib().bfhipMalloc(nbytes)
+        if not self.ptr:
+            check()
+            raise BfError("bfhipMalloc failed")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if lib().bfhipMemcpyHtoD(d.ptr, a.ctypes.data, a.nbytes):
+            check()
+        return d
+
+    def to_numpy(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        if lib().bfhipMemcpyDtoH(out.ctypes.data, self.ptr, out.nbytes):
+            check()
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().bfhipFree(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def mul_device(fac, dx, dy, nrhs, precision="fp64"):
+    """bfhipMatMulDevice: Y = A X on resident device buffers."""
+    rc = lib().bfhipMatMulDevice(fac.ptr, dx.ptr, dy.ptr, nrhs, PRECISION[precision])
+    if rc:
+        check()
+        raise BfError(f"apply failed ({rc})")
+
+
+def kernel_times():
+    """Last measured device ms: [sampling, compression, apply fp64, apply fp32]."""
+    out = np.zeros(8)
+    lib().bfhipGetKernelTimes(_dp(out))
+    return out
+
+
+def kernel_time_sums():
+    """(summed ms, counts) per slot since reset_kernel_times()."""
+    s, c = np.zeros(8), np.zeros(8)
+    lib().bfhipGetKernelTimeSums(_dp(s), _dp(c))
+    return s, c
+
+
+def reset_kernel_times():
+    lib().bfhipResetKernelTimes()

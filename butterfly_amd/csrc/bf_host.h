@@ -1,0 +1,147 @@
+/* Internal declarations of the host library (C11). */
+#pragma once
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/bf_amd/bf.h"
+#include "bf_plan.h"
+
+/* ---- launchers in bf_kernels.hip */
+int bfk_init(void);
+int bfk_device_count(void);
+size_t bfk_lds_cap(void);
+size_t bfk_lstsq_lds_bytes(int p, int n);
+void *bfk_malloc(size_t bytes);
+void bfk_free(void *p);
+int bfk_h2d(void *dst, const void *src, size_t bytes);
+int bfk_d2h(void *dst, const void *src, size_t bytes);
+int bfk_memset(void *dst, size_t bytes);
+int bfk_sync(void);
+void bfk_get_times(double out[8]);
+void bfk_get_time_sums(double sums[8], double counts[8]);
+void bfk_reset_times(void);
+int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
+               double *dArena);
+int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
+              const size_t *groupBytes, const double *dSrcXY, double k, double *dArena);
+int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
+              const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs);
+int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
+                  const float *dArenaHi, const float *dArenaLo, const float *dX, float *dPing, float *dPong,
+                  float *dY, int nrhs);
+int bfk_to_f32(const double *dArena, int64_t numEntries, float *dHi, float *dLo);
+int bfk_h0(int64_t n, const double *x, double *out);
+int bfk_lstsq_loaded(int64_t batch, int p, int n, const double *A, const double *B, double *X);
+
+/* ---- growable arrays */
+#define BF_VEC(T) struct { T *a; size_t n, cap; }
+#define BF_VEC_PUSH(v, x)                                                        \
+  do {                                                                           \
+    if ((v).n == (v).cap) {                                                      \
+      size_t nc_ = (v).cap ? 2 * (v).cap : 64;                                   \
+      void *na_ = realloc((v).a, nc_ * sizeof(*(v).a));                          \
+      if (!na_) abort();                                                         \
+      (v).a = na_;                                                               \
+      (v).cap = nc_;                                                             \
+    }                                                                            \
+    (v).a[(v).n++] = (x);                                                        \
+  } while (0)
+#define BF_VEC_FREE(v) (free((v).a), (v).a = NULL, (v).n = (v).cap = 0)
+
+/* ---- the device factorization */
+
+enum { BF_BLOCK_LSTSQ = 0, BF_BLOCK_KERNEL = 1, BF_BLOCK_IDENTITY = 2 };
+
+typedef struct {
+  int64_t dataOff;   /* complex entries into the arena, -1 for identity */
+  int32_t rowInd, colInd, m, n;
+  int32_t kind;
+  int32_t pad;
+} BfHipBlock;
+
+typedef struct {
+  int32_t isDiag;
+  int32_t pad;
+  int64_t numBlockRows, numBlockCols, numBlocks;
+  int64_t firstBlock;          /* into blocks */
+  int64_t rowOffStart;         /* into offs: numBlockRows + 1 entries */
+  int64_t colOffStart;         /* into offs: numBlockCols + 1 entries */
+} BfHipFactor;
+
+typedef struct {
+  int32_t isButterfly;
+  int32_t numFactors;
+  int64_t row0, col0, m, n;
+  int64_t firstFactor;         /* into factors */
+  int64_t denseOff;            /* arena offset of a dense leaf */
+} BfHipLeaf;
+
+typedef BF_VEC(BfHipLeaf) BfHipLeafVec;
+typedef BF_VEC(BfHipFactor) BfHipFactorVec;
+typedef BF_VEC(BfHipBlock) BfHipBlockVec;
+typedef BF_VEC(int64_t) BfI64Vec;
+typedef BF_VEC(int32_t) BfI32Vec;
+typedef BF_VEC(BfkLstsqJob) BfLstsqJobVec;
+typedef BF_VEC(BfkKernelJob) BfKernelJobVec;
+typedef BF_VEC(BfkSeg) BfSegVec;
+typedef BF_VEC(BfkTerm) BfTermVec;
+
+typedef struct BfMatHipButterfly {
+  BfMat super;
+  BfHelm2 helm;
+  BfFacHelm2Options opts;
+  BfSize numSrc, numTgt;
+
+  /* plan */
+  BfHipLeafVec leaves;
+  BfHipFactorVec factors;
+  BfHipBlockVec blocks;
+  BfI64Vec offs;
+  BfLstsqJobVec lstsqJobs;
+  BfKernelJobVec kernelJobs;
+  int64_t arenaEntries;
+
+  /* apply plan */
+  BfSegVec segs;
+  BfTermVec terms;
+  BfI64Vec stageSeg;
+  int numStages;
+  int64_t bufRows[3];  /* rows of ping, pong (per right-hand side) */
+  double applyFactorBytes;
+
+  /* device */
+  double *dArena;
+  float *dArena32, *dArena32lo;
+  double *dSrcXY, *dTgtXY;
+  BfkSeg *dSegs;
+  BfkTerm *dTerms;
+  void *dPing, *dPong;
+  size_t bufBytes;
+
+  double stats[16];
+} BfMatHipButterfly;
+
+/* planner (plan.c) */
+int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt);
+/* apply plan (fac.c) */
+int bfApplyPlanBuild(BfMatHipButterfly *F);
+
+/* matrix vtable (mat.c) */
+struct BfMatVtable {
+  BfType (*GetType)(BfMat const *);
+  BfSize (*NumBytes)(BfMat const *);
+  void (*Delete)(BfMat **);
+  BfMat *(*Copy)(BfMat const *);
+  BfMat *(*Mul)(BfMat const *, BfMat const *);
+  void (*PermuteRows)(BfMat *, BfPerm const *);
+};
+void bfMatInitBase(BfMat *mat, BfMatVtable *vtbl, BfSize numRows, BfSize numCols);
+BfMat *bfMatHipButterflyMul(BfMat const *A, BfMat const *X);
+void bfMatHipButterflyDelete(BfMat **mat);
+BfSize bfMatHipButterflyNumBytes(BfMat const *mat);
+extern BfMatVtable bfMatHipButterflyVtable;
+
+/* timing */
+double bfWallTime(void);

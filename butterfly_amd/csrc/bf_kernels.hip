@@ -1,0 +1,742 @@
+/*
+ * HIP kernels of the 2-D Helmholtz butterfly for gfx950, and their C-ABI
+ * launchers (called by the host library in C).
+ *
+ *   k_sample   kernel-block sampling: K(tgt <- src) blocks written to HBM
+ *              (last factors, evaluation blocks, dense leaves;
+ *              helm2.c:93-125, fac_helm2.c:403-509, :741-775)
+ *   k_lstsq    re-expansion blocks: sample K(test <- equiv) and
+ *              K(test <- orig) into LDS, one-sided Jacobi on the rows of
+ *              the augmented [Z_equiv | Z_orig], then
+ *              X = Z_equiv^+ Z_orig with the reference's truncation
+ *              (helm2.c:321-365, mat_dense_complex.c:1768-1849)
+ *   k_apply1   one stage of the apply for one right-hand side: every
+ *              output row segment sums its column-major blocks times the
+ *              input (mat_product.c:211-245, mat_block_coo.c:382-425,
+ *              mat_block_diag.c:372-407, mat_block_dense.c:512-572)
+ *   k_applyN   the same for many right-hand sides (one lane per column)
+ *
+ * One wave64 per job / segment; all jobs of a level or stage go into one
+ * launch.  Everything here is FP64 unless the name says f32.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "bf_h0.h"
+#include "bf_plan.h"
+
+#define BFK_CHECK(call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      fprintf(stderr, "bf_kernels: %s failed: %s\n", #call, hipGetErrorString(e_)); \
+      return (e_ == hipErrorOutOfMemory || e_ == hipErrorMemoryAllocation) ? -2 : -1; \
+    }                                                                          \
+  } while (0)
+
+static hipStream_t g_stream = nullptr;
+static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+static size_t g_ldsCap = 0;
+static double g_times[8];   /* last measured ms per category */
+static double g_sum[8];     /* accumulated ms per category since the last reset */
+static double g_cnt[8];     /* number of measurements since the last reset */
+
+static constexpr double TWO_PI = 6.283185307179586;
+
+/* ----------------------------------------------------------- helpers */
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+  return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
+__device__ __forceinline__ double2 shfl_xor2(double2 v, int m) {
+  return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+/* proxy point i of n on a circle (circle.c:12-34) */
+__device__ __forceinline__ double2 circle_pt(double cx, double cy, double r, int n, int i) {
+  double scale = TWO_PI / (double)n;
+  double th = scale * (double)i;
+  double s, c;
+  sincos(th, &s, &c);
+  return make_double2(r * c + cx, r * s + cy);
+}
+
+/* G(t <- s) with r = |s - t| (points.c:484-505 order of subtraction) */
+__device__ __forceinline__ double2 green(double k, double2 t, double2 s) {
+  double r = hypot(s.x - t.x, s.y - t.y);
+  double gr, gi;
+  bf_helm2_single(k, r, &gr, &gi);
+  return make_double2(gr, gi);
+}
+
+/* ---------------------------------------------------------- sampling */
+
+#define SAMPLE_THREADS 256
+#define SAMPLE_CHUNK 1024
+
+__global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *__restrict__ jobs,
+                                                           const double2 *__restrict__ srcXY,
+                                                           const double2 *__restrict__ tgtXY, double k,
+                                                           double2 *__restrict__ arena) {
+  __shared__ double2 spts[SAMPLE_CHUNK];
+  const BfkKernelJob J = jobs[blockIdx.x];
+  const int m = J.m;
+  for (int c0 = 0; c0 < J.n; c0 += SAMPLE_CHUNK) {
+    int cn = min(SAMPLE_CHUNK, J.n - c0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < cn; j += SAMPLE_THREADS)
+      spts[j] = J.srcOff >= 0 ? srcXY[J.srcOff + c0 + j] : circle_pt(J.scx, J.scy, J.sr, J.n, c0 + j);
+    __syncthreads();
+    const int64_t cnt = (int64_t)m * cn;
+    double2 *out = arena + J.out + (int64_t)c0 * m;
+    for (int64_t e = threadIdx.x; e < cnt; e += SAMPLE_THREADS) {
+      int j = (int)(e / m);
+      int i = (int)(e - (int64_t)j * m);
+      out[e] = green(k, tgtXY[J.tgtOff + i], spts[j]);
+    }
+  }
+}
+
+/* -------------------------------------------------------- compression */
+
+/* Round-robin pairing: position 0 is fixed, the others rotate by `step`. */
+__device__ __forceinline__ int rr_row(int pos, int step, int P) {
+  return pos == 0 ? 0 : 1 + (pos - 1 + step) % (P - 1);
+}
+
+/* LDS layout for one job: M (p x ld) row-major, ld = p + n + 1 (odd pad),
+ * then weights w[p] as double2 (x = 1/sigma^2). */
+__device__ __forceinline__ void jacobi_rows_and_solve(double2 *M, int p, int n, int ld, double2 *w,
+                                                      double2 *__restrict__ out) {
+  const int lane = threadIdx.x;
+  const int W = p + n;
+  const int P = p + (p & 1);
+  const int npairs = P / 2;
+  int Lp = 64;
+  while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
+  const int pairsPerPass = 64 / Lp;
+  const int slot = lane / Lp;
+  const int sub = lane - slot * Lp;
+  const double thr = 4.0 * 2.220446049250313e-16 * sqrt((double)p);
+
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    int rotated = 0;
+    for (int step = 0; step < P - 1; ++step) {
+      for (int pb = 0; pb < npairs; pb += pairsPerPass) {
+        int pr = pb + slot;
+        int i = -1, j = -1;
+        if (pr < npairs) {
+          int a = rr_row(pr, step, P), b = rr_row(P - 1 - pr, step, P);
+          i = min(a, b);
+          j = max(a, b);
+        }
+        bool valid = pr < npairs && j < p;
+        double al = 0, be = 0;
+        double2 g = make_double2(0, 0);
+        if (valid) {
+          const double2 *ri = M + i * ld, *rj = M + j * ld;
+          for (int c = sub; c < p; c += Lp) {
+            double2 x = ri[c], y = rj[c];
+            al = fma(x.x, x.x, fma(x.y, x.y, al));
+            be = fma(y.x, y.x, fma(y.y, y.y, be));
+            /* g += x * conj(y) */
+            g.x = fma(x.x, y.x, fma(x.y, y.y, g.x));
+            g.y = fma(x.y, y.x, fma(-x.x, y.y, g.y));
+          }
+        }
+        for (int o = 1; o < Lp; o <<= 1) {
+          al += __shfl_xor(al, o, 64);
+          be += __shfl_xor(be, o, 64);
+          g.x += __shfl_xor(g.x, o, 64);
+          g.y += __shfl_xor(g.y, o, 64);
+        }
+        double ag = hypot(g.x, g.y);
+        if (valid && ag > 0 && ag > thr * sqrt(al * be)) {
+          rotated = 1;
+          double ex = g.x / ag, ey = g.y / ag; /* e = g/|g| */
+          double zeta = (be - al) / (2 * ag);
+          double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+          double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+          double2 *ri = M + i * ld, *rj = M + j * ld;
+          for (int c = sub; c < W; c += Lp) {
+            double2 x = ri[c], y = rj[c];
+            double2 ey_ = make_double2(ex * y.x - ey * y.y, ex * y.y + ey * y.x); /* e*y */
+            ri[c] = make_double2(cs * x.x - sn * ey_.x, cs * x.y - sn * ey_.y);
+            rj[c] = make_double2(sn * x.x + cs * ey_.x, sn * x.y + cs * ey_.y);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!__any(rotated)) break;
+  }
+
+  /* sigma_i^2 = ||row i of the rotated Z_equiv||^2 */
+  double smax = 0;
+  for (int i = lane; i < p; i += 64) {
+    const double2 *ri = M + i * ld;
+    double s = 0;
+    for (int c = 0; c < p; ++c) s = fma(ri[c].x, ri[c].x, fma(ri[c].y, ri[c].y, s));
+    w[i].x = s;
+    smax = fmax(smax, s);
+  }
+  for (int o = 1; o < 64; o <<= 1) smax = fmax(smax, __shfl_xor(smax, o, 64));
+  __syncthreads();
+  /* keep sigma >= max(m,n) eps sigma_0 + eps (mat_dense_complex.c:1797-1809) */
+  const double sig0 = sqrt(smax);
+  const double tol = (double)p * 2.220446049250313e-16 * sig0 + 2.220446049250313e-16;
+  for (int i = lane; i < p; i += 64) {
+    double s = sqrt(w[i].x);
+    w[i].y = (s >= tol) ? 1.0 / w[i].x : 0.0;
+  }
+  __syncthreads();
+  /* X = sum_i conj(A'_i)^T w_i B'_i, column-major p x n */
+  const int cnt = p * n;
+  for (int e = lane; e < cnt; e += 64) {
+    int c = e / p;
+    int a = e - c * p;
+    double2 acc = make_double2(0, 0);
+    for (int i = 0; i < p; ++i) {
+      double wi = w[i].y;
+      if (wi == 0) continue;
+      double2 x = M[i * ld + a], y = M[i * ld + p + c];
+      /* conj(x) * y * wi */
+      acc.x = fma(wi, x.x * y.x + x.y * y.y, acc.x);
+      acc.y = fma(wi, x.x * y.y - x.y * y.x, acc.y);
+    }
+    out[e] = acc;
+  }
+}
+
+__device__ __forceinline__ size_t lstsq_lds_bytes(int p, int n) {
+  return ((size_t)p * (p + n + 1) + p + 2 * p + n) * sizeof(double2);
+}
+
+__global__ __launch_bounds__(64) void k_lstsq(const BfkLstsqJob *__restrict__ jobs,
+                                              const int32_t *__restrict__ order,
+                                              const double2 *__restrict__ srcXY, double k,
+                                              double2 *__restrict__ arena) {
+  extern __shared__ double2 lds[];
+  const BfkLstsqJob J = jobs[order[blockIdx.x]];
+  const int p = J.p, n = J.n, ld = p + n + 1;
+  double2 *M = lds;
+  double2 *w = M + (size_t)p * ld;
+  double2 *tp = w + p;     /* test points, p */
+  double2 *ep = tp + p;    /* equiv points, p */
+  double2 *op = ep + p;    /* orig points, n */
+  const int lane = threadIdx.x;
+  for (int i = lane; i < p; i += 64) {
+    tp[i] = circle_pt(J.tcx, J.tcy, J.tr, p, i);
+    ep[i] = circle_pt(J.ecx, J.ecy, J.er, p, i);
+  }
+  for (int c = lane; c < n; c += 64)
+    op[c] = J.origOff >= 0 ? srcXY[J.origOff + c] : circle_pt(J.ocx, J.ocy, J.orr, n, c);
+  __syncthreads();
+  const int W = p + n;
+  for (int e = lane; e < p * W; e += 64) {
+    int i = e / W;
+    int c = e - i * W;
+    double2 s = c < p ? ep[c] : op[c - p];
+    M[i * ld + c] = green(k, tp[i], s);
+  }
+  __syncthreads();
+  jacobi_rows_and_solve(M, p, n, ld, w, arena + J.out);
+}
+
+/* the same solve on matrices given in global memory (row-major A p x p,
+ * B p x n per batch entry); used to check the solver on its own */
+__global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                     int p, int n, double2 *__restrict__ X) {
+  extern __shared__ double2 lds[];
+  const int ld = p + n + 1;
+  double2 *M = lds;
+  double2 *w = M + (size_t)p * ld;
+  const double2 *a = A + (size_t)blockIdx.x * p * p;
+  const double2 *b = B + (size_t)blockIdx.x * p * n;
+  const int W = p + n;
+  for (int e = threadIdx.x; e < p * W; e += 64) {
+    int i = e / W;
+    int c = e - i * W;
+    M[i * ld + c] = c < p ? a[i * p + c] : b[i * n + (c - p)];
+  }
+  __syncthreads();
+  jacobi_rows_and_solve(M, p, n, ld, w, X + (size_t)blockIdx.x * p * n);
+}
+
+/* ---------------------------------------------------------------- apply */
+
+struct BfkBufs {
+  const double2 *in[4];
+  double2 *out[4];
+};
+
+__global__ __launch_bounds__(64) void k_apply1(const BfkSeg *__restrict__ segs, const BfkTerm *__restrict__ terms,
+                                               const double2 *__restrict__ arena, BfkBufs bufs) {
+  const BfkSeg S = segs[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int m = S.m;
+  int R = 64;
+  while (R > 1 && (R >> 1) >= m) R >>= 1;
+  const int G = 64 / R;
+  const int r = lane & (R - 1);
+  const int g = lane / R;
+  double2 *y = bufs.out[S.ybuf] + S.yOff;
+  for (int rb = 0; rb < m; rb += R) {
+    const int i = rb + r;
+    double2 acc = make_double2(0, 0);
+    if (i < m) {
+      for (int t = S.t0; t < S.t1; ++t) {
+        const BfkTerm T = terms[t];
+        const double2 *x = bufs.in[T.xbuf] + T.xOff;
+        if (T.aOff < 0) {
+          if (g == 0) {
+            double2 v = x[i];
+            acc.x += v.x;
+            acc.y += v.y;
+          }
+        } else {
+          const double2 *A = arena + T.aOff + T.rowOff + i;
+          const int64_t lda = T.lda;
+          for (int j = g; j < T.n; j += G) acc = cfma(A[j * lda], x[j], acc);
+        }
+      }
+    }
+    for (int o = R; o < 64; o <<= 1) {
+      acc.x += __shfl_xor(acc.x, o, 64);
+      acc.y += __shfl_xor(acc.y, o, 64);
+    }
+    if (g == 0 && i < m) y[i] = acc;
+  }
+}
+
+#define APPLYN_ROWS 4
+
+__global__ __launch_bounds__(64) void k_applyN(const BfkSeg *__restrict__ segs, const BfkTerm *__restrict__ terms,
+                                               const double2 *__restrict__ arena, BfkBufs bufs, int nrhs) {
+  const BfkSeg S = segs[blockIdx.x];
+  const int c = blockIdx.y * 64 + threadIdx.x;
+  const bool live = c < nrhs;
+  const int m = S.m;
+  double2 *y = bufs.out[S.ybuf];
+  for (int rb = 0; rb < m; rb += APPLYN_ROWS) {
+    double2 acc[APPLYN_ROWS];
+    for (int q = 0; q < APPLYN_ROWS; ++q) acc[q] = make_double2(0, 0);
+    const int nr = min(APPLYN_ROWS, m - rb);
+    for (int t = S.t0; t < S.t1; ++t) {
+      const BfkTerm T = terms[t];
+      const double2 *x = bufs.in[T.xbuf];
+      if (T.aOff < 0) {
+        for (int q = 0; q < nr; ++q)
+          if (live) {
+            double2 v = x[(T.xOff + rb + q) * nrhs + c];
+            acc[q].x += v.x;
+            acc[q].y += v.y;
+          }
+      } else {
+        const double2 *A = arena + T.aOff + T.rowOff + rb;
+        const int64_t lda = T.lda;
+        for (int j = 0; j < T.n; ++j) {
+          double2 xv = live ? x[(T.xOff + j) * nrhs + c] : make_double2(0, 0);
+          for (int q = 0; q < nr; ++q) acc[q] = cfma(A[j * lda + q], xv, acc[q]);
+        }
+      }
+    }
+    if (live)
+      for (int q = 0; q < nr; ++q) y[(S.yOff + rb + q) * nrhs + c] = acc[q];
+  }
+}
+
+/* ----------------------------------------------------------------- H0 */
+
+__global__ void k_h0(int64_t n, const double *__restrict__ x, double2 *__restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double j, y;
+    bf_h0(x[i], &j, &y);
+    out[i] = make_double2(j, y);
+  }
+}
+
+/* ============================================================ launchers */
+
+extern "C" {
+
+int bfk_init(void) {
+  if (g_stream) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    fprintf(stderr, "bf_kernels: no HIP device available\n");
+    return -1;
+  }
+  BFK_CHECK(hipSetDevice(0));
+  BFK_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  BFK_CHECK(hipEventCreate(&g_ev0));
+  BFK_CHECK(hipEventCreate(&g_ev1));
+  hipDeviceProp_t prop;
+  BFK_CHECK(hipGetDeviceProperties(&prop, 0));
+  g_ldsCap = prop.sharedMemPerBlock;
+  if (g_ldsCap > 65536) {
+    if (hipFuncSetAttribute((const void *)k_lstsq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_ldsCap) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void *)k_lstsq_loaded, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_ldsCap) != hipSuccess) {
+      (void)hipGetLastError();
+      g_ldsCap = 65536;
+    }
+  }
+  return 0;
+}
+
+int bfk_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+size_t bfk_lds_cap(void) { return g_ldsCap; }
+
+size_t bfk_lstsq_lds_bytes(int p, int n) {
+  return ((size_t)p * (p + n + 1) + p + 2 * p + n) * 16;
+}
+
+void *bfk_malloc(size_t bytes) {
+  if (bfk_init()) return nullptr;
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+
+void bfk_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+int bfk_h2d(void *dst, const void *src, size_t bytes) {
+  if (bfk_init()) return -1;
+  BFK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int bfk_d2h(void *dst, const void *src, size_t bytes) {
+  if (bfk_init()) return -1;
+  BFK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int bfk_memset(void *dst, size_t bytes) {
+  if (bfk_init()) return -1;
+  BFK_CHECK(hipMemsetAsync(dst, 0, bytes, g_stream));
+  return 0;
+}
+
+int bfk_sync(void) {
+  if (bfk_init()) return -1;
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+void bfk_get_times(double out[8]) {
+  for (int i = 0; i < 8; ++i) out[i] = g_times[i];
+}
+
+void bfk_get_time_sums(double sums[8], double counts[8]) {
+  for (int i = 0; i < 8; ++i) {
+    sums[i] = g_sum[i];
+    counts[i] = g_cnt[i];
+  }
+}
+
+void bfk_reset_times(void) {
+  for (int i = 0; i < 8; ++i) g_sum[i] = g_cnt[i] = 0;
+}
+
+static int time_begin(void) {
+  BFK_CHECK(hipEventRecord(g_ev0, g_stream));
+  return 0;
+}
+static int time_end(int slot) {
+  BFK_CHECK(hipEventRecord(g_ev1, g_stream));
+  BFK_CHECK(hipEventSynchronize(g_ev1));
+  float ms = 0;
+  BFK_CHECK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
+  g_times[slot] = ms;
+  g_sum[slot] += ms;
+  g_cnt[slot] += 1;
+  return 0;
+}
+
+/* kernel blocks; jobs already in device memory */
+int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
+               double *dArena) {
+  if (bfk_init()) return -1;
+  if (numJobs == 0) {
+    g_times[0] = 0;
+    return 0;
+  }
+  if (time_begin()) return -1;
+  const int64_t maxGrid = 1 << 30;
+  for (int64_t j0 = 0; j0 < numJobs; j0 += maxGrid) {
+    int64_t nj = numJobs - j0 < maxGrid ? numJobs - j0 : maxGrid;
+    hipLaunchKernelGGL(k_sample, dim3((unsigned)nj), dim3(SAMPLE_THREADS), 0, g_stream, dJobs + j0,
+                       (const double2 *)dSrcXY, (const double2 *)dTgtXY, k, (double2 *)dArena);
+    BFK_CHECK(hipGetLastError());
+  }
+  return time_end(0);
+}
+
+/* re-expansion blocks; `dOrder` lists job indices grouped by LDS need,
+ * `groupStart/groupBytes` give [numGroups] contiguous ranges of dOrder
+ * and the dynamic LDS each range needs */
+int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
+              const size_t *groupBytes, const double *dSrcXY, double k, double *dArena) {
+  if (bfk_init()) return -1;
+  if (time_begin()) return -1;
+  for (int gi = 0; gi < numGroups; ++gi) {
+    int64_t s = groupStart[gi], e = groupStart[gi + 1];
+    if (e <= s) continue;
+    if (groupBytes[gi] > g_ldsCap) {
+      fprintf(stderr, "bf_kernels: re-expansion block needs %zu B of LDS (cap %zu)\n", groupBytes[gi], g_ldsCap);
+      return -3;
+    }
+    const int64_t maxGrid = 1 << 30;
+    for (int64_t j0 = s; j0 < e; j0 += maxGrid) {
+      int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
+      hipLaunchKernelGGL(k_lstsq, dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs, dOrder + j0,
+                         (const double2 *)dSrcXY, k, (double2 *)dArena);
+      BFK_CHECK(hipGetLastError());
+    }
+  }
+  return time_end(1);
+}
+
+/* one apply: `numStages` launches; stage s covers segments
+ * [stageSeg[s], stageSeg[s+1]) */
+int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
+              const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs) {
+  if (bfk_init()) return -1;
+  BfkBufs b;
+  b.in[BFK_BUF_X] = (const double2 *)dX;
+  b.in[BFK_BUF_PING] = (const double2 *)dPing;
+  b.in[BFK_BUF_PONG] = (const double2 *)dPong;
+  b.in[BFK_BUF_Y] = (const double2 *)dY;
+  b.out[BFK_BUF_X] = nullptr;
+  b.out[BFK_BUF_PING] = (double2 *)dPing;
+  b.out[BFK_BUF_PONG] = (double2 *)dPong;
+  b.out[BFK_BUF_Y] = (double2 *)dY;
+  if (time_begin()) return -1;
+  for (int s = 0; s < numStages; ++s) {
+    int64_t n = stageSeg[s + 1] - stageSeg[s];
+    if (n == 0) continue;
+    if (nrhs == 1) {
+      hipLaunchKernelGGL(k_apply1, dim3((unsigned)n), dim3(64), 0, g_stream, dSegs + stageSeg[s], dTerms,
+                         (const double2 *)dArena, b);
+    } else {
+      hipLaunchKernelGGL(k_applyN, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
+                         dSegs + stageSeg[s], dTerms, (const double2 *)dArena, b, nrhs);
+    }
+    BFK_CHECK(hipGetLastError());
+  }
+  return time_end(2);
+}
+
+int bfk_h0(int64_t n, const double *x, double *out) {
+  if (bfk_init()) return -1;
+  double *dx = nullptr, *dout = nullptr;
+  BFK_CHECK(hipMalloc(&dx, (n ? n : 1) * sizeof(double)));
+  BFK_CHECK(hipMalloc(&dout, (n ? n : 1) * 2 * sizeof(double)));
+  BFK_CHECK(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_h0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g_stream, n, dx, (double2 *)dout);
+    BFK_CHECK(hipGetLastError());
+  }
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  BFK_CHECK(hipMemcpy(out, dout, n * 2 * sizeof(double), hipMemcpyDeviceToHost));
+  BFK_CHECK(hipFree(dx));
+  BFK_CHECK(hipFree(dout));
+  return 0;
+}
+
+int bfk_lstsq_loaded(int64_t batch, int p, int n, const double *A, const double *B, double *X) {
+  if (bfk_init()) return -1;
+  size_t lds = ((size_t)p * (p + n + 1) + p) * 16;
+  if (lds > g_ldsCap || p > 128 || p < 1 || n < 1) return -3;
+  size_t sa = (size_t)batch * p * p * 16, sb = (size_t)batch * p * n * 16;
+  double *dA = nullptr, *dB = nullptr, *dX = nullptr;
+  BFK_CHECK(hipMalloc(&dA, sa));
+  BFK_CHECK(hipMalloc(&dB, sb));
+  BFK_CHECK(hipMalloc(&dX, sb));
+  BFK_CHECK(hipMemcpy(dA, A, sa, hipMemcpyHostToDevice));
+  BFK_CHECK(hipMemcpy(dB, B, sb, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_lstsq_loaded, dim3((unsigned)batch), dim3(64), lds, g_stream, (const double2 *)dA,
+                     (const double2 *)dB, p, n, (double2 *)dX);
+  BFK_CHECK(hipGetLastError());
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  BFK_CHECK(hipMemcpy(X, dX, sb, hipMemcpyDeviceToHost));
+  BFK_CHECK(hipFree(dA));
+  BFK_CHECK(hipFree(dB));
+  BFK_CHECK(hipFree(dX));
+  return 0;
+}
+
+} /* extern "C" */
+
+/* ====================================================== FP32 apply path */
+
+struct BfkBufsF {
+  const float2 *in[4];
+  float2 *out[4];
+};
+
+__device__ __forceinline__ float2 cfmaf(float2 a, float2 b, float2 c) {
+  return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, c.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, c.y)));
+}
+
+/* factors as FP32 (hi) plus an optional FP32 remainder (lo), FP32
+ * vectors, FP32 accumulation */
+__global__ __launch_bounds__(64) void k_apply1_f32(const BfkSeg *__restrict__ segs, const BfkTerm *__restrict__ terms,
+                                                   const float2 *__restrict__ hi, const float2 *__restrict__ lo,
+                                                   BfkBufsF bufs) {
+  const BfkSeg S = segs[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int m = S.m;
+  int R = 64;
+  while (R > 1 && (R >> 1) >= m) R >>= 1;
+  const int G = 64 / R;
+  const int r = lane & (R - 1);
+  const int g = lane / R;
+  float2 *y = bufs.out[S.ybuf] + S.yOff;
+  for (int rb = 0; rb < m; rb += R) {
+    const int i = rb + r;
+    float2 acc = make_float2(0, 0);
+    if (i < m) {
+      for (int t = S.t0; t < S.t1; ++t) {
+        const BfkTerm T = terms[t];
+        const float2 *x = bufs.in[T.xbuf] + T.xOff;
+        if (T.aOff < 0) {
+          if (g == 0) {
+            float2 v = x[i];
+            acc.x += v.x;
+            acc.y += v.y;
+          }
+        } else {
+          const int64_t base = T.aOff + T.rowOff + i;
+          const int64_t lda = T.lda;
+          if (lo) {
+            for (int j = g; j < T.n; j += G) {
+              float2 xv = x[j];
+              acc = cfmaf(lo[base + j * lda], xv, acc);
+              acc = cfmaf(hi[base + j * lda], xv, acc);
+            }
+          } else {
+            for (int j = g; j < T.n; j += G) acc = cfmaf(hi[base + j * lda], x[j], acc);
+          }
+        }
+      }
+    }
+    for (int o = R; o < 64; o <<= 1) {
+      acc.x += __shfl_xor(acc.x, o, 64);
+      acc.y += __shfl_xor(acc.y, o, 64);
+    }
+    if (g == 0 && i < m) y[i] = acc;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_applyN_f32(const BfkSeg *__restrict__ segs, const BfkTerm *__restrict__ terms,
+                                                   const float2 *__restrict__ hi, const float2 *__restrict__ lo,
+                                                   BfkBufsF bufs, int nrhs) {
+  const BfkSeg S = segs[blockIdx.x];
+  const int c = blockIdx.y * 64 + threadIdx.x;
+  const bool live = c < nrhs;
+  const int m = S.m;
+  float2 *y = bufs.out[S.ybuf];
+  for (int rb = 0; rb < m; rb += APPLYN_ROWS) {
+    float2 acc[APPLYN_ROWS];
+    for (int q = 0; q < APPLYN_ROWS; ++q) acc[q] = make_float2(0, 0);
+    const int nr = min(APPLYN_ROWS, m - rb);
+    for (int t = S.t0; t < S.t1; ++t) {
+      const BfkTerm T = terms[t];
+      const float2 *x = bufs.in[T.xbuf];
+      if (T.aOff < 0) {
+        for (int q = 0; q < nr; ++q)
+          if (live) {
+            float2 v = x[(T.xOff + rb + q) * nrhs + c];
+            acc[q].x += v.x;
+            acc[q].y += v.y;
+          }
+      } else {
+        const int64_t base = T.aOff + T.rowOff + rb;
+        const int64_t lda = T.lda;
+        for (int j = 0; j < T.n; ++j) {
+          float2 xv = live ? x[(T.xOff + j) * nrhs + c] : make_float2(0, 0);
+          for (int q = 0; q < nr; ++q) {
+            if (lo) acc[q] = cfmaf(lo[base + j * lda + q], xv, acc[q]);
+            acc[q] = cfmaf(hi[base + j * lda + q], xv, acc[q]);
+          }
+        }
+      }
+    }
+    if (live)
+      for (int q = 0; q < nr; ++q) y[(S.yOff + rb + q) * nrhs + c] = acc[q];
+  }
+}
+
+__global__ void k_to_f32(const double2 *__restrict__ a, int64_t n, float2 *__restrict__ hi, float2 *__restrict__ lo) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    double2 v = a[i];
+    float2 h = make_float2((float)v.x, (float)v.y);
+    hi[i] = h;
+    if (lo) lo[i] = make_float2((float)(v.x - (double)h.x), (float)(v.y - (double)h.y));
+  }
+}
+
+extern "C" {
+
+int bfk_to_f32(const double *dArena, int64_t numEntries, float *dHi, float *dLo) {
+  if (bfk_init()) return -1;
+  if (numEntries == 0) return 0;
+  hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, g_stream, (const double2 *)dArena, numEntries,
+                     (float2 *)dHi, (float2 *)dLo);
+  BFK_CHECK(hipGetLastError());
+  return 0;
+}
+
+int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
+                  const float *dArenaHi, const float *dArenaLo, const float *dX, float *dPing, float *dPong, float *dY,
+                  int nrhs) {
+  if (bfk_init()) return -1;
+  BfkBufsF b;
+  b.in[BFK_BUF_X] = (const float2 *)dX;
+  b.in[BFK_BUF_PING] = (const float2 *)dPing;
+  b.in[BFK_BUF_PONG] = (const float2 *)dPong;
+  b.in[BFK_BUF_Y] = (const float2 *)dY;
+  b.out[BFK_BUF_X] = nullptr;
+  b.out[BFK_BUF_PING] = (float2 *)dPing;
+  b.out[BFK_BUF_PONG] = (float2 *)dPong;
+  b.out[BFK_BUF_Y] = (float2 *)dY;
+  if (time_begin()) return -1;
+  for (int s = 0; s < numStages; ++s) {
+    int64_t n = stageSeg[s + 1] - stageSeg[s];
+    if (n == 0) continue;
+    if (nrhs == 1) {
+      hipLaunchKernelGGL(k_apply1_f32, dim3((unsigned)n), dim3(64), 0, g_stream, dSegs + stageSeg[s], dTerms,
+                         (const float2 *)dArenaHi, (const float2 *)dArenaLo, b);
+    } else {
+      hipLaunchKernelGGL(k_applyN_f32, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
+                         dSegs + stageSeg[s], dTerms, (const float2 *)dArenaHi, (const float2 *)dArenaLo, b, nrhs);
+    }
+    BFK_CHECK(hipGetLastError());
+  }
+  return time_end(3);
+}
+
+} /* extern "C" */

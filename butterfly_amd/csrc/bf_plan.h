@@ -1,0 +1,67 @@
+/*
+ * Plan records shared by the host planner (C) and the HIP kernels.
+ * Plain C types only; every record is a fixed-size POD so whole tables
+ * are copied to HBM once and indexed by blockIdx.
+ */
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One re-expansion block (helm2.c:321-365):
+ *   X (p x n) = pinv(K(test <- equiv)) K(test <- orig)
+ * test: p points on circle (tcx, tcy, tr); equiv: p points on circle
+ * (ecx, ecy, er); orig: n points on circle (ocx, ocy, orr) when
+ * origOff < 0, else the source points [origOff, origOff + n) in
+ * source-tree order.  X is written column-major at `out`. */
+typedef struct BfkLstsqJob {
+  double ecx, ecy, er;
+  double tcx, tcy, tr;
+  double ocx, ocy, orr;
+  int64_t origOff;
+  int64_t out;
+  int32_t p, n;
+} BfkLstsqJob;
+
+/* One kernel block (helm2.c:93-125): K(tgt <- src), m x n, rows are the
+ * target points [tgtOff, tgtOff + m) in target-tree order, columns are
+ * n points on circle (scx, scy, sr) when srcOff < 0, else the source
+ * points [srcOff, srcOff + n).  Written column-major at `out`. */
+typedef struct BfkKernelJob {
+  double scx, scy, sr;
+  int64_t srcOff;
+  int64_t tgtOff;
+  int64_t out;
+  int32_t m, n;
+} BfkKernelJob;
+
+/* Vector buffers an apply stage reads or writes. */
+enum { BFK_BUF_X = 0, BFK_BUF_PING = 1, BFK_BUF_PONG = 2, BFK_BUF_Y = 3 };
+
+/* One output row segment of an apply stage: rows [yOff, yOff + m) of
+ * buffer ybuf are the sum of terms [t0, t1). */
+typedef struct BfkSeg {
+  int64_t yOff;
+  int32_t m;
+  int32_t ybuf;
+  int32_t t0, t1;
+} BfkSeg;
+
+/* One term: y += A[rowOff:rowOff+m, 0:n] x[xOff:xOff+n] with A stored
+ * column-major at aOff with leading dimension lda; aOff < 0 means an
+ * identity block (y += x[xOff:xOff+m]). */
+typedef struct BfkTerm {
+  int64_t aOff;
+  int64_t xOff;
+  int32_t n;
+  int32_t lda;
+  int32_t rowOff;
+  int32_t xbuf;
+} BfkTerm;
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,398 @@
+/*
+ * Host infrastructure of the path: error state, the seeded RNG, point
+ * sets, permutations and the quadtree.  C11.
+ */
+#define _DEFAULT_SOURCE 1
+#include <math.h>
+#include <time.h>
+
+#include "bf_host.h"
+
+double bfWallTime(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------ errors */
+
+/* error.c:7-24: one process-global code; reading it clears it */
+static enum BfError g_error = BF_ERROR_NONE;
+
+enum BfError bfGetError(void) {
+  enum BfError e = g_error;
+  g_error = BF_ERROR_NONE;
+  return e;
+}
+
+void bfSetError(enum BfError error) { g_error = error; }
+
+/* --------------------------------------------------------------- RNG */
+
+/* rand.c:19-76: splitmix64 expands the seed into the four words of a
+ * xoshiro256+ state; doubles take the top 53 bits of each draw; normals
+ * come in Box-Muller pairs (cos, sin) of consecutive uniform pairs. */
+static uint64_t g_sm;
+static uint64_t g_xs[4];
+
+static uint64_t sm_next(void) {
+  uint64_t z = (g_sm += UINT64_C(0x9e3779b97f4a7c15));
+  z = (z ^ (z >> 30)) * UINT64_C(0xbf58476d1ce4e5b9);
+  z = (z ^ (z >> 27)) * UINT64_C(0x94d049bb133111eb);
+  return z ^ (z >> 31);
+}
+
+static uint64_t xs_next(void) {
+  const uint64_t sum = g_xs[0] + g_xs[3];
+  const uint64_t shifted = g_xs[1] << 17;
+  g_xs[2] ^= g_xs[0];
+  g_xs[3] ^= g_xs[1];
+  g_xs[1] ^= g_xs[2];
+  g_xs[0] ^= g_xs[3];
+  g_xs[2] ^= shifted;
+  g_xs[3] = (g_xs[3] << 45) | (g_xs[3] >> 19);
+  return sum;
+}
+
+void bfSeed(BfSize seed) {
+  g_sm = (uint64_t)seed;
+  for (int i = 0; i < 4; ++i) g_xs[i] = sm_next();
+}
+
+BfReal bfRealUniform1(void) { return (BfReal)(xs_next() >> 11) * 0x1.0p-53; }
+
+void bfRealUniform(BfSize n, BfReal *x) {
+  for (BfSize i = 0; i < n; ++i) x[i] = bfRealUniform1();
+}
+
+void bfRealRandn(BfSize n, BfReal *x) {
+  const BfReal twoPi = 6.283185307179586;
+  BfSize pairs = n / 2;
+  bfRealUniform(2 * pairs, x);
+  for (BfSize i = 0; i < pairs; ++i) {
+    BfReal mag = sqrt(-2 * log(x[2 * i]));
+    BfReal ang = twoPi * x[2 * i + 1];
+    x[2 * i] = mag * cos(ang);
+    x[2 * i + 1] = mag * sin(ang);
+  }
+  if (n % 2 == 1) {
+    BfReal u[2];
+    bfRealUniform(2, u);
+    x[n - 1] = sqrt(-2 * log(u[0])) * cos(twoPi * u[1]);
+  }
+}
+
+void bfComplexRandn(BfSize n, BfComplex *x) { bfRealRandn(2 * n, (BfReal *)x); }
+
+void bfPoint2SampleUniformlyFromBoundingBox(BfReal const bboxMin[2], BfReal const bboxMax[2], BfPoint2 p) {
+  p[0] = (bboxMax[0] - bboxMin[0]) * bfRealUniform1() + bboxMin[0];
+  p[1] = (bboxMax[1] - bboxMin[1]) * bfRealUniform1() + bboxMin[1];
+}
+
+/* ------------------------------------------------------------ points */
+
+BfPoints2 *bfPoints2NewWithCapacity(BfSize capacity) {
+  BfPoints2 *p = calloc(1, sizeof(BfPoints2));
+  if (!p) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  p->capacity = capacity ? capacity : 1;
+  p->data = malloc(p->capacity * sizeof(BfPoint2));
+  if (!p->data) { free(p); bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  return p;
+}
+
+void bfPoints2Append(BfPoints2 *points, BfPoint2 const pt) {
+  if (points->isView) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return; }
+  if (points->size == points->capacity) {
+    BfSize nc = 2 * points->capacity;
+    void *nd = realloc(points->data, nc * sizeof(BfPoint2));
+    if (!nd) { bfSetError(BF_ERROR_MEMORY_ERROR); return; }
+    points->data = nd;
+    points->capacity = nc;
+  }
+  points->data[points->size][0] = pt[0];
+  points->data[points->size][1] = pt[1];
+  points->size++;
+}
+
+BfPoints2 *bfPoints2NewViewFromPtr(BfReal *xy, BfSize n) {
+  BfPoints2 *p = calloc(1, sizeof(BfPoints2));
+  if (!p) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  p->data = (BfPoint2 *)xy;
+  p->size = p->capacity = n;
+  p->isView = true;
+  return p;
+}
+
+void bfPoints2DeinitAndDealloc(BfPoints2 **points) {
+  if (!points || !*points) return;
+  if (!(*points)->isView) free((*points)->data);
+  free(*points);
+  *points = NULL;
+}
+
+/* -------------------------------------------------------------- perms */
+
+BfPerm *bfPermGetReversePerm(BfPerm const *perm) {
+  BfPerm *rev = calloc(1, sizeof(BfPerm));
+  if (!rev) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  rev->size = perm->size;
+  rev->index = malloc(perm->size * sizeof(BfSize));
+  if (!rev->index) { free(rev); bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  for (BfSize i = 0; i < perm->size; ++i) rev->index[i] = BF_SIZE_BAD_VALUE;
+  for (BfSize i = 0; i < perm->size; ++i) {
+    BfSize j = perm->index[i];
+    if (j >= perm->size || rev->index[j] != BF_SIZE_BAD_VALUE) {
+      free(rev->index);
+      free(rev);
+      bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+      return NULL;
+    }
+    rev->index[j] = i;
+  }
+  return rev;
+}
+
+void bfPermDelete(BfPerm **perm) {
+  if (!perm || !*perm) return;
+  if (!(*perm)->isView) free((*perm)->index);
+  free(*perm);
+  *perm = NULL;
+}
+
+/* ----------------------------------------------------------- quadtree */
+
+/* Child q of a node split at s holds the points with
+ * (x > sx) == (q >> 1) and (y > sy) == (q & 1): q = 0 lower-left,
+ * 1 upper-left, 2 lower-right, 3 upper-right; ties go low
+ * (quadtree_node.c:148-194). */
+static inline int quadrant_of(const BfReal *pt, const BfReal *split) {
+  return ((pt[0] > split[0]) << 1) | (pt[1] > split[1]);
+}
+
+typedef BF_VEC(BfQuadtreeNode) NodeVec;
+
+typedef struct {
+  NodeVec nodes;
+  BfPoint2 const *pts;
+  BfSize *perm;
+  BfSize leafSize;
+} TreeBuilder;
+
+/* Partition perm[lo:hi) by quadrant, visiting the index range in the same
+ * order as the reference's in-place sift so that the tree order of points
+ * inside every node matches it: for each of the first three quadrants,
+ * skip the leading run that already belongs to it, then swap every later
+ * member forward onto the boundary. */
+static void sift(TreeBuilder *b, BfSize lo, BfSize hi, const BfReal *split, BfSize off[5]) {
+  off[0] = lo;
+  off[4] = hi;
+  BfSize *perm = b->perm;
+  for (int q = 0; q < 3; ++q) {
+    BfSize bd = lo;
+    while (bd < hi && quadrant_of(b->pts[perm[bd]], split) == q) ++bd;
+    for (BfSize j = bd + 1; j < hi; ++j) {
+      if (quadrant_of(b->pts[perm[j]], split) == q) {
+        BfSize t = perm[bd];
+        perm[bd] = perm[j];
+        perm[j] = t;
+        ++bd;
+      }
+    }
+    off[q + 1] = bd;
+    lo = bd;
+  }
+}
+
+static void build_node(TreeBuilder *b, size_t ni) {
+  BfQuadtreeNode nd = b->nodes.a[ni];
+  BfSize off[5];
+  sift(b, nd.offset[0], nd.offset[4], nd.split, off);
+  memcpy(b->nodes.a[ni].offset, off, sizeof(off));
+  for (int q = 0; q < 4; ++q) {
+    BfSize cnt = off[q + 1] - off[q];
+    if (cnt == 0) continue;
+    BfQuadtreeNode c;
+    memset(&c, 0, sizeof(c));
+    const BfQuadtreeNode *p = &b->nodes.a[ni];
+    c.bboxMin[0] = (q >> 1) ? p->split[0] : p->bboxMin[0];
+    c.bboxMax[0] = (q >> 1) ? p->bboxMax[0] : p->split[0];
+    c.bboxMin[1] = (q & 1) ? p->split[1] : p->bboxMin[1];
+    c.bboxMax[1] = (q & 1) ? p->bboxMax[1] : p->split[1];
+    c.split[0] = (c.bboxMin[0] + c.bboxMax[0]) / 2;
+    c.split[1] = (c.bboxMin[1] + c.bboxMax[1]) / 2;
+    for (int r = 0; r < 4; ++r) c.offset[r] = off[q + 1];
+    c.offset[0] = off[q];
+    c.offset[4] = off[q + 1];
+    for (int r = 0; r < 4; ++r) c.child[r] = -1;
+    c.parent = (int32_t)ni;
+    c.depth = p->depth + 1;
+    c.index = q;
+    BF_VEC_PUSH(b->nodes, c);
+    size_t ci = b->nodes.n - 1;
+    b->nodes.a[ni].child[q] = (int32_t)ci;
+    if (cnt > b->leafSize) build_node(b, ci);
+  }
+}
+
+BfQuadtree *bfQuadtreeNew(void) {
+  BfQuadtree *t = calloc(1, sizeof(BfQuadtree));
+  if (!t) bfSetError(BF_ERROR_MEMORY_ERROR);
+  return t;
+}
+
+void bfQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points, BfVectors2 const *unitNormals,
+                                BfSize leafSize) {
+  memset(tree, 0, sizeof(*tree));
+  BfSize n = points ? points->size : 0;
+  if (n < 2 || leafSize < 1) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return;
+  }
+  tree->points = points;
+  tree->unitNormals = unitNormals;
+  tree->leafSize = leafSize;
+  TreeBuilder b;
+  memset(&b, 0, sizeof(b));
+  b.pts = (BfPoint2 const *)points->data;
+  b.leafSize = leafSize;
+  b.perm = malloc(n * sizeof(BfSize));
+  if (!b.perm) { bfSetError(BF_ERROR_MEMORY_ERROR); return; }
+  for (BfSize i = 0; i < n; ++i) b.perm[i] = i;
+
+  /* root box: bounding box stretched to a square about its centre
+   * (quadtree_node.c:284-306, bbox.c:18-30) */
+  BfReal mn[2] = {INFINITY, INFINITY}, mx[2] = {-INFINITY, -INFINITY};
+  for (BfSize i = 0; i < n; ++i)
+    for (int d = 0; d < 2; ++d) {
+      mn[d] = fmin(mn[d], b.pts[i][d]);
+      mx[d] = fmax(mx[d], b.pts[i][d]);
+    }
+  BfReal w = mx[0] - mn[0], h = mx[1] - mn[1];
+  if (w > h) {
+    BfReal c = (mn[1] + mx[1]) / 2;
+    mn[1] = w * (mn[1] - c) / h + c;
+    mx[1] = w * (mx[1] - c) / h + c;
+  } else {
+    BfReal c = (mn[0] + mx[0]) / 2;
+    mn[0] = h * (mn[0] - c) / w + c;
+    mx[0] = h * (mx[0] - c) / w + c;
+  }
+  BfQuadtreeNode root;
+  memset(&root, 0, sizeof(root));
+  root.bboxMin[0] = mn[0]; root.bboxMin[1] = mn[1];
+  root.bboxMax[0] = mx[0]; root.bboxMax[1] = mx[1];
+  root.split[0] = (mn[0] + mx[0]) / 2;
+  root.split[1] = (mn[1] + mx[1]) / 2;
+  root.offset[0] = 0;
+  root.offset[4] = n;
+  for (int r = 0; r < 4; ++r) root.child[r] = -1;
+  root.parent = -1;
+  root.index = -1;
+  BF_VEC_PUSH(b.nodes, root);
+  build_node(&b, 0);
+
+  /* renumber breadth-first, children in quadrant order (the reference's
+   * LR level order, tree_level_iter.c:10-43) */
+  size_t nn = b.nodes.n;
+  int32_t *order = malloc(nn * sizeof(int32_t)), *newId = malloc(nn * sizeof(int32_t));
+  BfQuadtreeNode *nodes = malloc(nn * sizeof(BfQuadtreeNode));
+  if (!order || !newId || !nodes) { bfSetError(BF_ERROR_MEMORY_ERROR); return; }
+  size_t head = 0, tail = 0;
+  order[tail++] = 0;
+  while (head < tail) {
+    int32_t v = order[head++];
+    for (int q = 0; q < 4; ++q)
+      if (b.nodes.a[v].child[q] >= 0) order[tail++] = b.nodes.a[v].child[q];
+  }
+  for (size_t i = 0; i < nn; ++i) newId[order[i]] = (int32_t)i;
+  int32_t maxDepth = 0;
+  for (size_t i = 0; i < nn; ++i) {
+    nodes[i] = b.nodes.a[order[i]];
+    if (nodes[i].parent >= 0) nodes[i].parent = newId[nodes[i].parent];
+    for (int q = 0; q < 4; ++q)
+      if (nodes[i].child[q] >= 0) nodes[i].child[q] = newId[nodes[i].child[q]];
+    if (nodes[i].depth > maxDepth) maxDepth = nodes[i].depth;
+  }
+  for (size_t i = nn; i-- > 0;) {
+    int32_t d = nodes[i].depth;
+    for (int q = 0; q < 4; ++q)
+      if (nodes[i].child[q] >= 0 && nodes[nodes[i].child[q]].maxDepthBelow > d)
+        d = nodes[nodes[i].child[q]].maxDepthBelow;
+    nodes[i].maxDepthBelow = d;
+  }
+  tree->levelOffset = calloc((size_t)maxDepth + 2, sizeof(BfSize));
+  for (size_t i = 0; i < nn; ++i) tree->levelOffset[nodes[i].depth + 1] = i + 1;
+  tree->numLevels = (BfSize)maxDepth + 1;
+  free(order);
+  free(newId);
+  BF_VEC_FREE(b.nodes);
+  tree->nodes = nodes;
+  tree->numNodes = nn;
+  tree->super.root = nodes;
+  tree->super.perm = calloc(1, sizeof(BfPerm));
+  tree->super.perm->index = b.perm;
+  tree->super.perm->size = n;
+}
+
+void bfQuadtreeInit(BfQuadtree *tree, BfPoints2 const *points, BfVectors2 const *unitNormals) {
+  /* LEAF_SIZE_THRESHOLD = 1 (quadtree_node.c:17) */
+  bfQuadtreeInitWithLeafSize(tree, points, unitNormals, 1);
+}
+
+void bfQuadtreeDeinit(BfQuadtree *tree) {
+  if (!tree) return;
+  free(tree->nodes);
+  free(tree->levelOffset);
+  if (tree->super.perm) {
+    free(tree->super.perm->index);
+    free(tree->super.perm);
+  }
+  memset(tree, 0, sizeof(*tree));
+}
+
+void bfQuadtreeDealloc(BfQuadtree **tree) {
+  if (!tree) return;
+  free(*tree);
+  *tree = NULL;
+}
+
+void bfQuadtreeDeinitAndDealloc(BfQuadtree **tree) {
+  if (!tree || !*tree) return;
+  bfQuadtreeDeinit(*tree);
+  bfQuadtreeDealloc(tree);
+}
+
+BfTree *bfQuadtreeToTree(BfQuadtree *tree) { return &tree->super; }
+
+BfPerm const *bfTreeGetPermConst(BfTree const *tree) { return tree->perm; }
+
+BfCircle bfQuadtreeNodeGetBoundingCircle(BfQuadtreeNode const *node) {
+  BfCircle c;
+  c.r = hypot(node->bboxMax[0] - node->bboxMin[0], node->bboxMax[1] - node->bboxMin[1]) / 2;
+  c.center[0] = (node->bboxMin[0] + node->bboxMax[0]) / 2;
+  c.center[1] = (node->bboxMin[1] + node->bboxMax[1]) / 2;
+  return c;
+}
+
+bool bfQuadtreeNodesAreSeparated(BfQuadtreeNode const *node1, BfQuadtreeNode const *node2) {
+  BfCircle c1 = bfQuadtreeNodeGetBoundingCircle(node1);
+  BfCircle c2 = bfQuadtreeNodeGetBoundingCircle(node2);
+  BfReal R = hypot(c2.center[0] - c1.center[0], c2.center[1] - c1.center[1]);
+  return R > c1.r + c2.r + 1e1 * BF_EPS_MACH;
+}
+
+BfSize bfQuadtreeNodeGetNumPoints(BfQuadtreeNode const *node) { return node->offset[4] - node->offset[0]; }
+
+/* helm2.c:14-36 */
+BfSize bfHelm2RankEstForTwoCircles(BfHelm2 const *helm, BfCircle const *circ1, BfCircle const *circ2, BfReal C,
+                                   BfReal eps) {
+  BfReal r1 = circ1->r, r2 = circ2->r;
+  BfReal R = hypot(circ2->center[0] - circ1->center[0], circ2->center[1] - circ1->center[1]);
+  BfReal d = R - r1 - r2;
+  if (!(helm->k > 0) || !(C > 0) || !(eps > 0) || !(d > 0)) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return BF_SIZE_BAD_VALUE;
+  }
+  BfReal p = helm->k * r1 * r2 / d - C * log10(eps);
+  return (BfSize)ceil(p);
+}

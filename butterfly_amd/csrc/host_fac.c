@@ -1,0 +1,644 @@
+/*
+ * The device factorization object: builds the plan, runs the sampling and
+ * compression kernels into one HBM arena, lays out the apply stages and
+ * applies.  Replaces bfFacHelm2MakeMultilevel (fac_helm2.c:943-1002) and,
+ * for its result, bfMatMul (mat.c:183-185).
+ */
+#define _DEFAULT_SOURCE 1
+#include <math.h>
+
+#include "bf_host.h"
+
+static BfType hipbf_get_type(BfMat const *m) {
+  (void)m;
+  return BF_TYPE_MAT_HIP_BUTTERFLY;
+}
+
+BfMatVtable bfMatHipButterflyVtable = {
+    .GetType = hipbf_get_type,
+    .NumBytes = bfMatHipButterflyNumBytes,
+    .Delete = bfMatHipButterflyDelete,
+    .Copy = NULL,
+    .Mul = bfMatHipButterflyMul,
+    .PermuteRows = NULL,
+};
+
+void bfFacHelm2OptionsInitDefault(BfFacHelm2Options *opts) {
+  opts->tol = 1e-15;
+  opts->rankConst = 1;
+  opts->maxDenseSize = 128 * 128;
+  opts->passThrough = false;
+}
+
+static void free_plan(BfMatHipButterfly *F) {
+  BF_VEC_FREE(F->leaves);
+  BF_VEC_FREE(F->factors);
+  BF_VEC_FREE(F->blocks);
+  BF_VEC_FREE(F->offs);
+  BF_VEC_FREE(F->lstsqJobs);
+  BF_VEC_FREE(F->kernelJobs);
+  BF_VEC_FREE(F->segs);
+  BF_VEC_FREE(F->terms);
+  BF_VEC_FREE(F->stageSeg);
+}
+
+static void free_device(BfMatHipButterfly *F) {
+  bfk_free(F->dArena);
+  bfk_free(F->dArena32);
+  bfk_free(F->dArena32lo);
+  bfk_free(F->dSrcXY);
+  bfk_free(F->dTgtXY);
+  bfk_free(F->dSegs);
+  bfk_free(F->dTerms);
+  bfk_free(F->dPing);
+  bfk_free(F->dPong);
+  F->dArena = NULL;
+  F->dArena32 = F->dArena32lo = NULL;
+  F->dSrcXY = F->dTgtXY = NULL;
+  F->dSegs = NULL;
+  F->dTerms = NULL;
+  F->dPing = F->dPong = NULL;
+}
+
+void bfMatHipButterflyDelete(BfMat **mat) {
+  if (!mat || !*mat) return;
+  BfMatHipButterfly *F = (BfMatHipButterfly *)*mat;
+  free_device(F);
+  free_plan(F);
+  free(F);
+  *mat = NULL;
+}
+
+BfSize bfMatHipButterflyNumBytes(BfMat const *mat) {
+  BfMatHipButterfly const *F = (BfMatHipButterfly const *)mat;
+  return (BfSize)F->arenaEntries * 16;
+}
+
+static int64_t *offs_at(BfMatHipButterfly *F, int64_t start) { return &F->offs.a[start]; }
+
+typedef struct {
+  int64_t start, len;
+  int64_t blk;   /* block index, or -1 - leafIdx for a dense leaf */
+  int64_t xOff;
+  int32_t xbuf;
+} Contrib;
+
+static int cmp_i64(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+static int32_t stage_buf(int s) { return (s % 2 == 0) ? BFK_BUF_PING : BFK_BUF_PONG; }
+
+/* Stage layout: butterfly b with nf factors runs factor f at stage
+ * Fmax - nf + f, so every last factor and every dense leaf lands in the
+ * final stage, which writes Y once per row segment (no atomics, a fixed
+ * summation order).  Earlier stages write ping (even) or pong (odd). */
+int bfApplyPlanBuild(BfMatHipButterfly *F) {
+  int Fmax = 1;
+  for (size_t l = 0; l < F->leaves.n; ++l)
+    if (F->leaves.a[l].isButterfly && F->leaves.a[l].numFactors > Fmax) Fmax = F->leaves.a[l].numFactors;
+  F->numStages = Fmax;
+  int64_t *fill = calloc(Fmax, sizeof(int64_t));
+  /* region of every non-last factor */
+  int64_t *region = malloc((F->factors.n + 1) * sizeof(int64_t));
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    BfHipLeaf *lf = &F->leaves.a[l];
+    if (!lf->isButterfly) continue;
+    for (int f = 0; f < lf->numFactors - 1; ++f) {
+      int s = Fmax - lf->numFactors + f;
+      BfHipFactor *fa = &F->factors.a[lf->firstFactor + f];
+      region[lf->firstFactor + f] = fill[s];
+      fill[s] += offs_at(F, fa->rowOffStart)[fa->numBlockRows];
+    }
+  }
+  F->bufRows[0] = F->bufRows[1] = 0;
+  for (int s = 0; s < Fmax; ++s) {
+    int b = s % 2;
+    if (fill[s] > F->bufRows[b]) F->bufRows[b] = fill[s];
+  }
+  double facBytes = 0, vecBytes = 0;
+  F->segs.n = F->terms.n = F->stageSeg.n = 0;
+  for (int s = 0; s < Fmax - 1; ++s) {
+    BF_VEC_PUSH(F->stageSeg, (int64_t)F->segs.n);
+    for (size_t l = 0; l < F->leaves.n; ++l) {
+      BfHipLeaf *lf = &F->leaves.a[l];
+      if (!lf->isButterfly) continue;
+      int f = s - (Fmax - lf->numFactors);
+      if (f < 0 || f >= lf->numFactors - 1) continue;
+      int64_t fi = lf->firstFactor + f;
+      BfHipFactor *fa = &F->factors.a[fi];
+      int64_t *ro = offs_at(F, fa->rowOffStart), *co = offs_at(F, fa->colOffStart);
+      vecBytes += 16.0 * (double)(ro[fa->numBlockRows] + co[fa->numBlockCols]);
+      int64_t b = fa->firstBlock, bend = fa->firstBlock + fa->numBlocks;
+      while (b < bend) {
+        int32_t row = F->blocks.a[b].rowInd;
+        BfkSeg sg;
+        sg.yOff = region[fi] + ro[row];
+        sg.m = (int32_t)(ro[row + 1] - ro[row]);
+        sg.ybuf = stage_buf(s);
+        sg.t0 = (int32_t)F->terms.n;
+        for (; b < bend && F->blocks.a[b].rowInd == row; ++b) {
+          BfHipBlock *bl = &F->blocks.a[b];
+          BfkTerm t;
+          t.aOff = bl->dataOff;
+          t.n = bl->n;
+          t.lda = bl->m;
+          t.rowOff = 0;
+          if (f == 0) {
+            t.xOff = lf->col0 + co[bl->colInd];
+            t.xbuf = BFK_BUF_X;
+          } else {
+            t.xOff = region[fi - 1] + co[bl->colInd];
+            t.xbuf = stage_buf(s - 1);
+          }
+          if (bl->dataOff >= 0) facBytes += 16.0 * bl->m * bl->n;
+          BF_VEC_PUSH(F->terms, t);
+        }
+        sg.t1 = (int32_t)F->terms.n;
+        BF_VEC_PUSH(F->segs, sg);
+      }
+    }
+  }
+  /* final stage */
+  BF_VEC_PUSH(F->stageSeg, (int64_t)F->segs.n);
+  BF_VEC(Contrib) cons;
+  memset(&cons, 0, sizeof(cons));
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    BfHipLeaf *lf = &F->leaves.a[l];
+    if (!lf->isButterfly) {
+      Contrib c = {lf->row0, lf->m, -1 - (int64_t)l, lf->col0, BFK_BUF_X};
+      BF_VEC_PUSH(cons, c);
+      facBytes += 16.0 * lf->m * lf->n;
+      vecBytes += 16.0 * (lf->m + lf->n);
+      continue;
+    }
+    int64_t fi = lf->firstFactor + lf->numFactors - 1;
+    BfHipFactor *fa = &F->factors.a[fi];
+    int64_t *ro = offs_at(F, fa->rowOffStart), *co = offs_at(F, fa->colOffStart);
+    vecBytes += 16.0 * (double)(ro[fa->numBlockRows] + co[fa->numBlockCols]);
+    for (int64_t b = fa->firstBlock; b < fa->firstBlock + fa->numBlocks; ++b) {
+      BfHipBlock *bl = &F->blocks.a[b];
+      Contrib c = {lf->row0 + ro[bl->rowInd], bl->m, b, region[fi - 1] + co[bl->colInd], stage_buf(Fmax - 2)};
+      BF_VEC_PUSH(cons, c);
+      if (bl->dataOff >= 0) facBytes += 16.0 * bl->m * bl->n;
+    }
+  }
+  int64_t *bnd = malloc((2 * cons.n + 2) * sizeof(int64_t));
+  size_t nb = 0;
+  bnd[nb++] = 0;
+  bnd[nb++] = (int64_t)F->numTgt;
+  for (size_t i = 0; i < cons.n; ++i) {
+    bnd[nb++] = cons.a[i].start;
+    bnd[nb++] = cons.a[i].start + cons.a[i].len;
+  }
+  qsort(bnd, nb, sizeof(int64_t), cmp_i64);
+  size_t nu = 0;
+  for (size_t i = 0; i < nb; ++i)
+    if (nu == 0 || bnd[i] != bnd[nu - 1]) bnd[nu++] = bnd[i];
+  size_t numSeg = nu - 1;
+  int64_t *cnt = calloc(numSeg + 1, sizeof(int64_t));
+  for (size_t i = 0; i < cons.n; ++i) {
+    int64_t a = cons.a[i].start, e = a + cons.a[i].len;
+    size_t lo = 0, hi = numSeg; /* first boundary == a */
+    while (lo < hi) { size_t mid = (lo + hi) / 2; if (bnd[mid] < a) lo = mid + 1; else hi = mid; }
+    for (size_t sgi = lo; sgi < numSeg && bnd[sgi] < e; ++sgi) cnt[sgi + 1]++;
+  }
+  for (size_t i = 0; i < numSeg; ++i) cnt[i + 1] += cnt[i];
+  size_t termBase = F->terms.n;
+  size_t totalTerms = termBase + (size_t)cnt[numSeg];
+  if (F->terms.cap < totalTerms) {
+    void *na = realloc(F->terms.a, totalTerms * sizeof(BfkTerm));
+    if (!na) abort();
+    F->terms.a = na;
+    F->terms.cap = totalTerms;
+  }
+  F->terms.n = totalTerms;
+  int64_t *pos = malloc(numSeg * sizeof(int64_t));
+  for (size_t i = 0; i < numSeg; ++i) pos[i] = cnt[i];
+  for (size_t i = 0; i < cons.n; ++i) {
+    Contrib *c = &cons.a[i];
+    int64_t a = c->start, e = a + c->len;
+    size_t lo = 0, hi = numSeg;
+    while (lo < hi) { size_t mid = (lo + hi) / 2; if (bnd[mid] < a) lo = mid + 1; else hi = mid; }
+    for (size_t sgi = lo; sgi < numSeg && bnd[sgi] < e; ++sgi) {
+      int64_t rOff = bnd[sgi] - a;
+      BfkTerm t;
+      if (c->blk < 0) {
+        BfHipLeaf *lf = &F->leaves.a[-1 - c->blk];
+        t.aOff = lf->denseOff;
+        t.n = (int32_t)lf->n;
+        t.lda = (int32_t)lf->m;
+        t.rowOff = (int32_t)rOff;
+        t.xOff = c->xOff;
+      } else {
+        BfHipBlock *bl = &F->blocks.a[c->blk];
+        t.aOff = bl->dataOff;
+        t.n = bl->n;
+        t.lda = bl->m;
+        t.rowOff = (int32_t)rOff;
+        t.xOff = bl->dataOff >= 0 ? c->xOff : c->xOff + rOff;
+      }
+      t.xbuf = c->xbuf;
+      F->terms.a[termBase + pos[sgi]++] = t;
+    }
+  }
+  for (size_t i = 0; i < numSeg; ++i) {
+    BfkSeg sg;
+    sg.yOff = bnd[i];
+    sg.m = (int32_t)(bnd[i + 1] - bnd[i]);
+    sg.ybuf = BFK_BUF_Y;
+    sg.t0 = (int32_t)(termBase + cnt[i]);
+    sg.t1 = (int32_t)(termBase + cnt[i + 1]);
+    BF_VEC_PUSH(F->segs, sg);
+  }
+  BF_VEC_PUSH(F->stageSeg, (int64_t)F->segs.n);
+  F->applyFactorBytes = facBytes;
+  F->stats[11] = facBytes;
+  F->stats[12] = vecBytes;
+  free(pos);
+  free(cnt);
+  free(bnd);
+  BF_VEC_FREE(cons);
+  free(fill);
+  free(region);
+  if (F->terms.n > (size_t)INT32_MAX) return -1;
+  return 0;
+}
+
+static int upload(void **dst, const void *src, size_t bytes) {
+  *dst = bfk_malloc(bytes);
+  if (!*dst) return -2;
+  if (bytes && bfk_h2d(*dst, src, bytes)) return -1;
+  return 0;
+}
+
+static double *tree_coords(BfQuadtree const *t) {
+  BfSize n = t->points->size;
+  double *xy = malloc(2 * n * sizeof(double));
+  for (BfSize i = 0; i < n; ++i) {
+    BfSize j = t->super.perm->index[i];
+    xy[2 * i] = t->points->data[j][0];
+    xy[2 * i + 1] = t->points->data[j][1];
+  }
+  return xy;
+}
+
+typedef struct { size_t bytes; int32_t idx; } JobKey;
+static int cmp_jobkey(const void *a, const void *b) {
+  const JobKey *x = a, *y = b;
+  if (x->bytes != y->bytes) return x->bytes < y->bytes ? -1 : 1;
+  return x->idx < y->idx ? -1 : x->idx > y->idx;
+}
+
+static int run_factor_kernels(BfMatHipButterfly *F) {
+  int rc;
+  size_t arenaBytes = (size_t)F->arenaEntries * 16;
+  F->dArena = bfk_malloc(arenaBytes);
+  if (!F->dArena) return -2;
+  void *dK = NULL, *dL = NULL, *dOrder = NULL;
+  if ((rc = upload(&dK, F->kernelJobs.a, F->kernelJobs.n * sizeof(BfkKernelJob)))) return rc;
+  rc = bfk_sample(dK, (int64_t)F->kernelJobs.n, F->dSrcXY, F->dTgtXY, F->helm.k, F->dArena);
+  bfk_free(dK);
+  if (rc) return rc;
+  /* re-expansion jobs grouped by the LDS they need (sizes rounded up to
+   * 4 KiB so a launch serves many jobs) */
+  size_t nj = F->lstsqJobs.n;
+  JobKey *keys = malloc((nj ? nj : 1) * sizeof(JobKey));
+  for (size_t i = 0; i < nj; ++i) {
+    size_t b = bfk_lstsq_lds_bytes(F->lstsqJobs.a[i].p, F->lstsqJobs.a[i].n);
+    keys[i].bytes = (b + 4095) & ~(size_t)4095;
+    keys[i].idx = (int32_t)i;
+  }
+  qsort(keys, nj, sizeof(JobKey), cmp_jobkey);
+  int32_t *order = malloc((nj ? nj : 1) * sizeof(int32_t));
+  BF_VEC(int64_t) gs;
+  BF_VEC(size_t) gb;
+  memset(&gs, 0, sizeof(gs));
+  memset(&gb, 0, sizeof(gb));
+  for (size_t i = 0; i < nj; ++i) {
+    order[i] = keys[i].idx;
+    if (i == 0 || keys[i].bytes != keys[i - 1].bytes) {
+      BF_VEC_PUSH(gs, (int64_t)i);
+      BF_VEC_PUSH(gb, keys[i].bytes);
+    }
+  }
+  BF_VEC_PUSH(gs, (int64_t)nj);
+  free(keys);
+  rc = upload(&dL, F->lstsqJobs.a, nj * sizeof(BfkLstsqJob));
+  if (!rc) rc = upload(&dOrder, order, nj * sizeof(int32_t));
+  if (!rc && nj)
+    rc = bfk_lstsq(dL, dOrder, (int)gb.n, gs.a, gb.a, F->dSrcXY, F->helm.k, F->dArena);
+  bfk_free(dL);
+  bfk_free(dOrder);
+  free(order);
+  BF_VEC_FREE(gs);
+  BF_VEC_FREE(gb);
+  return rc;
+}
+
+BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
+                                           BfFacHelm2Options const *opts) {
+  if (!helm || !srcTree || !tgtTree || !opts || !(helm->k > 0) || !srcTree->nodes || !tgtTree->nodes) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE) {
+    bfSetError(BF_ERROR_NOT_IMPLEMENTED);
+    return NULL;
+  }
+  if (bfk_init()) {
+    bfSetError(BF_ERROR_RUNTIME_ERROR);
+    return NULL;
+  }
+  double t0 = bfWallTime();
+  BfMatHipButterfly *F = calloc(1, sizeof(BfMatHipButterfly));
+  if (!F) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  F->helm = *helm;
+  F->opts = *opts;
+  F->numSrc = srcTree->points->size;
+  F->numTgt = tgtTree->points->size;
+  bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
+  if (bfPlanBuild(F, srcTree, tgtTree)) {
+    bfMatHipButterflyDelete((BfMat **)&F);
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  double t1 = bfWallTime();
+  int rc = 0;
+  double *sxy = tree_coords(srcTree), *txy = tree_coords(tgtTree);
+  rc = upload((void **)&F->dSrcXY, sxy, 2 * F->numSrc * sizeof(double));
+  if (!rc) rc = upload((void **)&F->dTgtXY, txy, 2 * F->numTgt * sizeof(double));
+  free(sxy);
+  free(txy);
+  if (!rc) rc = run_factor_kernels(F);
+  if (!rc) rc = bfk_sync();
+  double t2 = bfWallTime();
+  if (!rc) rc = bfApplyPlanBuild(F) ? -1 : 0;
+  if (!rc) rc = upload((void **)&F->dSegs, F->segs.a, F->segs.n * sizeof(BfkSeg));
+  if (!rc) rc = upload((void **)&F->dTerms, F->terms.a, F->terms.n * sizeof(BfkTerm));
+  if (rc) {
+    bfMatHipButterflyDelete((BfMat **)&F);
+    bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : BF_ERROR_RUNTIME_ERROR);
+    return NULL;
+  }
+  double t3 = bfWallTime();
+  double ktimes[8];
+  bfk_get_times(ktimes);
+  int64_t nL = 0, nK = 0, nI = 0, nD = 0, nB = 0;
+  double h0 = 0;
+  for (size_t b = 0; b < F->blocks.n; ++b) {
+    int kd = F->blocks.a[b].kind;
+    nL += kd == BF_BLOCK_LSTSQ;
+    nK += kd == BF_BLOCK_KERNEL;
+    nI += kd == BF_BLOCK_IDENTITY;
+  }
+  for (size_t i = 0; i < F->lstsqJobs.n; ++i)
+    h0 += (double)F->lstsqJobs.a[i].p * (F->lstsqJobs.a[i].p + F->lstsqJobs.a[i].n);
+  for (size_t i = 0; i < F->kernelJobs.n; ++i) h0 += (double)F->kernelJobs.a[i].m * F->kernelJobs.a[i].n;
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    nB += F->leaves.a[l].isButterfly;
+    nD += !F->leaves.a[l].isButterfly;
+  }
+  F->stats[0] = t3 - t0;
+  F->stats[1] = t1 - t0;
+  F->stats[2] = t2 - t1;
+  F->stats[3] = (double)F->arenaEntries;
+  F->stats[4] = (double)nL;
+  F->stats[5] = (double)nK;
+  F->stats[6] = (double)nI;
+  F->stats[7] = (double)nD;
+  F->stats[8] = (double)nB;
+  F->stats[9] = (double)F->numStages;
+  F->stats[10] = h0;
+  F->stats[13] = ktimes[0];
+  F->stats[14] = ktimes[1];
+  F->stats[15] = t3 - t2;
+  return &F->super;
+}
+
+BfMat *bfhipFacHelm2Plan(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
+                         BfFacHelm2Options const *opts) {
+  if (!helm || !srcTree || !tgtTree || !opts || !(helm->k > 0) || !srcTree->nodes || !tgtTree->nodes) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  BfMatHipButterfly *F = calloc(1, sizeof(BfMatHipButterfly));
+  if (!F) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  F->helm = *helm;
+  F->opts = *opts;
+  F->numSrc = srcTree->points->size;
+  F->numTgt = tgtTree->points->size;
+  bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
+  double t0 = bfWallTime();
+  if (bfPlanBuild(F, srcTree, tgtTree) || bfApplyPlanBuild(F)) {
+    bfMatHipButterflyDelete((BfMat **)&F);
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  F->stats[1] = bfWallTime() - t0;
+  F->stats[3] = (double)F->arenaEntries;
+  F->stats[9] = (double)F->numStages;
+  return &F->super;
+}
+
+BfMat *bfFacHelm2MakeMultilevel(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree) {
+  BfFacHelm2Options opts;
+  bfFacHelm2OptionsInitDefault(&opts);
+  return bfFacHelm2MakeMultilevelWithOptions(helm, srcTree, tgtTree, &opts);
+}
+
+static int ensure_buffers(BfMatHipButterfly *F, size_t nrhs, size_t elemBytes) {
+  size_t need = (size_t)(F->bufRows[0] > F->bufRows[1] ? F->bufRows[0] : F->bufRows[1]) * nrhs * elemBytes;
+  if (need == 0) need = 16;
+  if (need <= F->bufBytes) return 0;
+  bfk_free(F->dPing);
+  bfk_free(F->dPong);
+  F->dPing = bfk_malloc(need);
+  F->dPong = bfk_malloc(need);
+  if (!F->dPing || !F->dPong) {
+    F->bufBytes = 0;
+    return -2;
+  }
+  F->bufBytes = need;
+  return 0;
+}
+
+int bfhipMatMulDevice(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable || nrhs == 0) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  int rc;
+  if (!F->dArena || !F->dSegs) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  if (prec == BFHIP_PRECISION_FP64) {
+    if ((rc = ensure_buffers(F, nrhs, 16))) { bfSetError(BF_ERROR_MEMORY_ERROR); return rc; }
+    rc = bfk_apply(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena, dX, F->dPing, F->dPong, dY,
+                   (int)nrhs);
+  } else {
+    if (!F->dArena32 || (prec == BFHIP_PRECISION_FP32X2 && !F->dArena32lo)) {
+      bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+      return -1;
+    }
+    if ((rc = ensure_buffers(F, nrhs, 8))) { bfSetError(BF_ERROR_MEMORY_ERROR); return rc; }
+    rc = bfk_apply_f32(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena32,
+                       prec == BFHIP_PRECISION_FP32X2 ? F->dArena32lo : NULL, dX, F->dPing, F->dPong, dY,
+                       (int)nrhs);
+  }
+  if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
+  return rc;
+}
+
+int bfhipFacPrepareMixed(BfMat *fac, BfhipPrecision prec) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (prec == BFHIP_PRECISION_FP64) return 0;
+  size_t bytes = (size_t)F->arenaEntries * 8;
+  if (!F->dArena32) {
+    F->dArena32 = bfk_malloc(bytes);
+    if (!F->dArena32) { bfSetError(BF_ERROR_MEMORY_ERROR); return -2; }
+  }
+  if (prec == BFHIP_PRECISION_FP32X2 && !F->dArena32lo) {
+    F->dArena32lo = bfk_malloc(bytes);
+    if (!F->dArena32lo) { bfSetError(BF_ERROR_MEMORY_ERROR); return -2; }
+  }
+  int rc = bfk_to_f32(F->dArena, F->arenaEntries, F->dArena32, F->dArena32lo);
+  if (!rc) rc = bfk_sync();
+  if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
+  return rc;
+}
+
+BfMat *bfMatHipButterflyMul(BfMat const *A, BfMat const *Xm) {
+  BfMatHipButterfly *F = (BfMatHipButterfly *)A;
+  if (!Xm || bfMatGetType(Xm) != BF_TYPE_MAT_DENSE_COMPLEX || Xm->numRows != F->numSrc) {
+    bfSetError(BF_ERROR_INCOMPATIBLE_SHAPES);
+    return NULL;
+  }
+  BfMatDenseComplex const *X = (BfMatDenseComplex const *)Xm;
+  BfSize nrhs = Xm->numCols;
+  BfComplex *packed = malloc(F->numSrc * nrhs * sizeof(BfComplex));
+  if (!packed) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  for (BfSize i = 0; i < F->numSrc; ++i)
+    for (BfSize j = 0; j < nrhs; ++j) packed[i * nrhs + j] = X->data[i * X->rowStride + j * X->colStride];
+  void *dX = bfk_malloc(F->numSrc * nrhs * 16), *dY = bfk_malloc(F->numTgt * nrhs * 16);
+  BfMatDenseComplex *Y = NULL;
+  int rc = (!dX || !dY) ? -2 : 0;
+  if (!rc) rc = bfk_h2d(dX, packed, F->numSrc * nrhs * 16);
+  if (!rc) rc = bfhipMatMulDevice(A, dX, dY, nrhs, BFHIP_PRECISION_FP64);
+  if (!rc) {
+    Y = bfMatDenseComplexNew();
+    bfMatDenseComplexInit(Y, F->numTgt, nrhs);
+    rc = bfk_d2h(Y->data, dY, F->numTgt * nrhs * 16);
+  }
+  bfk_free(dX);
+  bfk_free(dY);
+  free(packed);
+  if (rc) {
+    if (Y) bfMatDenseComplexDeinitAndDealloc(&Y);
+    bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : BF_ERROR_RUNTIME_ERROR);
+    return NULL;
+  }
+  return &Y->super;
+}
+
+BfSize bfMatProductNumFactors(BfMat const *fac, BfSize leaf) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
+    bfSetError(BF_ERROR_TYPE_ERROR);
+    return BF_SIZE_BAD_VALUE;
+  }
+  BfMatHipButterfly const *F = (BfMatHipButterfly const *)fac;
+  if (leaf >= F->leaves.n) {
+    bfSetError(BF_ERROR_OUT_OF_RANGE);
+    return BF_SIZE_BAD_VALUE;
+  }
+  return F->leaves.a[leaf].isButterfly ? (BfSize)F->leaves.a[leaf].numFactors : 0;
+}
+
+void bfhipFacGetStats(BfMat const *fac, double out[16]) {
+  BfMatHipButterfly const *F = (BfMatHipButterfly const *)fac;
+  memcpy(out, F->stats, 16 * sizeof(double));
+}
+
+void bfhipGetKernelTimes(double out[8]) { bfk_get_times(out); }
+
+void bfhipGetKernelTimeSums(double sums[8], double counts[8]) { bfk_get_time_sums(sums, counts); }
+
+void bfhipResetKernelTimes(void) { bfk_reset_times(); }
+
+void bfhipFacPlanSizes(BfMat const *fac, BfSize *numLeaves, BfSize *numFactors, BfSize *numBlocks) {
+  BfMatHipButterfly const *F = (BfMatHipButterfly const *)fac;
+  *numLeaves = F->leaves.n;
+  *numFactors = F->factors.n;
+  *numBlocks = F->blocks.n;
+}
+
+void bfhipFacPlanExport(BfMat const *fac, int64_t *leafTab, int64_t *facTab, int64_t *blkTab) {
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    BfHipLeaf *lf = &F->leaves.a[l];
+    int64_t *L = &leafTab[6 * l];
+    L[0] = lf->isButterfly;
+    L[1] = lf->row0;
+    L[2] = lf->col0;
+    L[3] = lf->m;
+    L[4] = lf->n;
+    L[5] = lf->isButterfly ? lf->numFactors : 0;
+  }
+  for (size_t f = 0; f < F->factors.n; ++f) {
+    BfHipFactor *fa = &F->factors.a[f];
+    int64_t *R = &facTab[4 * f];
+    R[0] = fa->isDiag;
+    R[1] = fa->numBlockRows;
+    R[2] = fa->numBlockCols;
+    R[3] = fa->numBlocks;
+    int64_t *ro = offs_at(F, fa->rowOffStart), *co = offs_at(F, fa->colOffStart);
+    for (int64_t b = fa->firstBlock; b < fa->firstBlock + fa->numBlocks; ++b) {
+      BfHipBlock *bl = &F->blocks.a[b];
+      int64_t *B = &blkTab[7 * b];
+      B[0] = bl->rowInd;
+      B[1] = bl->colInd;
+      B[2] = bl->m;
+      B[3] = bl->n;
+      B[4] = bl->kind;
+      B[5] = ro[bl->rowInd];
+      B[6] = co[bl->colInd];
+    }
+  }
+}
+
+static int copy_colmajor_to_rowmajor(int64_t off, int64_t m, int64_t n, const double *dArena, BfComplex *out) {
+  BfComplex *tmp = malloc((size_t)(m * n) * sizeof(BfComplex));
+  if (!tmp) return -2;
+  int rc = bfk_d2h(tmp, dArena + 2 * off, (size_t)(m * n) * 16);
+  if (!rc)
+    for (int64_t i = 0; i < m; ++i)
+      for (int64_t j = 0; j < n; ++j) out[i * n + j] = tmp[j * m + i];
+  free(tmp);
+  return rc;
+}
+
+int bfhipFacBlockData(BfMat const *fac, BfSize idx, BfComplex *out) {
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (idx >= F->blocks.n) return -1;
+  BfHipBlock *b = &F->blocks.a[idx];
+  if (b->dataOff < 0) {
+    for (int64_t i = 0; i < b->m; ++i)
+      for (int64_t j = 0; j < b->n; ++j) out[i * b->n + j] = (i == j);
+    return 0;
+  }
+  return copy_colmajor_to_rowmajor(b->dataOff, b->m, b->n, F->dArena, out);
+}
+
+int bfhipFacDenseData(BfMat const *fac, BfSize leafIdx, BfComplex *out) {
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (leafIdx >= F->leaves.n || F->leaves.a[leafIdx].isButterfly) return -1;
+  BfHipLeaf *l = &F->leaves.a[leafIdx];
+  return copy_colmajor_to_rowmajor(l->denseOff, l->m, l->n, F->dArena, out);
+}

@@ -1,0 +1,474 @@
+/*
+ * Planner of the multilevel butterfly (fac_helm2.c:42-1002), producing
+ * flat tables: top-level leaves (dense blocks or butterflies), factors,
+ * blocks, and the sampling / re-expansion jobs the GPU runs.  The plan is
+ * integer-exact with the reference's decisions in its default mode; the
+ * pass-through mode adds the config-driven planner.
+ */
+#define _DEFAULT_SOURCE 1
+#include <math.h>
+
+#include "bf_host.h"
+
+enum { MODE_PROXY = 0, MODE_CHARGES = 1, MODE_VALUES = 2 };
+
+typedef struct {
+  BfQuadtree const *t;
+  double *cx, *cy, *cr; /* bounding circles per node */
+} TreeView;
+
+typedef struct {
+  BfMatHipButterfly *F;
+  TreeView src, tgt;
+  double k, tol, C;
+} Planner;
+
+typedef BF_VEC(int32_t) Level;
+
+static void tree_view(TreeView *v, BfQuadtree const *t) {
+  v->t = t;
+  v->cx = malloc(t->numNodes * sizeof(double));
+  v->cy = malloc(t->numNodes * sizeof(double));
+  v->cr = malloc(t->numNodes * sizeof(double));
+  for (size_t i = 0; i < t->numNodes; ++i) {
+    BfCircle c = bfQuadtreeNodeGetBoundingCircle(&t->nodes[i]);
+    v->cx[i] = c.center[0];
+    v->cy[i] = c.center[1];
+    v->cr[i] = c.r;
+  }
+}
+
+static void tree_view_free(TreeView *v) {
+  free(v->cx);
+  free(v->cy);
+  free(v->cr);
+}
+
+static inline const BfQuadtreeNode *node(const TreeView *v, int32_t i) { return &v->t->nodes[i]; }
+static inline int64_t npts(const TreeView *v, int32_t i) {
+  return (int64_t)(v->t->nodes[i].offset[4] - v->t->nodes[i].offset[0]);
+}
+static inline int is_leaf(const TreeView *v, int32_t i) {
+  const BfQuadtreeNode *n = node(v, i);
+  return n->child[0] < 0 && n->child[1] < 0 && n->child[2] < 0 && n->child[3] < 0;
+}
+
+/* p = ceil(k r1 r2 / d - C log10(tol)) with (r1, circle 1) first, in the
+ * argument order of each reference call site (helm2.c:14-36) */
+static int64_t rank_est(const Planner *P, const TreeView *a, int32_t ia, const TreeView *b, int32_t ib) {
+  double r1 = a->cr[ia], r2 = b->cr[ib];
+  double R = hypot(b->cx[ib] - a->cx[ia], b->cy[ib] - a->cy[ia]);
+  double d = R - r1 - r2;
+  double p = P->k * r1 * r2 / d - P->C * log10(P->tol);
+  return (int64_t)ceil(p);
+}
+
+static int separated(const TreeView *a, int32_t ia, const TreeView *b, int32_t ib) {
+  double R = hypot(b->cx[ib] - a->cx[ia], b->cy[ib] - a->cy[ia]);
+  return R > a->cr[ia] + b->cr[ib] + 1e1 * BF_EPS_MACH;
+}
+
+static int children(const TreeView *v, int32_t i, int32_t out[4], int selfIfLeaf) {
+  int n = 0;
+  for (int q = 0; q < 4; ++q)
+    if (node(v, i)->child[q] >= 0) out[n++] = node(v, i)->child[q];
+  if (n == 0 && selfIfLeaf) out[n++] = i;
+  return n;
+}
+
+static int64_t alloc_arena(BfMatHipButterfly *F, int64_t entries) {
+  int64_t off = F->arenaEntries;
+  F->arenaEntries += entries;
+  return off;
+}
+
+static void push_kernel_job(Planner *P, BfHipBlock *b, const TreeView *sv, int32_t s, int srcIsPoints,
+                            int64_t ncols, int32_t t) {
+  BfMatHipButterfly *F = P->F;
+  BfkKernelJob J;
+  memset(&J, 0, sizeof(J));
+  J.scx = sv->cx[s];
+  J.scy = sv->cy[s];
+  J.sr = sv->cr[s];
+  J.srcOff = srcIsPoints ? (int64_t)node(sv, s)->offset[0] : -1;
+  J.tgtOff = (int64_t)node(&P->tgt, t)->offset[0];
+  J.m = b->m;
+  J.n = (int32_t)ncols;
+  b->dataOff = alloc_arena(F, (int64_t)b->m * b->n);
+  J.out = b->dataOff;
+  BF_VEC_PUSH(F->kernelJobs, J);
+}
+
+/* re-expansion from `orig` (points of source node o when origIsPoints,
+ * else n proxies on its circle) to m proxies on the circle of source node
+ * e, tested on m proxies on the circle of target node tt */
+static void push_lstsq_job(Planner *P, BfHipBlock *b, int32_t e, int32_t o, int origIsPoints, int32_t tt) {
+  BfMatHipButterfly *F = P->F;
+  BfkLstsqJob J;
+  memset(&J, 0, sizeof(J));
+  J.ecx = P->src.cx[e]; J.ecy = P->src.cy[e]; J.er = P->src.cr[e];
+  J.tcx = P->tgt.cx[tt]; J.tcy = P->tgt.cy[tt]; J.tr = P->tgt.cr[tt];
+  J.ocx = P->src.cx[o]; J.ocy = P->src.cy[o]; J.orr = P->src.cr[o];
+  J.origOff = origIsPoints ? (int64_t)node(&P->src, o)->offset[0] : -1;
+  J.p = b->m;
+  J.n = b->n;
+  b->dataOff = alloc_arena(F, (int64_t)b->m * b->n);
+  J.out = b->dataOff;
+  BF_VEC_PUSH(F->lstsqJobs, J);
+}
+
+static int64_t push_offsets(BfMatHipButterfly *F, const int64_t *sizes, int64_t n) {
+  int64_t start = (int64_t)F->offs.n;
+  int64_t acc = 0;
+  BF_VEC_PUSH(F->offs, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    acc += sizes[i];
+    BF_VEC_PUSH(F->offs, acc);
+  }
+  return start;
+}
+
+/* Butterfly between S and T from level lists Ls[0..D] (Ls[0] = {S}) and
+ * Lt[0..D] (Lt[0] = {T}): the first factor is block diagonal over Ls[D]
+ * (makeFirstFactor, fac_helm2.c:42-160), factor i = 1..D is block-sparse
+ * with one block per (target child, source child) (makeFactor, :222-401),
+ * the last is block diagonal over Lt[D] (makeLastFactor, :403-509).
+ * With `modes` a pair may carry the source charges themselves or the
+ * target values themselves instead of proxies, whichever the estimate
+ * says is smaller, so leaves carried below their depth cost nothing. */
+static void make_butterfly(Planner *P, BfHipLeaf *leaf, Level *Ls, Level *Lt, int D, int modes) {
+  BfMatHipButterfly *F = P->F;
+  const TreeView *sv = &P->src, *tv = &P->tgt;
+  int32_t S = Ls[0].a[0], T = Lt[0].a[0];
+  leaf->numFactors = D + 2;
+  leaf->firstFactor = (int64_t)F->factors.n;
+
+  /* first factor */
+  int64_t nb = (int64_t)Ls[D].n;
+  int8_t *mode = malloc(nb);
+  int64_t *msize = malloc(nb * sizeof(int64_t));
+  int64_t *cs = malloc(nb * sizeof(int64_t));
+  BfHipFactor f0;
+  memset(&f0, 0, sizeof(f0));
+  f0.isDiag = 1;
+  f0.numBlockRows = f0.numBlockCols = f0.numBlocks = nb;
+  f0.firstBlock = (int64_t)F->blocks.n;
+  for (int64_t i = 0; i < nb; ++i) {
+    int32_t s = Ls[D].a[i];
+    int64_t p = rank_est(P, sv, s, tv, T);
+    int64_t ns = npts(sv, s);
+    BfHipBlock b;
+    memset(&b, 0, sizeof(b));
+    b.rowInd = b.colInd = (int32_t)i;
+    b.n = (int32_t)ns;
+    if (modes && is_leaf(sv, s) && p >= ns) {
+      mode[i] = MODE_CHARGES;
+      b.m = (int32_t)ns;
+      b.kind = BF_BLOCK_IDENTITY;
+      b.dataOff = -1;
+    } else {
+      mode[i] = MODE_PROXY;
+      b.m = (int32_t)p;
+      b.kind = BF_BLOCK_LSTSQ;
+      push_lstsq_job(P, &b, s, s, 1, T);
+    }
+    msize[i] = b.m;
+    cs[i] = b.n;
+    BF_VEC_PUSH(F->blocks, b);
+  }
+  f0.rowOffStart = push_offsets(F, msize, nb);
+  f0.colOffStart = push_offsets(F, cs, nb);
+  BF_VEC_PUSH(F->factors, f0);
+  free(cs);
+
+  /* middle factors */
+  for (int st = 1; st <= D; ++st) {
+    Level *sPar = &Ls[D - st], *sCh = &Ls[D - st + 1];
+    Level *tPar = &Lt[st - 1], *tCh = &Lt[st];
+    int64_t nRows = (int64_t)tCh->n * (int64_t)sPar->n;
+    int64_t nCols = (int64_t)tPar->n * (int64_t)sCh->n;
+    BfHipFactor f;
+    memset(&f, 0, sizeof(f));
+    f.isDiag = 0;
+    f.numBlockRows = nRows;
+    f.numBlockCols = nCols;
+    f.numBlocks = (int64_t)tCh->n * (int64_t)sCh->n;
+    f.firstBlock = (int64_t)F->blocks.n;
+    int64_t *rowSize = calloc(nRows, sizeof(int64_t));
+    int8_t *newMode = malloc(nRows);
+    int64_t tcIdx = 0;
+    for (int64_t tp = 0; tp < (int64_t)tPar->n; ++tp) {
+      int32_t tkids[4];
+      int ntk = children(tv, tPar->a[tp], tkids, 1);
+      for (int a = 0; a < ntk; ++a, ++tcIdx) {
+        int32_t tc = tkids[a];
+        int64_t scIdx = 0;
+        for (int64_t sp = 0; sp < (int64_t)sPar->n; ++sp) {
+          int32_t spn = sPar->a[sp];
+          int32_t skids[4];
+          int nsk = children(sv, spn, skids, 1);
+          int64_t row = tcIdx * (int64_t)sPar->n + sp;
+          int64_t est = 0;
+          int anyValues = 0, allCharges = 1;
+          for (int c = 0; c < nsk; ++c) {
+            int64_t r1 = rank_est(P, sv, skids[c], tv, tPar->a[tp]);
+            int64_t r2 = rank_est(P, sv, spn, tv, tc);
+            int64_t r = r1 > r2 ? r1 : r2;
+            if (r > est) est = r;
+            int64_t col = tp * (int64_t)sCh->n + scIdx + c;
+            if (mode[col] == MODE_VALUES) anyValues = 1;
+            if (mode[col] != MODE_CHARGES) allCharges = 0;
+          }
+          int rm = MODE_PROXY;
+          int64_t rows = est;
+          if (modes) {
+            if (anyValues || (is_leaf(tv, tc) && est >= npts(tv, tc))) {
+              rm = MODE_VALUES;
+              rows = npts(tv, tc);
+            } else if (is_leaf(sv, spn) && allCharges && est >= npts(sv, spn)) {
+              rm = MODE_CHARGES;
+              rows = npts(sv, spn);
+            }
+          }
+          rowSize[row] = rows;
+          newMode[row] = (int8_t)rm;
+          for (int c = 0; c < nsk; ++c) {
+            int64_t col = tp * (int64_t)sCh->n + scIdx + c;
+            int cm = mode[col];
+            BfHipBlock b;
+            memset(&b, 0, sizeof(b));
+            b.rowInd = (int32_t)row;
+            b.colInd = (int32_t)col;
+            b.m = (int32_t)rows;
+            b.n = (int32_t)msize[col];
+            if ((cm == MODE_CHARGES && rm == MODE_CHARGES) || (cm == MODE_VALUES && rm == MODE_VALUES)) {
+              b.kind = BF_BLOCK_IDENTITY;
+              b.dataOff = -1;
+            } else if (rm == MODE_VALUES) {
+              b.kind = BF_BLOCK_KERNEL;
+              push_kernel_job(P, &b, sv, skids[c], cm == MODE_CHARGES, b.n, tc);
+            } else {
+              b.kind = BF_BLOCK_LSTSQ;
+              push_lstsq_job(P, &b, spn, skids[c], cm == MODE_CHARGES, tc);
+            }
+            BF_VEC_PUSH(F->blocks, b);
+          }
+          scIdx += nsk;
+        }
+      }
+    }
+    f.rowOffStart = push_offsets(F, rowSize, nRows);
+    f.colOffStart = push_offsets(F, msize, nCols);
+    BF_VEC_PUSH(F->factors, f);
+    free(mode);
+    free(msize);
+    mode = newMode;
+    msize = rowSize;
+  }
+
+  /* last factor */
+  nb = (int64_t)Lt[D].n;
+  BfHipFactor fl;
+  memset(&fl, 0, sizeof(fl));
+  fl.isDiag = 1;
+  fl.numBlockRows = fl.numBlockCols = fl.numBlocks = nb;
+  fl.firstBlock = (int64_t)F->blocks.n;
+  int64_t *rs = malloc(nb * sizeof(int64_t));
+  for (int64_t i = 0; i < nb; ++i) {
+    int32_t t = Lt[D].a[i];
+    BfHipBlock b;
+    memset(&b, 0, sizeof(b));
+    b.rowInd = b.colInd = (int32_t)i;
+    b.m = (int32_t)npts(tv, t);
+    b.n = (int32_t)msize[i];
+    if (mode[i] == MODE_VALUES) {
+      b.kind = BF_BLOCK_IDENTITY;
+      b.dataOff = -1;
+    } else {
+      b.kind = BF_BLOCK_KERNEL;
+      push_kernel_job(P, &b, sv, S, mode[i] == MODE_CHARGES, b.n, t);
+    }
+    rs[i] = b.m;
+    BF_VEC_PUSH(F->blocks, b);
+  }
+  fl.rowOffStart = push_offsets(F, rs, nb);
+  fl.colOffStart = push_offsets(F, msize, nb);
+  BF_VEC_PUSH(F->factors, fl);
+  free(rs);
+  free(mode);
+  free(msize);
+}
+
+static BfHipLeaf *new_leaf(BfMatHipButterfly *F, const Planner *P, int32_t s, int32_t t, int isButterfly) {
+  BfHipLeaf l;
+  memset(&l, 0, sizeof(l));
+  l.isButterfly = isButterfly;
+  l.row0 = (int64_t)node(&P->tgt, t)->offset[0];
+  l.col0 = (int64_t)node(&P->src, s)->offset[0];
+  l.m = npts(&P->tgt, t);
+  l.n = npts(&P->src, s);
+  l.denseOff = -1;
+  BF_VEC_PUSH(F->leaves, l);
+  return &F->leaves.a[F->leaves.n - 1];
+}
+
+static void make_dense(Planner *P, int32_t s, int32_t t) {
+  BfHipLeaf *l = new_leaf(P->F, P, s, t, 0);
+  BfHipBlock b;
+  memset(&b, 0, sizeof(b));
+  b.m = (int32_t)l->m;
+  b.n = (int32_t)l->n;
+  push_kernel_job(P, &b, &P->src, s, 1, l->n, t);
+  P->F->leaves.a[P->F->leaves.n - 1].denseOff = b.dataOff;
+}
+
+/* breadth-first levels of the subtree under `root` */
+static int bfs_levels(const TreeView *v, int32_t root, Level **out) {
+  int cap = 8, n = 0;
+  Level *lv = calloc(cap, sizeof(Level));
+  BF_VEC_PUSH(lv[0], root);
+  n = 1;
+  for (;;) {
+    Level nxt;
+    memset(&nxt, 0, sizeof(nxt));
+    Level *cur = &lv[n - 1];
+    for (size_t i = 0; i < cur->n; ++i) {
+      int32_t kids[4];
+      int nk = children(v, cur->a[i], kids, 0);
+      for (int c = 0; c < nk; ++c) BF_VEC_PUSH(nxt, kids[c]);
+    }
+    if (nxt.n == 0) break;
+    if (n == cap) {
+      lv = realloc(lv, 2 * cap * sizeof(Level));
+      memset(lv + cap, 0, cap * sizeof(Level));
+      cap *= 2;
+    }
+    lv[n++] = nxt;
+  }
+  *out = lv;
+  return n;
+}
+
+static int64_t level_points(const TreeView *v, const Level *l) {
+  int64_t s = 0;
+  for (size_t i = 0; i < l->n; ++i) s += npts(v, l->a[i]);
+  return s;
+}
+
+static int level_internal(const TreeView *v, const Level *l) {
+  for (size_t i = 0; i < l->n; ++i)
+    if (is_leaf(v, l->a[i])) return 0;
+  return 1;
+}
+
+/* allRankEstimatesAreOK (fac_helm2.c:511-530) */
+static int all_ranks_ok(const Planner *P, int32_t T, const Level *l) {
+  for (size_t i = 0; i < l->n; ++i)
+    if (rank_est(P, &P->tgt, T, &P->src, l->a[i]) > npts(&P->src, l->a[i])) return 0;
+  return 1;
+}
+
+static void free_levels(Level *lv, int n) {
+  for (int i = 0; i < n; ++i) BF_VEC_FREE(lv[i]);
+  free(lv);
+}
+
+static void make_separated(Planner *P, int32_t s, int32_t t) {
+  const TreeView *sv = &P->src, *tv = &P->tgt;
+  if (P->F->opts.passThrough) {
+    int ds = node(sv, s)->maxDepthBelow - node(sv, s)->depth;
+    int dt = node(tv, t)->maxDepthBelow - node(tv, t)->depth;
+    int D = ds < dt ? ds : dt;
+    Level *Ls = calloc(D + 1, sizeof(Level)), *Lt = calloc(D + 1, sizeof(Level));
+    BF_VEC_PUSH(Ls[0], s);
+    BF_VEC_PUSH(Lt[0], t);
+    for (int j = 0; j < D; ++j) {
+      int32_t kids[4];
+      for (size_t i = 0; i < Ls[j].n; ++i) {
+        int nk = children(sv, Ls[j].a[i], kids, 1);
+        for (int c = 0; c < nk; ++c) BF_VEC_PUSH(Ls[j + 1], kids[c]);
+      }
+      for (size_t i = 0; i < Lt[j].n; ++i) {
+        int nk = children(tv, Lt[j].a[i], kids, 1);
+        for (int c = 0; c < nk; ++c) BF_VEC_PUSH(Lt[j + 1], kids[c]);
+      }
+    }
+    BfHipLeaf *l = new_leaf(P->F, P, s, t, 1);
+    make_butterfly(P, l, Ls, Lt, D, 1);
+    free_levels(Ls, D + 1);
+    free_levels(Lt, D + 1);
+    return;
+  }
+
+  /* bfFacHelm2Prepare (fac_helm2.c:551-651) */
+  Level *sl, *tl;
+  int nsl = bfs_levels(sv, s, &sl), ntl = bfs_levels(tv, t, &tl);
+  int tgtDepth = node(tv, t)->depth;
+  int maxAllow = tgtDepth;
+  for (int j = 1; j < ntl && level_internal(tv, &tl[j]); ++j) ++maxAllow;
+  int li = nsl - 1;
+  int srcDepth = node(sv, s)->depth + li;
+  while (srcDepth > maxAllow && li > 0) { --li; --srcDepth; }
+  while (level_points(sv, &sl[li]) != npts(sv, s) && li > 0) { --li; --srcDepth; }
+  while (!level_internal(sv, &sl[li]) && li > 0) { --li; --srcDepth; }
+  while (srcDepth > tgtDepth && !all_ranks_ok(P, t, &sl[li]) && li > 0) { --li; --srcDepth; }
+  int nf = all_ranks_ok(P, t, &sl[li]) ? srcDepth - tgtDepth + 2 : 0;
+  if (nf == 0) {
+    make_dense(P, s, t);
+  } else {
+    BfHipLeaf *l = new_leaf(P->F, P, s, t, 1);
+    make_butterfly(P, l, sl, tl, nf - 2, 0);
+  }
+  free_levels(sl, nsl);
+  free_levels(tl, ntl);
+}
+
+/* facHelm2MakeMultilevel_rec (fac_helm2.c:860-941) */
+static void multilevel_rec(Planner *P, const int32_t *sNodes, int ns, const int32_t *tNodes, int nt) {
+  const TreeView *sv = &P->src, *tv = &P->tgt;
+  for (int i = 0; i < nt; ++i) {
+    for (int j = 0; j < ns; ++j) {
+      int32_t s = sNodes[j], t = tNodes[i];
+      if ((BfSize)(npts(tv, t) * npts(sv, s)) < P->F->opts.maxDenseSize) {
+        make_dense(P, s, t);
+      } else if (separated(sv, s, tv, t)) {
+        make_separated(P, s, t);
+      } else {
+        /* a leaf stands in for its own child list; two leaves that are
+         * still not separated become one dense block */
+        int32_t sc[4], tc[4];
+        int nsc = children(sv, s, sc, 1), ntc = children(tv, t, tc, 1);
+        if (nsc == 1 && ntc == 1 && sc[0] == s && tc[0] == t) {
+          make_dense(P, s, t);
+          continue;
+        }
+        multilevel_rec(P, sc, nsc, tc, ntc);
+      }
+    }
+  }
+}
+
+int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt) {
+  Planner P;
+  memset(&P, 0, sizeof(P));
+  P.F = F;
+  P.k = F->helm.k;
+  P.tol = F->opts.tol;
+  P.C = F->opts.rankConst;
+  tree_view(&P.src, src);
+  tree_view(&P.tgt, tgt);
+  if (src->numLevels < 3 || tgt->numLevels < 3) {
+    tree_view_free(&P.src);
+    tree_view_free(&P.tgt);
+    return -1;
+  }
+  /* start from the second level of both trees (fac_helm2.c:954-982) */
+  int32_t s2[16], t2[16];
+  int n2s = 0, n2t = 0;
+  for (BfSize i = src->levelOffset[2]; i < src->levelOffset[3]; ++i) s2[n2s++] = (int32_t)i;
+  for (BfSize i = tgt->levelOffset[2]; i < tgt->levelOffset[3]; ++i) t2[n2t++] = (int32_t)i;
+  multilevel_rec(&P, s2, n2s, t2, n2t);
+  tree_view_free(&P.src);
+  tree_view_free(&P.tgt);
+  return 0;
+}

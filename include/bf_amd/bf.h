@@ -1,0 +1,365 @@
+/*
+ * butterfly_amd -- the C ABI of the MI355X 2-D Helmholtz butterfly
+ * factor-and-apply path.
+ *
+ * Every entry point below keeps the name, argument meaning, ownership rule
+ * and error behaviour of the reference function it replaces (cited
+ * path:line, relative to the reference repository).  Entry points prefixed
+ * `bfhip` have no reference counterpart; each says who it is for.
+ *
+ * Conventions (as in the reference):
+ *   - `New` allocates, `Init` fills caller storage, `Deinit` releases what
+ *     `Init` made, `Dealloc` frees the struct, `Delete` does both
+ *     (docs/implementation.rst:27-48).
+ *   - Errors: functions return NULL / 0 / NaN and record a code that
+ *     bfGetError() returns and clears (error.c:7-24).  Unlike the
+ *     reference's debug builds, a pending error never traps; HIP failures
+ *     map to BF_ERROR_RUNTIME_ERROR or BF_ERROR_MEMORY_ERROR.
+ *   - Complex numbers are interleaved FP64 (C99 double _Complex, def.h:27).
+ *     Dense matrices are row-major with explicit strides.
+ *   - A factorization maps X in source-tree order to Y in target-tree
+ *     order; callers permute with the tree permutations (helm2_bie.c:73-81).
+ *   - Single-threaded host API: the error code, the RNG state and the
+ *     library's HIP stream are process-global.
+ */
+#pragma once
+
+#include <complex.h>
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types */
+
+/* def.h:23-41 */
+typedef double BfReal;
+typedef double _Complex BfComplex;
+typedef size_t BfSize;
+typedef BfReal BfPoint2[2];
+typedef BfReal BfVector2[2];
+#define BF_EPS_MACH 2.220446049250313e-16
+#define BF_SIZE_BAD_VALUE ((BfSize)-1)
+
+/* error.h:3-17 */
+enum BfError {
+  BF_ERROR_NONE,
+  BF_ERROR_INVALID_ARGUMENTS,
+  BF_ERROR_RUNTIME_ERROR,
+  BF_ERROR_NOT_IMPLEMENTED,
+  BF_ERROR_MEMORY_ERROR,
+  BF_ERROR_OUT_OF_RANGE,
+  BF_ERROR_FILE_ERROR,
+  BF_ERROR_TYPE_ERROR,
+  BF_ERROR_INCOMPATIBLE_SHAPES,
+};
+
+/* types.h:78-144 (matrix and tree types this path uses; the device
+ * factorization is appended as a new type) */
+typedef enum BfTypes {
+  BF_TYPE_MAT = 0,
+  BF_TYPE_MAT_PRODUCT = 10,
+  BF_TYPE_MAT_BLOCK_COO = 15,
+  BF_TYPE_MAT_BLOCK_DENSE = 16,
+  BF_TYPE_MAT_BLOCK_DIAG = 17,
+  BF_TYPE_MAT_DENSE_COMPLEX = 19,
+  BF_TYPE_QUADTREE = 34,
+  BF_TYPE_QUADTREE_NODE = 39,
+  BF_TYPE_MAT_HIP_BUTTERFLY = 42,
+  BF_TYPE_NONE = 1000
+} BfType;
+
+/* layer_pot.h:28-69 */
+typedef enum BfLayerPotentials {
+  BF_LAYER_POTENTIAL_UNKNOWN = 0,
+  BF_LAYER_POTENTIAL_SINGLE,
+  BF_LAYER_POTENTIAL_PV_DOUBLE,
+  BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_SINGLE,
+  BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_DOUBLE,
+  BF_LAYER_POTENTIAL_COMBINED_FIELD,
+  BF_LAYER_POTENTIAL_COUNT,
+} BfLayerPotential;
+
+/* helm2.h:10-15 */
+typedef struct BfHelm2 {
+  BfReal k;
+  BfLayerPotential layerPot;
+  BfComplex alpha;
+  BfComplex beta;
+} BfHelm2;
+
+/* circle.h:5-8 */
+typedef struct BfCircle {
+  BfReal r;
+  BfPoint2 center;
+} BfCircle;
+
+/* points.h:39-44, vectors.h:19-24 */
+typedef struct BfPoints2 {
+  BfPoint2 *data;
+  BfSize size;
+  BfSize capacity;
+  bool isView;
+} BfPoints2;
+
+typedef struct BfVectors2 {
+  BfVector2 *data;
+  BfSize size;
+  BfSize capacity;
+  bool isView;
+} BfVectors2;
+
+/* perm.h:6-10 */
+typedef struct BfPerm {
+  BfSize *index;
+  BfSize size;
+  bool isView;
+} BfPerm;
+
+/* A quadtree node.  The reference links nodes by pointers
+ * (quadtree_node.h, tree_node.h); here nodes live in one array in
+ * breadth-first order, so `child` and `parent` are indices into it
+ * (-1 = none).  `offset[q]..offset[q+1]` is child q's range of the
+ * tree permutation, as in tree_node.h. */
+typedef struct BfQuadtreeNode {
+  BfReal bboxMin[2], bboxMax[2];
+  BfReal split[2];
+  BfSize offset[5];
+  int32_t child[4];
+  int32_t parent;
+  int32_t depth;
+  int32_t maxDepthBelow; /* depth of the deepest leaf in this subtree */
+  int32_t index;         /* position among the parent's children (0..3) */
+} BfQuadtreeNode;
+
+/* tree.h:30-39 */
+typedef struct BfTree {
+  void *vtable;          /* unused; kept so `perm` sits where callers expect */
+  BfQuadtreeNode *root;
+  BfPerm *perm;
+} BfTree;
+
+/* quadtree.h:14-24 */
+typedef struct BfQuadtree {
+  BfTree super;
+  BfPoints2 const *points;
+  BfVectors2 const *unitNormals;
+  BfQuadtreeNode *nodes; /* breadth-first order, nodes[0] is the root */
+  BfSize numNodes;
+  BfSize leafSize;       /* a node with <= leafSize points is a leaf */
+  BfSize *levelOffset;   /* nodes of depth d are [levelOffset[d], levelOffset[d+1]) */
+  BfSize numLevels;
+} BfQuadtree;
+
+/* --------------------------------------------------------- matrix types */
+
+typedef struct BfMat BfMat;
+typedef struct BfMatVtable BfMatVtable;
+
+/* mat.h:30-39 */
+typedef enum BfMatProps {
+  BF_MAT_PROPS_NONE = 0,
+  BF_MAT_PROPS_VIEW = 1 << 0,
+  BF_MAT_PROPS_TRANS = 1 << 1,
+  BF_MAT_PROPS_CONJ = 1 << 2,
+  BF_MAT_PROPS_ORTHO = 1 << 3,
+} BfMatProps;
+
+/* mat.h:187-196: the base every matrix type embeds first */
+struct BfMat {
+  BfMatVtable *vtbl;
+  BfMatProps props;
+  BfSize numRows;
+  BfSize numCols;
+};
+
+/* mat_dense_complex.h:69-80 */
+typedef struct BfMatDenseComplex {
+  BfMat super;
+  BfSize rowStride;
+  BfSize colStride;
+  BfComplex *data;
+  void *pyArray;
+} BfMatDenseComplex;
+
+/* ----------------------------------------------------------- functions */
+
+/* error.c:10-24 (no trap on a pending error) */
+enum BfError bfGetError(void);
+void bfSetError(enum BfError error);
+
+/* rand.c:19-76 */
+void bfSeed(BfSize seed);
+BfReal bfRealUniform1(void);
+void bfRealUniform(BfSize n, BfReal *x);
+void bfRealRandn(BfSize n, BfReal *x);
+void bfComplexRandn(BfSize n, BfComplex *x);
+/* points.c:25-28 */
+void bfPoint2SampleUniformlyFromBoundingBox(BfReal const bboxMin[2], BfReal const bboxMax[2], BfPoint2 p);
+
+/* points.c: construction and release of point sets */
+BfPoints2 *bfPoints2NewWithCapacity(BfSize capacity);
+void bfPoints2Append(BfPoints2 *points, BfPoint2 const p);
+void bfPoints2DeinitAndDealloc(BfPoints2 **points);
+/* new: a view over caller-owned n x 2 coordinates (no counterpart; for
+ * FFI callers that hold coordinates in their own arrays) */
+BfPoints2 *bfPoints2NewViewFromPtr(BfReal *xy, BfSize n);
+
+/* perm.c:195-220 */
+BfPerm *bfPermGetReversePerm(BfPerm const *perm);
+void bfPermDelete(BfPerm **perm);
+
+/* quadtree.c:80-98, tree.c:110; tree.h accessors */
+BfQuadtree *bfQuadtreeNew(void);
+void bfQuadtreeInit(BfQuadtree *tree, BfPoints2 const *points, BfVectors2 const *unitNormals);
+/* new: the same with an explicit leaf threshold (reference: fixed at 1,
+ * quadtree_node.c:17) */
+void bfQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points,
+                                BfVectors2 const *unitNormals, BfSize leafSize);
+void bfQuadtreeDeinit(BfQuadtree *tree);
+void bfQuadtreeDealloc(BfQuadtree **tree);
+void bfQuadtreeDeinitAndDealloc(BfQuadtree **tree);
+BfTree *bfQuadtreeToTree(BfQuadtree *tree);
+BfPerm const *bfTreeGetPermConst(BfTree const *tree);
+/* quadtree_node.c:321-330, :393-401 */
+BfCircle bfQuadtreeNodeGetBoundingCircle(BfQuadtreeNode const *node);
+bool bfQuadtreeNodesAreSeparated(BfQuadtreeNode const *node1, BfQuadtreeNode const *node2);
+BfSize bfQuadtreeNodeGetNumPoints(BfQuadtreeNode const *node);
+
+/* helm2.c:14-36 */
+BfSize bfHelm2RankEstForTwoCircles(BfHelm2 const *helm, BfCircle const *circ1,
+                                   BfCircle const *circ2, BfReal C, BfReal eps);
+/* helm2.c:282-319: dense kernel matrix, rows = targets, cols = sources.
+ * Sampled on the GPU by the sampling kernel; returns a host matrix. */
+BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoints2 const *Xtgt,
+                              BfVectors2 const *Nsrc, BfVectors2 const *Ntgt);
+
+/* fac_helm2.c:943-1002: multilevel butterfly of the kernel from the points
+ * of `srcTree` to the points of `tgtTree`, with the reference's planning
+ * rules (dense below 128*128 entries, rank estimates at 1e-15,
+ * bfFacHelm2Prepare's level choice).  Factors live in GPU memory; the
+ * returned matrix has type BF_TYPE_MAT_HIP_BUTTERFLY and is applied with
+ * bfMatMul. */
+BfMat *bfFacHelm2MakeMultilevel(BfHelm2 const *helm, BfQuadtree const *srcTree,
+                                BfQuadtree const *tgtTree);
+
+/* new: planner options beside the reference rules (no counterpart: the
+ * reference hard-codes them, fac_helm2.c:20, :102, :291-296, :523) */
+typedef struct BfFacHelm2Options {
+  BfReal tol;            /* rank estimate p = ceil(k r1 r2/d - C log10(tol)); reference 1e-15 */
+  BfReal rankConst;      /* C above; reference 1 */
+  BfSize maxDenseSize;   /* blocks with m*n below this are dense; reference 128*128 */
+  bool passThrough;      /* carry shallow leaves down the butterfly levels
+                            instead of bfFacHelm2Prepare's depth caps */
+} BfFacHelm2Options;
+void bfFacHelm2OptionsInitDefault(BfFacHelm2Options *opts);
+BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const *srcTree,
+                                           BfQuadtree const *tgtTree, BfFacHelm2Options const *opts);
+
+/* new (no counterpart): run only the host planner -- the quadtree level
+ * choices, block shapes and apply stages -- without touching a GPU.  The
+ * result supports the plan export below and bfMatDelete, not bfMatMul.
+ * For tools that size a factorization before running it. */
+BfMat *bfhipFacHelm2Plan(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
+                         BfFacHelm2Options const *opts);
+
+/* mat.c dispatchers (mat.h:45-110) */
+BfType bfMatGetType(BfMat const *mat);
+BfSize bfMatGetNumRows(BfMat const *mat);
+BfSize bfMatGetNumCols(BfMat const *mat);
+BfSize bfMatNumBytes(BfMat const *mat);
+BfMat *bfMatCopy(BfMat const *mat);
+void bfMatDelete(BfMat **mat);
+/* mat.c:183-185: Y = A X.  For a device factorization X is N_src x nrhs in
+ * source-tree order and Y comes back N_tgt x nrhs in target-tree order. */
+BfMat *bfMatMul(BfMat const *op1, BfMat const *op2);
+/* mat_dense_complex.c:741-774: row i moves to row perm->index[i] */
+void bfMatPermuteRows(BfMat *mat, BfPerm const *perm);
+
+/* mat_dense_complex.h:69-137 */
+BfMatDenseComplex *bfMatDenseComplexNew(void);
+void bfMatDenseComplexInit(BfMatDenseComplex *mat, BfSize numRows, BfSize numCols);
+void bfMatDenseComplexInitViewFromPtr(BfMatDenseComplex *mat, BfSize numRows, BfSize numCols,
+                                      BfComplex *data);
+BfMatDenseComplex *bfMatDenseComplexNewRandn(BfSize numRows, BfSize numCols);
+void bfMatDenseComplexDeinit(BfMatDenseComplex *mat);
+void bfMatDenseComplexDeinitAndDealloc(BfMatDenseComplex **mat);
+BfComplex *bfMatDenseComplexGetDataPtr(BfMatDenseComplex *mat);
+BfMat *bfMatDenseComplexToMat(BfMatDenseComplex *mat);
+BfMatDenseComplex *bfMatToMatDenseComplex(BfMat *mat);
+
+/* mat_product.h:38-41 on the device factorization: number of top-level
+ * blocks (dense or butterfly) and factors of butterfly leaf `i` */
+BfSize bfMatProductNumFactors(BfMat const *fac, BfSize leaf);
+
+/* ------------------------------------------------ device entry points */
+
+/* Precision of the apply: FP64 factors, FP32 factors with FP32
+ * accumulation, or FP32 hi+lo factors (two FP32 products summed). */
+typedef enum BfhipPrecision {
+  BFHIP_PRECISION_FP64 = 0,
+  BFHIP_PRECISION_FP32 = 1,
+  BFHIP_PRECISION_FP32X2 = 2,
+} BfhipPrecision;
+
+/* new (no counterpart): device memory helpers, so callers without a GPU
+ * runtime of their own (ctypes, FFI) can stage inputs in HBM */
+void *bfhipMalloc(BfSize numBytes);
+void bfhipFree(void *ptr);
+int bfhipMemcpyHtoD(void *dst, void const *src, BfSize numBytes);
+int bfhipMemcpyDtoH(void *dst, void const *src, BfSize numBytes);
+int bfhipSynchronize(void);
+int bfhipDeviceCount(void);
+
+/* new (no counterpart): Y = A X with X, Y already in device memory, for
+ * callers that keep vectors resident in HBM (benchmarks, solvers).
+ * X: N_src x nrhs, Y: N_tgt x nrhs, complex, row-major, tree orders;
+ * FP64 interleaved for BFHIP_PRECISION_FP64, FP32 interleaved otherwise.
+ * Returns 0 on success. */
+int bfhipMatMulDevice(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec);
+
+/* new: convert the stored factors for the FP32 / FP32x2 applies (kept
+ * beside the FP64 factors until bfMatDelete) */
+int bfhipFacPrepareMixed(BfMat *fac, BfhipPrecision prec);
+
+/* new (no counterpart): factorization statistics
+ * out[0] factor wall seconds, [1] tree+plan seconds, [2] sampling+compression
+ * seconds, [3] stored complex entries, [4] LstSq blocks, [5] kernel blocks,
+ * [6] identity blocks, [7] dense leaves, [8] butterfly leaves, [9] stages,
+ * [10] H0 evaluations */
+void bfhipFacGetStats(BfMat const *fac, double out[16]);
+
+/* new: average device time (ms) of the last launches of each kernel,
+ * measured with HIP events on the library stream:
+ * out[0] sampling, [1] compression, [2] apply (sum of stages of one apply),
+ * [3] bytes of factor data read by one FP64 apply, [4] number of apply stages */
+void bfhipGetKernelTimes(double out[8]);
+/* new: the same, summed since the last reset, with the number of
+ * measurements per slot (slot 3 is the FP32 apply) */
+void bfhipGetKernelTimeSums(double sums[8], double counts[8]);
+void bfhipResetKernelTimes(void);
+
+/* new: plan export for parity checks (integers, in the oracle's canonical
+ * order): leafTab [L x 6], facTab [F x 4], blkTab [B x 7] */
+void bfhipFacPlanSizes(BfMat const *fac, BfSize *numLeaves, BfSize *numFactors, BfSize *numBlocks);
+void bfhipFacPlanExport(BfMat const *fac, int64_t *leafTab, int64_t *facTab, int64_t *blkTab);
+/* copy block `idx` (export order) as m x n row-major complex FP64 */
+int bfhipFacBlockData(BfMat const *fac, BfSize idx, BfComplex *out);
+int bfhipFacDenseData(BfMat const *fac, BfSize leafIdx, BfComplex *out);
+
+/* new: device H0^(1) on n arguments (host arrays), out = [J0, Y0] pairs */
+int bfhipHankel0(BfSize n, BfReal const *x, BfReal *out);
+
+/* new: batched pseudo-inverse least squares on the device with the
+ * reference's truncation rule (mat_dense_complex.c:1768-1849):
+ * X_b = pinv(A_b) B_b, A_b p x p, B_b p x n, all row-major complex,
+ * host arrays.  Returns 0 on success. */
+int bfhipLstSqBatched(BfSize batch, BfSize p, BfSize n, BfComplex const *A, BfComplex const *B,
+                      BfComplex *X);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,133 @@
+"""Host-side checks of the product library that need no GPU: it loads and
+exports what the header declares, its RNG, quadtree and planner agree with
+the oracle integer-for-integer, and the device H0 source compiled for the
+host matches mpmath."""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import butterfly_amd as bf
+import oracle as orc
+from conftest import ROOT, box_problem, curve_problem
+
+HEADER = os.path.join(ROOT, "include", "bf_amd", "bf.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(bf[A-Za-z0-9_]+)\s*\(", text))
+    return sorted(n for n in names if not n.startswith("bfk_"))
+
+
+def test_library_exports_every_declared_symbol():
+    L = bf.lib()
+    names = declared_functions()
+    assert len(names) > 60
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_error_state_is_read_and_cleared():
+    L = bf.lib()
+    L.bfSetError(3)
+    assert L.bfGetError() == 3
+    assert L.bfGetError() == 0
+
+
+def test_rng_matches_oracle_bit_for_bit():
+    for s in (0, 1, 12345):
+        bf.seed(s)
+        orc.seed(s)
+        a = bf.real_uniform(1000)
+        b = np.zeros(1000)
+        orc.lib().orc_uniform_fill(1000, orc._dp(b))
+        assert np.array_equal(a, b)
+        bf.seed(s)
+        orc.seed(s)
+        assert np.array_equal(bf.complex_randn(777), orc.complex_randn(777))
+
+
+@pytest.mark.parametrize("leaf", [1, 7, 64])
+def test_quadtree_perm_matches_oracle(leaf):
+    src, tgt, _ = box_problem(5000, seed=3)
+    for pts in (src, tgt, curve_problem(4096)[0]):
+        t = bf.Quadtree(pts, leaf)
+        perm, nodes, _ = orc.tree(pts, leaf)
+        assert np.array_equal(t.perm, perm)
+        assert t.num_nodes == len(nodes)
+
+
+def test_quadtree_rejects_bad_input():
+    with pytest.raises(bf.BfError):
+        bf.Quadtree(np.zeros((1, 2)), 1)
+
+
+def _plan_pair(src, tgt, k, leaf, tol, pt):
+    ts, tt = bf.Quadtree(src, leaf), bf.Quadtree(tgt, leaf)
+    F = bf.FacHelm2.plan_only(bf.Helm2(k), ts, tt, bf.options(tol=tol, pass_through=pt))
+    O = orc.Fac(k, src, tgt, leaf, tol, 128 * 128, pt, plan_only=True)
+    return F, O
+
+
+@pytest.mark.parametrize("case", ["box_ref", "box_pass", "curve_pass", "curve_ref"])
+def test_plan_matches_oracle(case):
+    if case.startswith("box"):
+        src, tgt, k = box_problem(4096)
+    else:
+        src, tgt, k = curve_problem(4096)
+    leaf, tol, pt = (1, 1e-15, False) if case.endswith("ref") else (64, 1e-8, True)
+    F, O = _plan_pair(src, tgt, k, leaf, tol, pt)
+    for a, b in zip(F.plan(), O.plan()):
+        assert a.shape == b.shape and np.array_equal(a, b)
+
+
+def test_plan_config0_census():
+    # config 0 under the reference rules: 256 butterflies of 4 factors,
+    # 4096 + 32768 + 4096 blocks, ranks 16 everywhere
+    src, tgt, k = box_problem(16384)
+    ts, tt = bf.Quadtree(src, 1), bf.Quadtree(tgt, 1)
+    F = bf.FacHelm2.plan_only(bf.Helm2(k), ts, tt)
+    lt, ft, bt = F.plan()
+    assert len(lt) == 256 and lt[:, 0].all() and (lt[:, 5] == 4).all()
+    assert len(bt) == 4096 + 32768 + 4096
+    lsq = bt[bt[:, 4] == 0]
+    assert (lsq[:, 2] == 16).all()
+
+
+def test_plan_covers_every_entry_once():
+    # the top-level leaves tile the N_tgt x N_src index square exactly
+    src, tgt, k = curve_problem(8192)
+    ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
+    F = bf.FacHelm2.plan_only(bf.Helm2(k), ts, tt, bf.options(tol=1e-8, pass_through=True))
+    lt, _, _ = F.plan()
+    assert (lt[:, 3] * lt[:, 4]).sum() == len(src) * len(tgt)
+    # no two leaves overlap: sort by (row0, col0) and check each row band
+    # of columns is a disjoint cover
+    for r0 in np.unique(lt[:, 1]):
+        sel = lt[lt[:, 1] == r0]
+        assert len(np.unique(sel[:, 2])) == len(sel)
+
+
+def test_device_h0_source_on_host(tmp_path):
+    # compile the device H0 for the host and compare with the mpmath table
+    shim = tmp_path / "h0shim.cpp"
+    shim.write_text('#include "bf_h0.h"\nextern "C" void h0v(int n, const double *x, double *o)'
+                    '{for(int i=0;i<n;i++) bf_h0(x[i], &o[2*i], &o[2*i+1]);}\n')
+    so = tmp_path / "h0shim.so"
+    subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-I",
+                    os.path.join(ROOT, "butterfly_amd", "csrc"), str(shim), "-o", str(so)], check=True)
+    L = C.CDLL(str(so))
+    tab = json.load(open(os.path.join(ROOT, "tests", "golden", "h0_table.json")))["rows"]
+    x = np.array([float(r[0]) for r in tab])
+    want = np.array([complex(float(r[1]), float(r[2])) for r in tab])
+    out = np.zeros(2 * len(x))
+    L.h0v(len(x), x.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
+    got = out.view(np.complex128)
+    err = np.abs(got - want) / np.abs(want)
+    assert err.max() < 2e-15, (x[np.argmax(err)], err.max())

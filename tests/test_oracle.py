@@ -1,0 +1,186 @@
+"""The oracle pinned against known answers, closed forms, mpmath and numpy
+(CPU only)."""
+import json
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import box_problem, curve_problem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_splitmix64_known_answers():
+    # published first outputs of splitmix64 seeded with 0
+    L = orc.lib()
+    L.orc_splitmix_seed(0)
+    got = [L.orc_splitmix_draw() for _ in range(3)]
+    assert got == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_xoshiro256plus_against_textbook_recurrence():
+    # an independent restatement of xoshiro256+ 1.0 (Blackman & Vigna 2018)
+    L = orc.lib()
+    L.orc_splitmix_seed(7)
+    s = [L.orc_splitmix_draw() for _ in range(4)]
+    M = (1 << 64) - 1
+    L.orc_seed(7)
+    for _ in range(100):
+        want = (s[0] + s[3]) & M
+        t = (s[1] << 17) & M
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t
+        s[3] = ((s[3] << 45) | (s[3] >> 19)) & M
+        assert L.orc_next_u64() == want
+
+
+def test_uniform_and_randn_statistics():
+    orc.seed(0)
+    u = np.array([orc.lib().orc_uniform() for _ in range(20000)])
+    assert u.min() >= 0 and u.max() < 1
+    assert abs(u.mean() - 0.5) < 0.01
+    z = orc.complex_randn(50000)
+    assert abs(z.real.mean()) < 0.02 and abs(z.imag.std() - 1) < 0.02
+    # Box-Muller pairs: |z|^2 = -2 log u0 is exponential with mean 2
+    assert abs(np.mean(np.abs(z) ** 2) - 2) < 0.05
+
+
+def test_h0_matches_mpmath_table():
+    tab = json.load(open(os.path.join(HERE, "golden", "h0_table.json")))["rows"]
+    x = np.array([float(r[0]) for r in tab])
+    want = np.array([complex(float(r[1]), float(r[2])) for r in tab])
+    got = orc.h0(x)
+    err = np.abs(got - want) / np.abs(want)
+    # glibc j0/y0: a few ulps, plus the phase conditioning x*eps at large x
+    assert np.all(err <= 1e-14 + 4e-16 * x), (x[np.argmax(err)], err.max())
+
+
+def test_kernel_matrix_closed_form_and_zero_distance():
+    k = 7.5
+    tgt = np.array([[0.0, 0.0], [1.0, 2.0], [0.5, 0.5]])
+    src = np.array([[0.0, 0.0], [3.0, -1.0]])
+    K = orc.kernel_matrix(k, tgt, src)
+    assert K[0, 0] == 0  # r == 0 gives 0 (helm2.c:114)
+    for i in range(3):
+        for j in range(2):
+            r = float(np.hypot(*(src[j] - tgt[i])))
+            if r == 0:
+                continue
+            want = complex(1j * mp.hankel1(0, k * r) / 4)
+            assert abs(K[i, j] - want) < 1e-14 * abs(want)
+
+
+def test_rank_estimate_closed_form():
+    L = orc.lib()
+    # p = ceil(k r1 r2 / (R - r1 - r2) - log10(eps))
+    for k, r1, r2, R, eps in [(100.0, 0.3, 0.2, 1.5, 1e-15), (73064.8, 0.01, 0.02, 0.5, 1e-8)]:
+        want = int(np.ceil(k * r1 * r2 / (R - r1 - r2) - np.log10(eps)))
+        assert L.orc_rank_est(k, r1, 0.0, 0.0, r2, R, 0.0, eps) == want
+
+
+def test_circle_points_closed_form():
+    P = orc.circle_points(0.25, -1.0, 0.5, 7)
+    th = 2 * np.pi / 7 * np.arange(7)
+    np.testing.assert_allclose(P, np.stack([0.5 * np.cos(th) + 0.25, 0.5 * np.sin(th) - 1.0], 1), atol=1e-15)
+
+
+@pytest.mark.parametrize("p,n", [(8, 5), (16, 40), (33, 20)])
+def test_lstsq_well_conditioned_equals_numpy_solve(p, n):
+    rng = np.random.default_rng(p)
+    A = rng.standard_normal((p, p)) + 1j * rng.standard_normal((p, p))
+    B = rng.standard_normal((p, n)) + 1j * rng.standard_normal((p, n))
+    X, kept = orc.lstsq(A, B)
+    assert kept == p
+    np.testing.assert_allclose(X, np.linalg.solve(A, B), rtol=0, atol=1e-10 * np.abs(np.linalg.solve(A, B)).max())
+
+
+def test_lstsq_truncation_matches_numpy_pinv_rule():
+    # an exactly rank-deficient A: pinv keeps the 10 nonzero singular values
+    rng = np.random.default_rng(3)
+    U, _ = np.linalg.qr(rng.standard_normal((20, 20)) + 1j * rng.standard_normal((20, 20)))
+    V, _ = np.linalg.qr(rng.standard_normal((20, 20)) + 1j * rng.standard_normal((20, 20)))
+    s = np.r_[np.geomspace(1, 1e-6, 10), np.zeros(10)]
+    A = (U * s) @ V.conj().T
+    B = rng.standard_normal((20, 4)) + 1j * rng.standard_normal((20, 4))
+    X, kept = orc.lstsq(A, B)
+    assert kept == 10
+    # numpy's pinv with rcond = max(m,n) eps (sigma_0 = 1 here, atol eps)
+    Xn = np.linalg.pinv(A, rcond=20 * np.finfo(float).eps) @ B
+    np.testing.assert_allclose(X, Xn, atol=1e-6)
+
+
+def test_svd_values_match_numpy():
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((12, 12)) + 1j * rng.standard_normal((12, 12))
+    np.testing.assert_allclose(orc.svd_values(A), np.linalg.svd(A, compute_uv=False), rtol=1e-13)
+
+
+def test_quadtree_invariants():
+    src, _, _ = box_problem(3000)
+    for leaf in (1, 16):
+        perm, nodes, bbox = orc.tree(src, leaf)
+        assert np.array_equal(np.sort(perm), np.arange(len(src)))
+        P = src[perm]
+        for i, (depth, i0, i1, parent, *kids) in enumerate(nodes):
+            pts = P[i0:i1]
+            assert np.all(pts >= bbox[i, :2]) and np.all(pts <= bbox[i, 2:])
+            live = [c for c in kids if c >= 0]
+            if live:
+                assert nodes[live[0]][1] == i0 and nodes[live[-1]][2] == i1
+                assert sum(nodes[c][2] - nodes[c][1] for c in live) == i1 - i0
+            else:
+                assert i1 - i0 <= leaf or depth == 0
+            # Z-order: quadrant q = 2 [x > sx] + [y > sy], ties go low
+            for q, c in enumerate(kids):
+                if c < 0:
+                    continue
+                sx, sy = (bbox[i, 0] + bbox[i, 2]) / 2, (bbox[i, 1] + bbox[i, 3]) / 2
+                cp = P[nodes[c][1]:nodes[c][2]]
+                assert np.all((cp[:, 0] > sx) == bool(q >> 1)) and np.all((cp[:, 1] > sy) == bool(q & 1))
+        # root box is the bounding box stretched to a square
+        assert abs((bbox[0, 2] - bbox[0, 0]) - (bbox[0, 3] - bbox[0, 1])) < 1e-15
+
+
+def test_oracle_apply_equals_dense_rows_reference_rules():
+    src, tgt, k = box_problem(2048)
+    F = orc.Fac(k, src, tgt)
+    sp, tp = F.perms()
+    orc.seed(1)
+    x = orc.complex_randn(len(src))
+    y = np.empty(len(tgt), complex)
+    y[tp] = F.apply(x[sp])[:, 0]
+    rows = np.arange(0, len(tgt), 37)
+    yd = orc.dense_rows(k, tgt[rows], src, x)
+    assert np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd) < 1e-11
+
+
+def test_oracle_butterfly_leaf_reproduces_its_kernel_block():
+    # densify one butterfly by applying it to unit vectors and compare with
+    # the exact kernel block between its target and source nodes
+    src, tgt, k = box_problem(4096)
+    F = orc.Fac(k, src, tgt)
+    lt, ft, bt = F.plan()
+    assert lt[:, 0].all() and lt[0, 5] >= 3
+    sp, tp = F.perms()
+    row0, col0, m, n = lt[0, 1:5]
+    cols = np.arange(col0, col0 + n, max(1, n // 12))
+    X = np.zeros((len(src), len(cols)), complex)
+    X[cols, np.arange(len(cols))] = 1
+    Y = F.apply(X)[row0:row0 + m]
+    K = orc.kernel_matrix(k, tgt[tp[row0:row0 + m]], src[sp[cols]])
+    assert np.linalg.norm(Y - K) / np.linalg.norm(K) < 1e-11
+
+
+def test_oracle_pass_through_curve_accuracy():
+    src, tgt, k = curve_problem(4096)
+    F = orc.Fac(k, src, tgt, leaf_size=64, tol=1e-10, pass_through=True)
+    sp, tp = F.perms()
+    orc.seed(2)
+    x = orc.complex_randn(len(src))
+    y = np.empty(len(tgt), complex)
+    y[tp] = F.apply(x[sp])[:, 0]
+    rows = np.arange(0, len(tgt), 61)
+    yd = orc.dense_rows(k, tgt[rows], src, x)
+    assert np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd) < 1e-6
