@@ -121,6 +121,7 @@ def lib():
         "bfhipFacPlanExport": (None, [vp, i64p, i64p, i64p]),
         "bfhipFacBlockData": (C.c_int, [vp, sz, dp]),
         "bfhipFacDenseData": (C.c_int, [vp, sz, dp]),
+        "bfhipFacBlockGeometry": (C.c_int, [vp, sz, dp]),
         "bfhipHankel0": (C.c_int, [sz, dp, dp]),
         "bfhipLstSqBatched": (C.c_int, [sz, sz, sz, dp, dp, dp]),
     }
@@ -302,6 +303,11 @@ class Factorization(Mat):
             check()
             raise BfError("block copy failed")
         return out.view(np.complex128).reshape(m, n)
+
+    def block_geometry(self, idx):
+        out = np.zeros(12)
+        lib().bfhipFacBlockGeometry(self.ptr, idx, _dp(out))
+        return out
 
     def dense_leaf(self, idx, m, n):
         out = np.zeros(2 * m * n)

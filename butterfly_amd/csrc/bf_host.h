@@ -58,7 +58,7 @@ typedef struct {
   int64_t dataOff;   /* complex entries into the arena, -1 for identity */
   int32_t rowInd, colInd, m, n;
   int32_t kind;
-  int32_t pad;
+  int32_t job;       /* index into lstsqJobs or kernelJobs, -1 for identity */
 } BfHipBlock;
 
 typedef struct {

@@ -216,14 +216,19 @@ __device__ __forceinline__ size_t lstsq_lds_bytes(int p, int n) {
   return ((size_t)p * (p + n + 1) + p + 2 * p + n) * sizeof(double2);
 }
 
+/* One wave per job.  The working set (M, weights, point lists) lives in
+ * LDS; jobs too large for LDS (a few per million at the largest sizes)
+ * run the same code on a per-block slice of a global scratch buffer. */
+template <bool GLOBAL>
 __global__ __launch_bounds__(64) void k_lstsq(const BfkLstsqJob *__restrict__ jobs,
                                               const int32_t *__restrict__ order,
                                               const double2 *__restrict__ srcXY, double k,
-                                              double2 *__restrict__ arena) {
+                                              double2 *__restrict__ arena, double2 *__restrict__ gwork,
+                                              int64_t gstride) {
   extern __shared__ double2 lds[];
   const BfkLstsqJob J = jobs[order[blockIdx.x]];
   const int p = J.p, n = J.n, ld = p + n + 1;
-  double2 *M = lds;
+  double2 *M = GLOBAL ? gwork + (int64_t)blockIdx.x * gstride : lds;
   double2 *w = M + (size_t)p * ld;
   double2 *tp = w + p;     /* test points, p */
   double2 *ep = tp + p;    /* equiv points, p */
@@ -380,7 +385,7 @@ int bfk_init(void) {
   BFK_CHECK(hipGetDeviceProperties(&prop, 0));
   g_ldsCap = prop.sharedMemPerBlock;
   if (g_ldsCap > 65536) {
-    if (hipFuncSetAttribute((const void *)k_lstsq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_ldsCap) !=
+    if (hipFuncSetAttribute((const void *)k_lstsq<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_ldsCap) !=
             hipSuccess ||
         hipFuncSetAttribute((const void *)k_lstsq_loaded, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)g_ldsCap) != hipSuccess) {
@@ -503,14 +508,27 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
     int64_t s = groupStart[gi], e = groupStart[gi + 1];
     if (e <= s) continue;
     if (groupBytes[gi] > g_ldsCap) {
-      fprintf(stderr, "bf_kernels: re-expansion block needs %zu B of LDS (cap %zu)\n", groupBytes[gi], g_ldsCap);
-      return -3;
+      /* global scratch: at most `chunk` blocks in flight per launch */
+      const int64_t stride = (int64_t)(groupBytes[gi] / sizeof(double2));
+      const int64_t chunk = 2048;
+      double2 *work = nullptr;
+      BFK_CHECK(hipMalloc(&work, (size_t)chunk * groupBytes[gi]));
+      for (int64_t j0 = s; j0 < e; j0 += chunk) {
+        int64_t nj = e - j0 < chunk ? e - j0 : chunk;
+        hipLaunchKernelGGL(k_lstsq<true>, dim3((unsigned)nj), dim3(64), 0, g_stream, dJobs, dOrder + j0,
+                           (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride);
+        BFK_CHECK(hipGetLastError());
+      }
+      BFK_CHECK(hipStreamSynchronize(g_stream));
+      BFK_CHECK(hipFree(work));
+      continue;
     }
     const int64_t maxGrid = 1 << 30;
     for (int64_t j0 = s; j0 < e; j0 += maxGrid) {
       int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
-      hipLaunchKernelGGL(k_lstsq, dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs, dOrder + j0,
-                         (const double2 *)dSrcXY, k, (double2 *)dArena);
+      hipLaunchKernelGGL(k_lstsq<false>, dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs,
+                         dOrder + j0, (const double2 *)dSrcXY, k, (double2 *)dArena, (double2 *)nullptr,
+                         (int64_t)0);
       BFK_CHECK(hipGetLastError());
     }
   }

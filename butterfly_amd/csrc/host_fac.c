@@ -636,6 +636,25 @@ int bfhipFacBlockData(BfMat const *fac, BfSize idx, BfComplex *out) {
   return copy_colmajor_to_rowmajor(b->dataOff, b->m, b->n, F->dArena, out);
 }
 
+int bfhipFacBlockGeometry(BfMat const *fac, BfSize idx, BfReal out[12]) {
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (idx >= F->blocks.n) return -1;
+  BfHipBlock *b = &F->blocks.a[idx];
+  for (int i = 0; i < 12; ++i) out[i] = 0;
+  if (b->kind == BF_BLOCK_LSTSQ) {
+    BfkLstsqJob *J = &F->lstsqJobs.a[b->job];
+    double v[12] = {J->ecx, J->ecy, J->er, J->tcx, J->tcy, J->tr, J->ocx, J->ocy, J->orr,
+                    (double)J->origOff, (double)J->p, (double)J->n};
+    memcpy(out, v, sizeof(v));
+  } else if (b->kind == BF_BLOCK_KERNEL) {
+    BfkKernelJob *J = &F->kernelJobs.a[b->job];
+    double v[12] = {J->scx, J->scy, J->sr, 0, 0, 0, 0, 0, 0, (double)J->srcOff, (double)J->m, (double)J->n};
+    memcpy(out, v, sizeof(v));
+    out[3] = (double)J->tgtOff;
+  }
+  return 0;
+}
+
 int bfhipFacDenseData(BfMat const *fac, BfSize leafIdx, BfComplex *out) {
   BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
   if (leafIdx >= F->leaves.n || F->leaves.a[leafIdx].isButterfly) return -1;

@@ -284,7 +284,16 @@ int bfhipHankel0(BfSize n, BfReal const *x, BfReal *out) {
 }
 
 int bfhipLstSqBatched(BfSize batch, BfSize p, BfSize n, BfComplex const *A, BfComplex const *B, BfComplex *X) {
-  int rc = bfk_lstsq_loaded((int64_t)batch, (int)p, (int)n, (const double *)A, (const double *)B, (double *)X);
+  /* the solver writes each X column-major (the apply's block layout) */
+  BfComplex *colMajor = malloc((batch * p * n > 0 ? batch * p * n : 1) * sizeof(BfComplex));
+  if (!colMajor) { bfSetError(BF_ERROR_MEMORY_ERROR); return -2; }
+  int rc = bfk_lstsq_loaded((int64_t)batch, (int)p, (int)n, (const double *)A, (const double *)B,
+                            (double *)colMajor);
+  if (!rc)
+    for (BfSize b = 0; b < batch; ++b)
+      for (BfSize i = 0; i < p; ++i)
+        for (BfSize j = 0; j < n; ++j) X[(b * p + i) * n + j] = colMajor[b * p * n + j * p + i];
+  free(colMajor);
   if (rc == -3) bfSetError(BF_ERROR_NOT_IMPLEMENTED);
   else if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
   return rc;

@@ -94,6 +94,7 @@ static void push_kernel_job(Planner *P, BfHipBlock *b, const TreeView *sv, int32
   J.tgtOff = (int64_t)node(&P->tgt, t)->offset[0];
   J.m = b->m;
   J.n = (int32_t)ncols;
+  b->job = (int32_t)F->kernelJobs.n;
   b->dataOff = alloc_arena(F, (int64_t)b->m * b->n);
   J.out = b->dataOff;
   BF_VEC_PUSH(F->kernelJobs, J);
@@ -112,6 +113,7 @@ static void push_lstsq_job(Planner *P, BfHipBlock *b, int32_t e, int32_t o, int 
   J.origOff = origIsPoints ? (int64_t)node(&P->src, o)->offset[0] : -1;
   J.p = b->m;
   J.n = b->n;
+  b->job = (int32_t)F->lstsqJobs.n;
   b->dataOff = alloc_arena(F, (int64_t)b->m * b->n);
   J.out = b->dataOff;
   BF_VEC_PUSH(F->lstsqJobs, J);
@@ -166,6 +168,7 @@ static void make_butterfly(Planner *P, BfHipLeaf *leaf, Level *Ls, Level *Lt, in
       b.m = (int32_t)ns;
       b.kind = BF_BLOCK_IDENTITY;
       b.dataOff = -1;
+      b.job = -1;
     } else {
       mode[i] = MODE_PROXY;
       b.m = (int32_t)p;
@@ -244,6 +247,7 @@ static void make_butterfly(Planner *P, BfHipLeaf *leaf, Level *Ls, Level *Lt, in
             if ((cm == MODE_CHARGES && rm == MODE_CHARGES) || (cm == MODE_VALUES && rm == MODE_VALUES)) {
               b.kind = BF_BLOCK_IDENTITY;
               b.dataOff = -1;
+              b.job = -1;
             } else if (rm == MODE_VALUES) {
               b.kind = BF_BLOCK_KERNEL;
               push_kernel_job(P, &b, sv, skids[c], cm == MODE_CHARGES, b.n, tc);
@@ -284,6 +288,7 @@ static void make_butterfly(Planner *P, BfHipLeaf *leaf, Level *Ls, Level *Lt, in
     if (mode[i] == MODE_VALUES) {
       b.kind = BF_BLOCK_IDENTITY;
       b.dataOff = -1;
+      b.job = -1;
     } else {
       b.kind = BF_BLOCK_KERNEL;
       push_kernel_job(P, &b, sv, S, mode[i] == MODE_CHARGES, b.n, t);

@@ -349,6 +349,11 @@ void bfhipFacPlanExport(BfMat const *fac, int64_t *leafTab, int64_t *facTab, int
 /* copy block `idx` (export order) as m x n row-major complex FP64 */
 int bfhipFacBlockData(BfMat const *fac, BfSize idx, BfComplex *out);
 int bfhipFacDenseData(BfMat const *fac, BfSize leafIdx, BfComplex *out);
+/* geometry of block `idx`: for a re-expansion block
+ * [equiv cx, cy, r, test cx, cy, r, orig cx, cy, r, origOff (-1 = circle), p, n];
+ * for a kernel block [src cx, cy, r, tgtOff, 0.., srcOff (-1 = circle), m, n]
+ * (offsets index the tree-ordered points) */
+int bfhipFacBlockGeometry(BfMat const *fac, BfSize idx, BfReal out[12]);
 
 /* new: device H0^(1) on n arguments (host arrays), out = [J0, Y0] pairs */
 int bfhipHankel0(BfSize n, BfReal const *x, BfReal *out);
