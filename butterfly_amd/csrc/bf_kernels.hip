@@ -103,153 +103,272 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *_
 
 /* -------------------------------------------------------- compression */
 
+/* Per-job LDS (or scratch) layout, in 16-byte units:
+ *   M    p x ld   row-major [Z_equiv | Z_orig], ld = p + n + 1 (odd pad)
+ *   w    p        x = sigma^2, y = 1/sigma^2 or 0 (truncated)
+ *   v    p        Householder vector
+ *   pts  2p + n   test, equiv, orig points (sampling only)
+ *   red  8        block reductions
+ *   piv  p int32  column pivots (p/4 units rounded up) */
+__host__ __device__ __forceinline__ size_t lstsq_units(int p, int n) {
+  return (size_t)p * (p + n + 1) + p + p + (2 * p + n) + 8 + (p + 3) / 4;
+}
+
 /* Round-robin pairing: position 0 is fixed, the others rotate by `step`. */
 __device__ __forceinline__ int rr_row(int pos, int step, int P) {
   return pos == 0 ? 0 : 1 + (pos - 1 + step) % (P - 1);
 }
 
-/* LDS layout for one job: M (p x ld) row-major, ld = p + n + 1 (odd pad),
- * then weights w[p] as double2 (x = 1/sigma^2). */
-__device__ __forceinline__ void jacobi_rows_and_solve(double2 *M, int p, int n, int ld, double2 *w,
-                                                      double2 *__restrict__ out) {
+template <int T>
+__device__ __forceinline__ bool block_any(int v) {
+  if (T == 64) return __any(v);
+  return __syncthreads_or(v);
+}
+
+/* max of v over the T threads of the block, with the index of the first
+ * maximal entry (smallest index on ties) */
+template <int T>
+__device__ __forceinline__ void block_argmax(double &v, int &idx, double *red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o, 64);
+    int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  if (T > 64) {
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) { red[2 * wv] = v; red[2 * wv + 1] = (double)idx; }
+    __syncthreads();
+    v = red[0];
+    idx = (int)red[1];
+    for (int q = 1; q < T / 64; ++q) {
+      double ov = red[2 * q];
+      int oi = (int)red[2 * q + 1];
+      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    __syncthreads();
+  }
+}
+
+/* X = Z_equiv^+ Z_orig for M = [Z_equiv | Z_orig] (p x (p+n)):
+ *  1. Householder QR of Z_equiv with column pivoting, Z_equiv P = Q R,
+ *     applied to all of M, giving [R | Q^H Z_orig];
+ *  2. one-sided Jacobi on the rows of R (rotations applied to whole rows)
+ *     until they are orthogonal: rows become sigma_i v_i^H, the right part
+ *     becomes U^H Q^H Z_orig (the QR step makes the graded rows converge
+ *     in a few sweeps);
+ *  3. keep sigma_i >= p eps sigma_0 + eps (mat_dense_complex.c:1797-1809)
+ *     and form X = P sum_i R'_i^H B'_i / sigma_i^2, written column-major. */
+template <int T>
+__device__ void qr_jacobi_solve(double2 *M, int p, int n, int ld, double2 *w, double2 *v, double *red, int *piv,
+                                double2 *__restrict__ out) {
   const int lane = threadIdx.x;
   const int W = p + n;
+  const double eps = 2.220446049250313e-16;
+
+  for (int i = lane; i < p; i += T) piv[i] = i;
+  __syncthreads();
+
+  /* ---- 1. pivoted Householder QR */
+  for (int j = 0; j < p; ++j) {
+    double best = -1.0;
+    int bidx = p;
+    for (int c = j + lane; c < p; c += T) {
+      double s = 0;
+      for (int r = j; r < p; ++r) {
+        double2 a = M[r * ld + c];
+        s = fma(a.x, a.x, fma(a.y, a.y, s));
+      }
+      if (s > best) { best = s; bidx = c; }
+    }
+    block_argmax<T>(best, bidx, red);
+    const int q = bidx;
+    if (q != j) {
+      for (int r = lane; r < p; r += T) {
+        double2 t = M[r * ld + j];
+        M[r * ld + j] = M[r * ld + q];
+        M[r * ld + q] = t;
+      }
+      if (lane == 0) { int t = piv[j]; piv[j] = piv[q]; piv[q] = t; }
+    }
+    __syncthreads();
+    const double sig = sqrt(best);
+    if (sig > 0) {
+      const double2 x0 = M[j * ld + j];
+      const double ax0 = hypot(x0.x, x0.y);
+      const double phx = ax0 > 0 ? x0.x / ax0 : 1.0, phy = ax0 > 0 ? x0.y / ax0 : 0.0;
+      for (int r = j + lane; r < p; r += T) {
+        double2 a = M[r * ld + j];
+        if (r == j) a = make_double2(x0.x + phx * sig, x0.y + phy * sig);
+        v[r] = a;
+      }
+      __syncthreads();
+      const double inv = 1.0 / (sig * (sig + ax0));
+      for (int c = j + 1 + lane; c < W; c += T) {
+        double sr = 0, si = 0; /* v^H y */
+        for (int r = j; r < p; ++r) {
+          double2 vr = v[r], y = M[r * ld + c];
+          sr = fma(vr.x, y.x, fma(vr.y, y.y, sr));
+          si = fma(vr.x, y.y, fma(-vr.y, y.x, si));
+        }
+        sr *= inv;
+        si *= inv;
+        for (int r = j; r < p; ++r) {
+          double2 vr = v[r], y = M[r * ld + c];
+          y.x -= sr * vr.x - si * vr.y;
+          y.y -= sr * vr.y + si * vr.x;
+          M[r * ld + c] = y;
+        }
+      }
+      __syncthreads();
+      for (int r = j + lane; r < p; r += T)
+        M[r * ld + j] = (r == j) ? make_double2(-phx * sig, -phy * sig) : make_double2(0, 0);
+    }
+    __syncthreads();
+  }
+
+  /* ---- 2. one-sided Jacobi on the rows of R */
   const int P = p + (p & 1);
   const int npairs = P / 2;
   int Lp = 64;
-  while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
-  const int pairsPerPass = 64 / Lp;
+  while (Lp > 1 && Lp * npairs > T) Lp >>= 1;
+  const int pairsPerPass = T / Lp;
   const int slot = lane / Lp;
   const int sub = lane - slot * Lp;
-  const double thr = 4.0 * 2.220446049250313e-16 * sqrt((double)p);
-
+  const double thr = 4.0 * eps * sqrt((double)p);
   for (int sweep = 0; sweep < 40; ++sweep) {
     int rotated = 0;
     for (int step = 0; step < P - 1; ++step) {
       for (int pb = 0; pb < npairs; pb += pairsPerPass) {
-        int pr = pb + slot;
+        const int pr = pb + slot;
         int i = -1, j = -1;
         if (pr < npairs) {
           int a = rr_row(pr, step, P), b = rr_row(P - 1 - pr, step, P);
           i = min(a, b);
           j = max(a, b);
         }
-        bool valid = pr < npairs && j < p;
-        double al = 0, be = 0;
-        double2 g = make_double2(0, 0);
+        const bool valid = pr < npairs && j < p;
+        double al = 0, be = 0, gx = 0, gy = 0;
         if (valid) {
           const double2 *ri = M + i * ld, *rj = M + j * ld;
           for (int c = sub; c < p; c += Lp) {
             double2 x = ri[c], y = rj[c];
             al = fma(x.x, x.x, fma(x.y, x.y, al));
             be = fma(y.x, y.x, fma(y.y, y.y, be));
-            /* g += x * conj(y) */
-            g.x = fma(x.x, y.x, fma(x.y, y.y, g.x));
-            g.y = fma(x.y, y.x, fma(-x.x, y.y, g.y));
+            gx = fma(x.x, y.x, fma(x.y, y.y, gx)); /* x conj(y) */
+            gy = fma(x.y, y.x, fma(-x.x, y.y, gy));
           }
         }
         for (int o = 1; o < Lp; o <<= 1) {
           al += __shfl_xor(al, o, 64);
           be += __shfl_xor(be, o, 64);
-          g.x += __shfl_xor(g.x, o, 64);
-          g.y += __shfl_xor(g.y, o, 64);
+          gx += __shfl_xor(gx, o, 64);
+          gy += __shfl_xor(gy, o, 64);
         }
-        double ag = hypot(g.x, g.y);
-        if (valid && ag > 0 && ag > thr * sqrt(al * be)) {
+        const double ag2 = gx * gx + gy * gy;
+        if (valid && ag2 > 0 && ag2 > thr * thr * al * be) {
           rotated = 1;
-          double ex = g.x / ag, ey = g.y / ag; /* e = g/|g| */
-          double zeta = (be - al) / (2 * ag);
-          double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
-          double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+          const double ag = sqrt(ag2);
+          const double ex = gx / ag, ey = gy / ag;
+          const double zeta = (be - al) / (2 * ag);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+          const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
           double2 *ri = M + i * ld, *rj = M + j * ld;
           for (int c = sub; c < W; c += Lp) {
             double2 x = ri[c], y = rj[c];
-            double2 ey_ = make_double2(ex * y.x - ey * y.y, ex * y.y + ey * y.x); /* e*y */
-            ri[c] = make_double2(cs * x.x - sn * ey_.x, cs * x.y - sn * ey_.y);
-            rj[c] = make_double2(sn * x.x + cs * ey_.x, sn * x.y + cs * ey_.y);
+            double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x; /* e y */
+            ri[c] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
+            rj[c] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
           }
         }
         __syncthreads();
       }
     }
-    if (!__any(rotated)) break;
+    if (!block_any<T>(rotated)) break;
   }
 
-  /* sigma_i^2 = ||row i of the rotated Z_equiv||^2 */
+  /* ---- 3. truncation and X */
   double smax = 0;
-  for (int i = lane; i < p; i += 64) {
+  for (int i = lane; i < p; i += T) {
     const double2 *ri = M + i * ld;
     double s = 0;
     for (int c = 0; c < p; ++c) s = fma(ri[c].x, ri[c].x, fma(ri[c].y, ri[c].y, s));
     w[i].x = s;
     smax = fmax(smax, s);
   }
-  for (int o = 1; o < 64; o <<= 1) smax = fmax(smax, __shfl_xor(smax, o, 64));
+  int dummy = 0;
+  block_argmax<T>(smax, dummy, red);
   __syncthreads();
-  /* keep sigma >= max(m,n) eps sigma_0 + eps (mat_dense_complex.c:1797-1809) */
-  const double sig0 = sqrt(smax);
-  const double tol = (double)p * 2.220446049250313e-16 * sig0 + 2.220446049250313e-16;
-  for (int i = lane; i < p; i += 64) {
+  const double tol = (double)p * eps * sqrt(smax) + eps;
+  for (int i = lane; i < p; i += T) {
     double s = sqrt(w[i].x);
     w[i].y = (s >= tol) ? 1.0 / w[i].x : 0.0;
   }
   __syncthreads();
-  /* X = sum_i conj(A'_i)^T w_i B'_i, column-major p x n */
   const int cnt = p * n;
-  for (int e = lane; e < cnt; e += 64) {
-    int c = e / p;
-    int a = e - c * p;
-    double2 acc = make_double2(0, 0);
+  for (int e = lane; e < cnt; e += T) {
+    const int c = e / p;
+    const int a = e - c * p;
+    double ax = 0, ay = 0;
     for (int i = 0; i < p; ++i) {
-      double wi = w[i].y;
+      const double wi = w[i].y;
       if (wi == 0) continue;
       double2 x = M[i * ld + a], y = M[i * ld + p + c];
-      /* conj(x) * y * wi */
-      acc.x = fma(wi, x.x * y.x + x.y * y.y, acc.x);
-      acc.y = fma(wi, x.x * y.y - x.y * y.x, acc.y);
+      ax = fma(wi, x.x * y.x + x.y * y.y, ax); /* conj(x) y */
+      ay = fma(wi, x.x * y.y - x.y * y.x, ay);
     }
-    out[e] = acc;
+    out[(size_t)c * p + piv[a]] = make_double2(ax, ay);
   }
 }
 
-__device__ __forceinline__ size_t lstsq_lds_bytes(int p, int n) {
-  return ((size_t)p * (p + n + 1) + p + 2 * p + n) * sizeof(double2);
+struct LstsqView {
+  double2 *M, *w, *v, *pts;
+  double *red;
+  int *piv;
+};
+
+__device__ __forceinline__ LstsqView lstsq_view(double2 *base, int p, int n) {
+  LstsqView L;
+  const int ld = p + n + 1;
+  L.M = base;
+  L.w = L.M + (size_t)p * ld;
+  L.v = L.w + p;
+  L.pts = L.v + p;
+  L.red = (double *)(L.pts + 2 * p + n);
+  L.piv = (int *)(L.pts + 2 * p + n + 8);
+  return L;
 }
 
-/* One wave per job.  The working set (M, weights, point lists) lives in
- * LDS; jobs too large for LDS (a few per million at the largest sizes)
- * run the same code on a per-block slice of a global scratch buffer. */
-template <bool GLOBAL>
-__global__ __launch_bounds__(64) void k_lstsq(const BfkLstsqJob *__restrict__ jobs,
-                                              const int32_t *__restrict__ order,
-                                              const double2 *__restrict__ srcXY, double k,
-                                              double2 *__restrict__ arena, double2 *__restrict__ gwork,
-                                              int64_t gstride) {
+/* One job per block of T threads.  The working set lives in LDS, or for
+ * jobs too large for LDS (about one in 10^4 at N = 2^20) in a per-block
+ * slice of a global scratch buffer. */
+template <int T, bool GLOBAL>
+__global__ __launch_bounds__(T) void k_lstsq(const BfkLstsqJob *__restrict__ jobs, const int32_t *__restrict__ order,
+                                             const double2 *__restrict__ srcXY, double k,
+                                             double2 *__restrict__ arena, double2 *__restrict__ gwork,
+                                             int64_t gstride) {
   extern __shared__ double2 lds[];
   const BfkLstsqJob J = jobs[order[blockIdx.x]];
   const int p = J.p, n = J.n, ld = p + n + 1;
-  double2 *M = GLOBAL ? gwork + (int64_t)blockIdx.x * gstride : lds;
-  double2 *w = M + (size_t)p * ld;
-  double2 *tp = w + p;     /* test points, p */
-  double2 *ep = tp + p;    /* equiv points, p */
-  double2 *op = ep + p;    /* orig points, n */
+  LstsqView L = lstsq_view(GLOBAL ? gwork + (int64_t)blockIdx.x * gstride : lds, p, n);
+  double2 *tp = L.pts, *ep = tp + p, *op = ep + p;
   const int lane = threadIdx.x;
-  for (int i = lane; i < p; i += 64) {
+  for (int i = lane; i < p; i += T) {
     tp[i] = circle_pt(J.tcx, J.tcy, J.tr, p, i);
     ep[i] = circle_pt(J.ecx, J.ecy, J.er, p, i);
   }
-  for (int c = lane; c < n; c += 64)
+  for (int c = lane; c < n; c += T)
     op[c] = J.origOff >= 0 ? srcXY[J.origOff + c] : circle_pt(J.ocx, J.ocy, J.orr, n, c);
   __syncthreads();
   const int W = p + n;
-  for (int e = lane; e < p * W; e += 64) {
+  for (int e = lane; e < p * W; e += T) {
     int i = e / W;
     int c = e - i * W;
     double2 s = c < p ? ep[c] : op[c - p];
-    M[i * ld + c] = green(k, tp[i], s);
+    L.M[i * ld + c] = green(k, tp[i], s);
   }
   __syncthreads();
-  jacobi_rows_and_solve(M, p, n, ld, w, arena + J.out);
+  qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, arena + J.out);
 }
 
 /* the same solve on matrices given in global memory (row-major A p x p,
@@ -258,18 +377,17 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
                                                      int p, int n, double2 *__restrict__ X) {
   extern __shared__ double2 lds[];
   const int ld = p + n + 1;
-  double2 *M = lds;
-  double2 *w = M + (size_t)p * ld;
+  LstsqView L = lstsq_view(lds, p, n);
   const double2 *a = A + (size_t)blockIdx.x * p * p;
   const double2 *b = B + (size_t)blockIdx.x * p * n;
   const int W = p + n;
   for (int e = threadIdx.x; e < p * W; e += 64) {
     int i = e / W;
     int c = e - i * W;
-    M[i * ld + c] = c < p ? a[i * p + c] : b[i * n + (c - p)];
+    L.M[i * ld + c] = c < p ? a[i * p + c] : b[i * n + (c - p)];
   }
   __syncthreads();
-  jacobi_rows_and_solve(M, p, n, ld, w, X + (size_t)blockIdx.x * p * n);
+  qr_jacobi_solve<64>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, X + (size_t)blockIdx.x * p * n);
 }
 
 /* ---------------------------------------------------------------- apply */
@@ -385,8 +503,8 @@ int bfk_init(void) {
   BFK_CHECK(hipGetDeviceProperties(&prop, 0));
   g_ldsCap = prop.sharedMemPerBlock;
   if (g_ldsCap > 65536) {
-    if (hipFuncSetAttribute((const void *)k_lstsq<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_ldsCap) !=
-            hipSuccess ||
+    if (hipFuncSetAttribute((const void *)k_lstsq<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_ldsCap) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_lstsq_loaded, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)g_ldsCap) != hipSuccess) {
       (void)hipGetLastError();
@@ -404,9 +522,7 @@ int bfk_device_count(void) {
 
 size_t bfk_lds_cap(void) { return g_ldsCap; }
 
-size_t bfk_lstsq_lds_bytes(int p, int n) {
-  return ((size_t)p * (p + n + 1) + p + 2 * p + n) * 16;
-}
+size_t bfk_lstsq_lds_bytes(int p, int n) { return lstsq_units(p, n) * 16; }
 
 void *bfk_malloc(size_t bytes) {
   if (bfk_init()) return nullptr;
@@ -499,38 +615,48 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
 
 /* re-expansion blocks; `dOrder` lists job indices grouped by LDS need,
  * `groupStart/groupBytes` give [numGroups] contiguous ranges of dOrder
- * and the dynamic LDS each range needs */
+ * and the bytes each job of the range needs.  Groups that fit in LDS run
+ * one wave per job; the rest run 256 threads per job on global scratch,
+ * all of them in one launch per chunk of 4096 jobs. */
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
               const size_t *groupBytes, const double *dSrcXY, double k, double *dArena) {
   if (bfk_init()) return -1;
   if (time_begin()) return -1;
+  int64_t gs = -1, ge = -1;
+  size_t gmax = 0;
   for (int gi = 0; gi < numGroups; ++gi) {
     int64_t s = groupStart[gi], e = groupStart[gi + 1];
     if (e <= s) continue;
     if (groupBytes[gi] > g_ldsCap) {
-      /* global scratch: at most `chunk` blocks in flight per launch */
-      const int64_t stride = (int64_t)(groupBytes[gi] / sizeof(double2));
-      const int64_t chunk = 2048;
-      double2 *work = nullptr;
-      BFK_CHECK(hipMalloc(&work, (size_t)chunk * groupBytes[gi]));
-      for (int64_t j0 = s; j0 < e; j0 += chunk) {
-        int64_t nj = e - j0 < chunk ? e - j0 : chunk;
-        hipLaunchKernelGGL(k_lstsq<true>, dim3((unsigned)nj), dim3(64), 0, g_stream, dJobs, dOrder + j0,
-                           (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride);
-        BFK_CHECK(hipGetLastError());
-      }
-      BFK_CHECK(hipStreamSynchronize(g_stream));
-      BFK_CHECK(hipFree(work));
+      /* groups are sorted by size, so the oversized ones are contiguous */
+      if (gs < 0) gs = s;
+      ge = e;
+      if (groupBytes[gi] > gmax) gmax = groupBytes[gi];
       continue;
     }
     const int64_t maxGrid = 1 << 30;
     for (int64_t j0 = s; j0 < e; j0 += maxGrid) {
       int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
-      hipLaunchKernelGGL(k_lstsq<false>, dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs,
+      hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs,
                          dOrder + j0, (const double2 *)dSrcXY, k, (double2 *)dArena, (double2 *)nullptr,
                          (int64_t)0);
       BFK_CHECK(hipGetLastError());
     }
+  }
+  if (gs >= 0) {
+    const int64_t stride = (int64_t)(gmax / sizeof(double2));
+    const int64_t chunk = 4096;
+    const int64_t nbig = ge - gs;
+    double2 *work = nullptr;
+    BFK_CHECK(hipMalloc(&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax));
+    for (int64_t j0 = gs; j0 < ge; j0 += chunk) {
+      int64_t nj = ge - j0 < chunk ? ge - j0 : chunk;
+      hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, g_stream, dJobs, dOrder + j0,
+                         (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride);
+      BFK_CHECK(hipGetLastError());
+    }
+    BFK_CHECK(hipStreamSynchronize(g_stream));
+    BFK_CHECK(hipFree(work));
   }
   return time_end(1);
 }
@@ -584,8 +710,8 @@ int bfk_h0(int64_t n, const double *x, double *out) {
 
 int bfk_lstsq_loaded(int64_t batch, int p, int n, const double *A, const double *B, double *X) {
   if (bfk_init()) return -1;
-  size_t lds = ((size_t)p * (p + n + 1) + p) * 16;
-  if (lds > g_ldsCap || p > 128 || p < 1 || n < 1) return -3;
+  size_t lds = lstsq_units(p, n) * 16;
+  if (lds > g_ldsCap || p < 1 || n < 1) return -3;
   size_t sa = (size_t)batch * p * p * 16, sb = (size_t)batch * p * n * 16;
   double *dA = nullptr, *dB = nullptr, *dX = nullptr;
   BFK_CHECK(hipMalloc(&dA, sa));
