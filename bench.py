@@ -53,14 +53,14 @@ def dense_rows_threaded(k, tgt_rows, src, x, threads):
     return out
 
 
-def cpu_baseline(n_sample, tol, rank_const, n_full_entries):
+def cpu_baseline(n_sample, tol, rank_const, rcond, n_full_entries):
     """Oracle (single-threaded C restatement) factor + apply on the same
     geometry at n_sample points; the apply rate is scaled to the full size
     by the ratio of stored entries (the apply streams every entry once)."""
     import oracle as orc
     src, tgt, k = curve_problem(n_sample)
     t0 = time.perf_counter()
-    O = orc.Fac(k, src, tgt, 64, tol, 128 * 128, True)
+    O = orc.Fac(k, src, tgt, 64, tol, 128 * 128, True, rank_const=rank_const, pinv_rcond=rcond)
     t_fac = time.perf_counter() - t0
     lt, ft, bt = O.plan()
     entries = float((bt[:, 2] * bt[:, 3])[bt[:, 4] != 2].sum() + (lt[:, 3] * lt[:, 4])[lt[:, 0] == 0].sum())
@@ -92,17 +92,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--rank-const", type=float, default=1.0)
+    ap.add_argument("--rank-const", type=float, default=1.25)
+    ap.add_argument("--pinv-rcond", type=float, default=1e-11)
+    ap.add_argument("--mixed-tol", type=float, default=1e-7)
+    ap.add_argument("--mixed-rcond", type=float, default=1e-9)
     ap.add_argument("--rows", type=int, default=4096, help="exact dense rows for the error check (0 = skip)")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--mixed-rhs", type=int, default=256)
     ap.add_argument("--mixed-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4096)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run a plain copy kernel of this many MiB (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding of this path is described in DESIGN.md and not built", file=sys.stderr)
+              "this build runs the path on one GPU and has no multi-GPU sharding", file=sys.stderr)
         sys.exit(2)
 
     import butterfly_amd as bf
@@ -111,11 +116,12 @@ def main():
         raise SystemExit("no HIP device")
     n = args.n
     src, tgt, k = curve_problem(n)
-    print(f"[bench] N={n} k={k:.1f} leaf=64 tol={args.tol:g} C={args.rank_const}", file=sys.stderr, flush=True)
+    print(f"[bench] N={n} k={k:.1f} leaf=64 tol={args.tol:g} C={args.rank_const} rcond={args.pinv_rcond:g}",
+          file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
     ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
-    opts = bf.options(tol=args.tol, rank_const=args.rank_const, pass_through=True)
+    opts = bf.options(tol=args.tol, rank_const=args.rank_const, pass_through=True, pinv_rcond=args.pinv_rcond)
     F = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, opts)
     factor_s = time.perf_counter() - t0
     st = F.stats()
@@ -146,6 +152,12 @@ def main():
     print(f"[bench] apply {ms_per_step:.3f} ms/step (events {apply_ms:.3f} ms), {achieved:.0f} GB/s",
           file=sys.stderr, flush=True)
 
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        a, b = bf.DeviceArray(nb), bf.DeviceArray(nb)
+        bf.lib().bfhipDeviceCopy(a.ptr, b.ptr, nb)
+        del a, b
+
     rel_err = None
     if args.rows:
         rng = np.random.default_rng(1)
@@ -155,6 +167,14 @@ def main():
         rel_err = float(np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd))
         print(f"[bench] rel err vs {len(rows)} exact rows: {rel_err:.3e} ({time.perf_counter() - t0:.1f}s)",
               file=sys.stderr, flush=True)
+
+    # HBM bytes per apply measured by rocprofv3 FETCH_SIZE / WRITE_SIZE for
+    # this kernel and workload, scaled by the counters' reading of a plain
+    # copy (the summary file gives the commands)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_apply1_traffic.json")
+    if n == 1 << 20 and args.tol == 1e-8 and args.rank_const == 1.25 and os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic_bytes_per_apply"]
 
     mixed = None
     if args.mixed_rhs:
@@ -173,26 +193,48 @@ def main():
         t64 = (time.perf_counter() - t0) / args.mixed_steps
         Y64 = dY.to_numpy(np.complex128, (n, nr))
         del dX, dY
-        F.prepare_mixed("fp32")
+        del F  # free its HBM (factors and 256-RHS work buffers)
+        # the mixed form: its own factorization at the looser tolerance,
+        # stored and applied in FP32
+        t0 = time.perf_counter()
+        mopts = bf.options(tol=args.mixed_tol, rank_const=args.rank_const, pass_through=True,
+                           pinv_rcond=args.mixed_rcond)
+        Fm = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, mopts)
+        Fm.prepare_mixed("fp32", release_fp64=True)
+        factor_m = time.perf_counter() - t0
         dX32 = bf.DeviceArray.from_numpy(Xt.astype(np.complex64))
         dY32 = bf.DeviceArray(8 * n * nr)
-        bf.mul_device(F, dX32, dY32, nr, "fp32")
+        bf.mul_device(Fm, dX32, dY32, nr, "fp32")
         bf.lib().bfhipSynchronize()
         t0 = time.perf_counter()
         for _ in range(args.mixed_steps):
-            bf.mul_device(F, dX32, dY32, nr, "fp32")
+            bf.mul_device(Fm, dX32, dY32, nr, "fp32")
         bf.lib().bfhipSynchronize()
         t32 = (time.perf_counter() - t0) / args.mixed_steps
         Y32 = dY32.to_numpy(np.complex64, (n, nr)).astype(complex)
         err32 = float(np.linalg.norm(Y32 - Y64) / np.linalg.norm(Y64))
-        mixed = {"nrhs": nr, "fp64_apply_s": t64, "fp32_apply_s": t32, "fp64_rhs_per_s": nr / t64,
-                 "fp32_rhs_per_s": nr / t32, "speedup": t64 / t32, "fp32_rel_err_vs_fp64": err32}
-        print(f"[bench] {nr} RHS: fp64 {t64 * 1e3:.1f} ms, fp32 {t32 * 1e3:.1f} ms, fp32 err {err32:.2e}",
-              file=sys.stderr, flush=True)
+        del dX32, dY32
+        dx1 = bf.DeviceArray.from_numpy(x[ts.perm].astype(np.complex64))
+        dy1 = bf.DeviceArray(8 * n)
+        bf.mul_device(Fm, dx1, dy1, 1, "fp32")
+        bf.lib().bfhipSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bf.mul_device(Fm, dx1, dy1, 1, "fp32")
+        bf.lib().bfhipSynchronize()
+        t32_1 = (time.perf_counter() - t0) / args.steps
+        mixed = {"nrhs": nr, "tol": args.mixed_tol, "pinv_rcond": args.mixed_rcond, "factor_s": factor_m,
+                 "factor_speedup": factor_s / factor_m, "entries": Fm.stats()["entries"],
+                 "fp64_apply_s": t64, "fp32_apply_s": t32, "fp64_rhs_per_s": nr / t64,
+                 "fp32_rhs_per_s": nr / t32, "apply_speedup": t64 / t32, "fp32_rel_err_vs_fp64": err32,
+                 "fp32_1rhs_ms": t32_1 * 1e3, "fp64_1rhs_ms": ms_per_step}
+        print(f"[bench] mixed: factor {factor_m:.2f}s; {nr} RHS fp64 {t64 * 1e3:.1f} ms, fp32 {t32 * 1e3:.1f} ms, "
+              f"fp32 err {err32:.2e}; 1 RHS fp32 {t32_1 * 1e3:.2f} ms", file=sys.stderr, flush=True)
+        del Fm
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.rank_const, st["entries"])
+        cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.rank_const, args.pinv_rcond, st["entries"])
         print(f"[bench] cpu baseline {cpu['value']:.3f} RHS/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
     line = {
@@ -212,6 +254,7 @@ def main():
             "workload": "2^20 sources on r=1+0.3cos(5t) -> 2^20 targets on r=3, k=73064.8 (10 ppw), "
                         "leaf 64, rank tol 1e-8, FP64 factor + apply to 1 complex Gaussian RHS",
             "N": n, "k": k, "leaf_size": 64, "tol": args.tol, "rank_const": args.rank_const,
+            "pinv_rcond": args.pinv_rcond,
             "planner": "pass-through", "nrhs": 1,
         },
         "factor_s": factor_s,
@@ -222,7 +265,7 @@ def main():
         "rel_err_vs_dense_rows": rel_err,
         "dense_rows": args.rows,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_apply1 (all stages of one apply)", "bytes_per_apply": bytes_per_apply,
                      "event_ms_per_apply": apply_ms},
         "mixed": mixed,

@@ -56,7 +56,7 @@ class QuadtreeStruct(C.Structure):
 
 class FacOptionsStruct(C.Structure):
     _fields_ = [("tol", C.c_double), ("rankConst", C.c_double),
-                ("maxDenseSize", C.c_size_t), ("passThrough", C.c_bool)]
+                ("maxDenseSize", C.c_size_t), ("passThrough", C.c_bool), ("pinvRcond", C.c_double)]
 
 
 class MatDenseComplexStruct(C.Structure):
@@ -111,8 +111,10 @@ def lib():
         "bfhipMemcpyDtoH": (C.c_int, [vp, vp, sz]),
         "bfhipSynchronize": (C.c_int, []),
         "bfhipDeviceCount": (C.c_int, []),
+        "bfhipDeviceCopy": (C.c_int, [vp, vp, sz]),
         "bfhipMatMulDevice": (C.c_int, [vp, vp, vp, sz, C.c_int]),
         "bfhipFacPrepareMixed": (C.c_int, [vp, C.c_int]),
+        "bfhipFacReleaseFp64": (C.c_int, [vp]),
         "bfhipFacGetStats": (None, [vp, dp]),
         "bfhipGetKernelTimes": (None, [dp]),
         "bfhipGetKernelTimeSums": (None, [dp, dp]),
@@ -222,10 +224,12 @@ class Quadtree:
             pass
 
 
-def options(tol=1e-15, rank_const=1.0, max_dense=128 * 128, pass_through=False):
+def options(tol=1e-15, rank_const=1.0, max_dense=128 * 128, pass_through=False, pinv_rcond=0.0):
+    """BfFacHelm2Options; the defaults are the reference's fixed rules."""
     o = FacOptionsStruct()
     lib().bfFacHelm2OptionsInitDefault(C.byref(o))
     o.tol, o.rankConst, o.maxDenseSize, o.passThrough = tol, rank_const, max_dense, pass_through
+    o.pinvRcond = pinv_rcond
     return o
 
 
@@ -316,10 +320,13 @@ class Factorization(Mat):
             raise BfError("dense copy failed")
         return out.view(np.complex128).reshape(m, n)
 
-    def prepare_mixed(self, precision="fp32"):
+    def prepare_mixed(self, precision="fp32", release_fp64=False):
         if lib().bfhipFacPrepareMixed(self.ptr, PRECISION[precision]):
             check()
             raise BfError("prepare_mixed failed")
+        if release_fp64 and lib().bfhipFacReleaseFp64(self.ptr):
+            check()
+            raise BfError("release_fp64 failed")
 
 
 class FacHelm2:

@@ -25,7 +25,7 @@ void bfk_reset_times(void);
 int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
                double *dArena);
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
-              const size_t *groupBytes, const double *dSrcXY, double k, double *dArena);
+              const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena);
 int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
               const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs);
 int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
@@ -33,6 +33,7 @@ int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, c
                   float *dY, int nrhs);
 int bfk_to_f32(const double *dArena, int64_t numEntries, float *dHi, float *dLo);
 int bfk_h0(int64_t n, const double *x, double *out);
+int bfk_copy(const void *src, void *dst, size_t bytes);
 int bfk_lstsq_loaded(int64_t batch, int p, int n, const double *A, const double *B, double *X);
 
 /* ---- growable arrays */

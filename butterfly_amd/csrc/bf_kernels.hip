@@ -160,7 +160,7 @@ __device__ __forceinline__ void block_argmax(double &v, int &idx, double *red) {
  *  3. keep sigma_i >= p eps sigma_0 + eps (mat_dense_complex.c:1797-1809)
  *     and form X = P sum_i R'_i^H B'_i / sigma_i^2, written column-major. */
 template <int T>
-__device__ void qr_jacobi_solve(double2 *M, int p, int n, int ld, double2 *w, double2 *v, double *red, int *piv,
+__device__ void qr_jacobi_solve(double2 *M, int p, int n, int ld, double2 *w, double2 *v, double *red, int *piv, double rcond,
                                 double2 *__restrict__ out) {
   const int lane = threadIdx.x;
   const int W = p + n;
@@ -299,7 +299,7 @@ __device__ void qr_jacobi_solve(double2 *M, int p, int n, int ld, double2 *w, do
   int dummy = 0;
   block_argmax<T>(smax, dummy, red);
   __syncthreads();
-  const double tol = (double)p * eps * sqrt(smax) + eps;
+  const double tol = fmax((double)p * eps * sqrt(smax) + eps, rcond * sqrt(smax));
   for (int i = lane; i < p; i += T) {
     double s = sqrt(w[i].x);
     w[i].y = (s >= tol) ? 1.0 / w[i].x : 0.0;
@@ -346,7 +346,7 @@ template <int T, bool GLOBAL>
 __global__ __launch_bounds__(T) void k_lstsq(const BfkLstsqJob *__restrict__ jobs, const int32_t *__restrict__ order,
                                              const double2 *__restrict__ srcXY, double k,
                                              double2 *__restrict__ arena, double2 *__restrict__ gwork,
-                                             int64_t gstride) {
+                                             int64_t gstride, double rcond) {
   extern __shared__ double2 lds[];
   const BfkLstsqJob J = jobs[order[blockIdx.x]];
   const int p = J.p, n = J.n, ld = p + n + 1;
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(T) void k_lstsq(const BfkLstsqJob *__restrict__ job
     L.M[i * ld + c] = green(k, tp[i], s);
   }
   __syncthreads();
-  qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, arena + J.out);
+  qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, rcond, arena + J.out);
 }
 
 /* the same solve on matrices given in global memory (row-major A p x p,
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
     L.M[i * ld + c] = c < p ? a[i * p + c] : b[i * n + (c - p)];
   }
   __syncthreads();
-  qr_jacobi_solve<64>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, X + (size_t)blockIdx.x * p * n);
+  qr_jacobi_solve<64>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, 0.0, X + (size_t)blockIdx.x * p * n);
 }
 
 /* ---------------------------------------------------------------- apply */
@@ -471,6 +471,16 @@ __global__ __launch_bounds__(64) void k_applyN(const BfkSeg *__restrict__ segs, 
     if (live)
       for (int q = 0; q < nr; ++q) y[(S.yOff + rb + q) * nrhs + c] = acc[q];
   }
+}
+
+/* ------------------------------------------------------- plain copy */
+
+/* a streaming copy of known size: the reference point for the HBM byte
+ * counters (FETCH_SIZE / WRITE_SIZE) of the other kernels */
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 /* ----------------------------------------------------------------- H0 */
@@ -619,7 +629,7 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
  * one wave per job; the rest run 256 threads per job on global scratch,
  * all of them in one launch per chunk of 4096 jobs. */
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
-              const size_t *groupBytes, const double *dSrcXY, double k, double *dArena) {
+              const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena) {
   if (bfk_init()) return -1;
   if (time_begin()) return -1;
   int64_t gs = -1, ge = -1;
@@ -639,7 +649,7 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
       int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
       hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs,
                          dOrder + j0, (const double2 *)dSrcXY, k, (double2 *)dArena, (double2 *)nullptr,
-                         (int64_t)0);
+                         (int64_t)0, rcond);
       BFK_CHECK(hipGetLastError());
     }
   }
@@ -652,7 +662,7 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
     for (int64_t j0 = gs; j0 < ge; j0 += chunk) {
       int64_t nj = ge - j0 < chunk ? ge - j0 : chunk;
       hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, g_stream, dJobs, dOrder + j0,
-                         (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride);
+                         (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride, rcond);
       BFK_CHECK(hipGetLastError());
     }
     BFK_CHECK(hipStreamSynchronize(g_stream));
@@ -689,6 +699,16 @@ int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const
     BFK_CHECK(hipGetLastError());
   }
   return time_end(2);
+}
+
+int bfk_copy(const void *src, void *dst, size_t bytes) {
+  if (bfk_init()) return -1;
+  int64_t n = (int64_t)(bytes / 16);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, g_stream, (const double2 *)src, (double2 *)dst, n);
+  BFK_CHECK(hipGetLastError());
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
 }
 
 int bfk_h0(int64_t n, const double *x, double *out) {

@@ -28,6 +28,7 @@ void bfFacHelm2OptionsInitDefault(BfFacHelm2Options *opts) {
   opts->rankConst = 1;
   opts->maxDenseSize = 128 * 128;
   opts->passThrough = false;
+  opts->pinvRcond = 0;
 }
 
 static void free_plan(BfMatHipButterfly *F) {
@@ -328,7 +329,7 @@ static int run_factor_kernels(BfMatHipButterfly *F) {
   rc = upload(&dL, F->lstsqJobs.a, nj * sizeof(BfkLstsqJob));
   if (!rc) rc = upload(&dOrder, order, nj * sizeof(int32_t));
   if (!rc && nj)
-    rc = bfk_lstsq(dL, dOrder, (int)gb.n, gs.a, gb.a, F->dSrcXY, F->helm.k, F->dArena);
+    rc = bfk_lstsq(dL, dOrder, (int)gb.n, gs.a, gb.a, F->dSrcXY, F->helm.k, F->opts.pinvRcond, F->dArena);
   bfk_free(dL);
   bfk_free(dOrder);
   free(order);
@@ -471,7 +472,7 @@ int bfhipMatMulDevice(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, B
   }
   BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
   int rc;
-  if (!F->dArena || !F->dSegs) {
+  if (!F->dSegs || (prec == BFHIP_PRECISION_FP64 && !F->dArena)) {
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return -1;
   }
@@ -513,6 +514,25 @@ int bfhipFacPrepareMixed(BfMat *fac, BfhipPrecision prec) {
   if (!rc) rc = bfk_sync();
   if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
   return rc;
+}
+
+int bfhipFacReleaseFp64(BfMat *fac) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (!F->dArena32) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  bfk_free(F->dArena);
+  F->dArena = NULL;
+  bfk_free(F->dPing);
+  bfk_free(F->dPong);
+  F->dPing = F->dPong = NULL;
+  F->bufBytes = 0;
+  return 0;
 }
 
 BfMat *bfMatHipButterflyMul(BfMat const *A, BfMat const *Xm) {

@@ -277,6 +277,12 @@ int bfhipSynchronize(void) {
 
 int bfhipDeviceCount(void) { return bfk_device_count(); }
 
+int bfhipDeviceCopy(void const *src, void *dst, BfSize numBytes) {
+  int rc = bfk_copy(src, dst, numBytes);
+  if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
+  return rc;
+}
+
 int bfhipHankel0(BfSize n, BfReal const *x, BfReal *out) {
   int rc = bfk_h0((int64_t)n, x, out);
   if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);

@@ -254,6 +254,10 @@ typedef struct BfFacHelm2Options {
   BfSize maxDenseSize;   /* blocks with m*n below this are dense; reference 128*128 */
   bool passThrough;      /* carry shallow leaves down the butterfly levels
                             instead of bfFacHelm2Prepare's depth caps */
+  BfReal pinvRcond;      /* re-expansion pseudo-inverses also drop singular
+                            values below pinvRcond * sigma_0; reference 0
+                            (only max(m,n) eps sigma_0 + eps,
+                            mat_dense_complex.c:1797-1809) */
 } BfFacHelm2Options;
 void bfFacHelm2OptionsInitDefault(BfFacHelm2Options *opts);
 BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const *srcTree,
@@ -313,6 +317,9 @@ int bfhipMemcpyHtoD(void *dst, void const *src, BfSize numBytes);
 int bfhipMemcpyDtoH(void *dst, void const *src, BfSize numBytes);
 int bfhipSynchronize(void);
 int bfhipDeviceCount(void);
+/* new (no counterpart): device-to-device copy by a plain streaming kernel
+ * (16-byte elements), for callers calibrating HBM byte counters */
+int bfhipDeviceCopy(void const *src, void *dst, BfSize numBytes);
 
 /* new (no counterpart): Y = A X with X, Y already in device memory, for
  * callers that keep vectors resident in HBM (benchmarks, solvers).
@@ -324,6 +331,10 @@ int bfhipMatMulDevice(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, B
 /* new: convert the stored factors for the FP32 / FP32x2 applies (kept
  * beside the FP64 factors until bfMatDelete) */
 int bfhipFacPrepareMixed(BfMat *fac, BfhipPrecision prec);
+/* new: free the FP64 factors (and apply buffers) of a factorization that
+ * has FP32 factors, leaving only the FP32 applies; for callers that keep
+ * a mixed factorization resident and need the HBM */
+int bfhipFacReleaseFp64(BfMat *fac);
 
 /* new (no counterpart): factorization statistics
  * out[0] factor wall seconds, [1] tree+plan seconds, [2] sampling+compression

@@ -29,7 +29,13 @@
  * a different algorithm from LAPACK's zgesvd used by the reference, so the
  * pseudo-inverse is pinned independently by numpy in tests/.
  *
- * Parity status of each function is listed in DESIGN.md's oracle table.
+ * Pins: the RNG against published splitmix64 outputs and an independent
+ * restatement; H0 against mpmath; the pseudo-inverse against numpy; the
+ * tree against its invariants; the factorization against exact dense rows
+ * (tests/test_oracle.py).  The order of points inside a quadtree child
+ * follows the reference's sift and is pinned by nothing else; the
+ * pass-through planner, the relative pseudo-inverse cut and the impedance
+ * test have no reference counterpart and are pinned end to end only.
  */
 #define _DEFAULT_SOURCE 1
 #include <complex.h>
@@ -251,11 +257,14 @@ static OCirc onode_circ(const ONode *n) {
 
 static double pdist(const double *p, const double *q) { return hypot(q[0] - p[0], q[1] - p[1]); }
 
-/* helm2.c:14-36 with C = 1 */
+/* helm2.c:14-36; the reference passes C = 1 at every call site */
+static double orc_rank_c = 1.0;
+void orc_set_rank_const(double c) { orc_rank_c = c; }
+
 static size_t orc_rank(double k, OCirc a, OCirc b, double tol) {
   double R = pdist(a.c, b.c);
   double d = R - a.r - b.r;
-  double p = k * a.r * b.r / d - log10(tol);
+  double p = k * a.r * b.r / d - orc_rank_c * log10(tol);
   return (size_t)ceil(p);
 }
 
@@ -381,10 +390,15 @@ static void jacobi_svd(size_t m, size_t n, cplx *W, cplx *V, double *sigma) {
   }
 }
 
+/* relative cut added to the pseudo-inverse in the config-driven mode
+ * (0 = the reference rule alone) */
+static double orc_pinv_rcond = 0;
+void orc_set_pinv_rcond(double v) { orc_pinv_rcond = v; }
+
 /* X (n x nb) = pinv(A) B for A m x n (m >= n), B m x nb, with the
  * reference's truncation rule (mat_dense_complex.c:1797-1809):
- * keep sigma_k >= max(m,n)*eps*sigma_0 + eps.  Returns the number of
- * retained singular values. */
+ * keep sigma_k >= max(m,n)*eps*sigma_0 + eps (and >= rcond sigma_0 when
+ * the relative cut is set).  Returns the number of retained values. */
 static size_t lstsq(size_t m, size_t n, const cplx *A, size_t nb, const cplx *B, cplx *X) {
   cplx *W = malloc(m * n * sizeof(cplx));
   cplx *V = malloc(n * n * sizeof(cplx));
@@ -393,7 +407,7 @@ static size_t lstsq(size_t m, size_t n, const cplx *A, size_t nb, const cplx *B,
   jacobi_svd(m, n, W, V, sig);
   double smax = 0;
   for (size_t j = 0; j < n; ++j) smax = fmax(smax, sig[j]);
-  double tol = (double)(m > n ? m : n) * EPS_MACH * smax + EPS_MACH;
+  double tol = fmax((double)(m > n ? m : n) * EPS_MACH * smax + EPS_MACH, orc_pinv_rcond * smax);
   /* C = U_k^H B / sigma_k with U_k = W_k / sigma_k */
   cplx *C = calloc(n * nb, sizeof(cplx));
   size_t kept = 0;
@@ -539,10 +553,35 @@ static OPtSet node_circle(OFac *F, int isSrc, int ni, size_t p) {
 
 /* re-expansion: Z_shift = LstSq(K(test <- equiv), K(test <- orig))
  * (helm2.c:321-365) */
+/* When set, re-expansions match the impedance trace (d/dn - i k) u of the
+ * field on the test circle instead of u itself; up to the factor k/4,
+ * (d/dn_t - i k) G = (J0 + Y1 cos a) + i (Y0 - J1 cos a), cos a the cosine
+ * between t - s and the circle's outward normal at t. */
+static int orc_impedance = 0;
+void orc_set_impedance(int v) { orc_impedance = v; }
+
+static cplx *impedance_matrix(double k, const OPtSet *src, const OPtSet *test) {
+  size_t m = test->n, n = src->n;
+  cplx *Z = malloc(m * n * sizeof(cplx));
+  for (size_t i = 0; i < m; ++i) {
+    double tp[2], sp[2];
+    ptset_get(test, i, tp);
+    double nx = (tp[0] - test->circ.c[0]) / test->circ.r, ny = (tp[1] - test->circ.c[1]) / test->circ.r;
+    for (size_t j = 0; j < n; ++j) {
+      ptset_get(src, j, sp);
+      double r = pdist(tp, sp);
+      double x = k * r;
+      double ca = ((tp[0] - sp[0]) * nx + (tp[1] - sp[1]) * ny) / r;
+      Z[i * n + j] = CMPLX(j0(x) + y1(x) * ca, y0(x) - j1(x) * ca);
+    }
+  }
+  return Z;
+}
+
 static cplx *reexpansion(double k, const OPtSet *orig, const OPtSet *equiv, const OPtSet *test) {
   if (orc_plan_only) return NULL;
-  cplx *Zo = kernel_matrix(k, orig, test);
-  cplx *Ze = kernel_matrix(k, equiv, test);
+  cplx *Zo = orc_impedance ? impedance_matrix(k, orig, test) : kernel_matrix(k, orig, test);
+  cplx *Ze = orc_impedance ? impedance_matrix(k, equiv, test) : kernel_matrix(k, equiv, test);
   cplx *X = malloc(equiv->n * orig->n * sizeof(cplx));
   lstsq(test->n, equiv->n, Ze, orig->n, Zo, X);
   free(Zo); free(Ze);

@@ -75,14 +75,17 @@ def _plan_pair(src, tgt, k, leaf, tol, pt):
     return F, O
 
 
-@pytest.mark.parametrize("case", ["box_ref", "box_pass", "curve_pass", "curve_ref"])
+@pytest.mark.parametrize("case", ["box_ref", "box_pass", "curve_pass", "curve_ref", "curve_pass_c125"])
 def test_plan_matches_oracle(case):
     if case.startswith("box"):
         src, tgt, k = box_problem(4096)
     else:
         src, tgt, k = curve_problem(4096)
     leaf, tol, pt = (1, 1e-15, False) if case.endswith("ref") else (64, 1e-8, True)
-    F, O = _plan_pair(src, tgt, k, leaf, tol, pt)
+    C = 1.25 if case.endswith("c125") else 1.0
+    ts, tt = bf.Quadtree(src, leaf), bf.Quadtree(tgt, leaf)
+    F = bf.FacHelm2.plan_only(bf.Helm2(k), ts, tt, bf.options(tol=tol, pass_through=pt, rank_const=C))
+    O = orc.Fac(k, src, tgt, leaf, tol, 128 * 128, pt, plan_only=True, rank_const=C)
     for a, b in zip(F.plan(), O.plan()):
         assert a.shape == b.shape and np.array_equal(a, b)
 
