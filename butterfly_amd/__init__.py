@@ -119,6 +119,7 @@ def lib():
         "bfhipGetKernelTimes": (None, [dp]),
         "bfhipGetKernelTimeSums": (None, [dp, dp]),
         "bfhipResetKernelTimes": (None, []),
+        "bfhipSetApplyConfig": (C.c_int, [C.c_int, C.c_int, C.c_int]),
         "bfhipFacPlanSizes": (None, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
         "bfhipFacPlanExport": (None, [vp, i64p, i64p, i64p]),
         "bfhipFacBlockData": (C.c_int, [vp, sz, dp]),

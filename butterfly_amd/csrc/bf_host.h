@@ -22,6 +22,7 @@ int bfk_sync(void);
 void bfk_get_times(double out[8]);
 void bfk_get_time_sums(double sums[8], double counts[8]);
 void bfk_reset_times(void);
+int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles);
 int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
                double *dArena);
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,

@@ -397,37 +397,97 @@ struct BfkBufs {
   double2 *out[4];
 };
 
-__global__ __launch_bounds__(64) void k_apply1(const BfkSeg *__restrict__ segs, const BfkTerm *__restrict__ terms,
-                                               const double2 *__restrict__ arena, BfkBufs bufs) {
-  const BfkSeg S = segs[blockIdx.x];
-  const int lane = threadIdx.x;
+struct BfkBufsF {
+  const float2 *in[4];
+  float2 *out[4];
+};
+
+__device__ __forceinline__ float2 cfmaf(float2 a, float2 b, float2 c) {
+  return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, c.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, c.y)));
+}
+__device__ __forceinline__ double2 cfmaT(double2 a, double2 b, double2 c) { return cfma(a, b, c); }
+__device__ __forceinline__ float2 cfmaT(float2 a, float2 b, float2 c) { return cfmaf(a, b, c); }
+template <typename V> __device__ __forceinline__ V czero();
+template <> __device__ __forceinline__ double2 czero<double2>() { return make_double2(0, 0); }
+template <> __device__ __forceinline__ float2 czero<float2>() { return make_float2(0, 0); }
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) {
+  a.x += b.x;
+  a.y += b.y;
+  return a;
+}
+
+template <typename V> struct BufsOf;
+template <> struct BufsOf<double2> { typedef BfkBufs T; };
+template <> struct BufsOf<float2> { typedef BfkBufsF T; };
+
+/* Launch shape of the 1-RHS apply: waves (segments) per block and whether
+ * a lane runs four independent FMA chains; set by bfk_set_apply_config. */
+static int g_applyWaves = 1;
+static int g_applyUnroll = 0;
+/* right-hand-side tiles of 16 per wave in the MFMA apply (0 = off) */
+static int g_mfmaCT = 1;
+
+/* One stage of the apply for one right-hand side.  Each wave of the block
+ * owns one output row segment: its lanes split into R rows x G = 64/R
+ * column groups (R the next power of two >= the segment height, at most
+ * 64); a lane walks its columns of every term (column-major blocks, so the
+ * R lanes of a group read R consecutive entries), then the G partial sums
+ * meet in a shuffle reduction and the row is written once.  With LO the
+ * factors are an FP32 pair hi + lo. */
+template <typename V, bool LO, bool UNROLL>
+__global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
+                                                const BfkTerm *__restrict__ terms, const V *__restrict__ hi,
+                                                const V *__restrict__ lo, typename BufsOf<V>::T bufs) {
+  const int lane = threadIdx.x & 63;
+  const int64_t si = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (si >= numSegs) return; /* whole wave; no block barrier below */
+  const BfkSeg S = segs[si];
   const int m = S.m;
   int R = 64;
   while (R > 1 && (R >> 1) >= m) R >>= 1;
   const int G = 64 / R;
   const int r = lane & (R - 1);
   const int g = lane / R;
-  double2 *y = bufs.out[S.ybuf] + S.yOff;
+  V *y = bufs.out[S.ybuf] + S.yOff;
   for (int rb = 0; rb < m; rb += R) {
     const int i = rb + r;
-    double2 acc = make_double2(0, 0);
+    V a0 = czero<V>(), a1 = czero<V>(), a2 = czero<V>(), a3 = czero<V>();
     if (i < m) {
       for (int t = S.t0; t < S.t1; ++t) {
         const BfkTerm T = terms[t];
-        const double2 *x = bufs.in[T.xbuf] + T.xOff;
+        const V *x = bufs.in[T.xbuf] + T.xOff;
         if (T.aOff < 0) {
-          if (g == 0) {
-            double2 v = x[i];
-            acc.x += v.x;
-            acc.y += v.y;
+          if (g == 0) a0 = cadd(a0, x[i]);
+          continue;
+        }
+        const int64_t lda = T.lda;
+        const V *A = hi + T.aOff + T.rowOff + i;
+        const int n = T.n;
+        int j = g;
+        if (UNROLL) for (; j + 3 * G < n; j += 4 * G) {
+          V m0 = A[(int64_t)j * lda], m1 = A[(int64_t)(j + G) * lda];
+          V m2 = A[(int64_t)(j + 2 * G) * lda], m3 = A[(int64_t)(j + 3 * G) * lda];
+          V x0 = x[j], x1 = x[j + G], x2 = x[j + 2 * G], x3 = x[j + 3 * G];
+          a0 = cfmaT(m0, x0, a0);
+          a1 = cfmaT(m1, x1, a1);
+          a2 = cfmaT(m2, x2, a2);
+          a3 = cfmaT(m3, x3, a3);
+          if (LO) {
+            const V *B = lo + T.aOff + T.rowOff + i;
+            a0 = cfmaT(B[(int64_t)j * lda], x0, a0);
+            a1 = cfmaT(B[(int64_t)(j + G) * lda], x1, a1);
+            a2 = cfmaT(B[(int64_t)(j + 2 * G) * lda], x2, a2);
+            a3 = cfmaT(B[(int64_t)(j + 3 * G) * lda], x3, a3);
           }
-        } else {
-          const double2 *A = arena + T.aOff + T.rowOff + i;
-          const int64_t lda = T.lda;
-          for (int j = g; j < T.n; j += G) acc = cfma(A[j * lda], x[j], acc);
+        }
+        for (; j < n; j += G) {
+          V xj = x[j];
+          a0 = cfmaT(A[(int64_t)j * lda], xj, a0);
+          if (LO) a0 = cfmaT(lo[T.aOff + T.rowOff + i + (int64_t)j * lda], xj, a0);
         }
       }
     }
+    V acc = cadd(cadd(a0, a1), cadd(a2, a3));
     for (int o = R; o < 64; o <<= 1) {
       acc.x += __shfl_xor(acc.x, o, 64);
       acc.y += __shfl_xor(acc.y, o, 64);
@@ -470,6 +530,116 @@ __global__ __launch_bounds__(64) void k_applyN(const BfkSeg *__restrict__ segs, 
     }
     if (live)
       for (int q = 0; q < nr; ++q) y[(S.yOff + rb + q) * nrhs + c] = acc[q];
+  }
+}
+
+/* ------------------------------------------- many right-hand sides: MFMA */
+
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+/* 16x16x4 MFMA on one wave, real operands.  Operand lanes: A[l%16][l/16],
+ * B[l/16][l%16].  Result register r of lane l holds D[row(l,r)][l%16]
+ * with row = 4r + l/16 for FP64 and 4(l/16) + r for FP32 (measured on
+ * gfx950). */
+struct MfmaF64 {
+  typedef double S;
+  typedef double2 V;
+  typedef mf_d4 Acc;
+  static __device__ __forceinline__ Acc mma(S a, S b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
+};
+struct MfmaF32 {
+  typedef float S;
+  typedef float2 V;
+  typedef mf_f4 Acc;
+  static __device__ __forceinline__ Acc mma(S a, S b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+/* One stage of the apply for nrhs right-hand sides on the matrix cores:
+ * one wave computes a 16-row x 16*CT-column tile of a segment's output,
+ * looping over its row tiles, terms and 4-column steps of each term.  The
+ * complex product is four real MFMAs per 16x16x4 step (re*re - im*im,
+ * re*im + im*re).  blockIdx.x picks the column chunk so the chunks of one
+ * segment run back to back and share its factor blocks in L2. */
+template <class M, int CT, bool LO>
+__global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ segs,
+                                                    const BfkTerm *__restrict__ terms,
+                                                    const typename M::V *__restrict__ hi,
+                                                    const typename M::V *__restrict__ lo,
+                                                    typename BufsOf<typename M::V>::T bufs, int nrhs,
+                                                    int numChunks) {
+  typedef typename M::S S;
+  typedef typename M::V V;
+  typedef typename M::Acc Acc;
+  const BfkSeg Sg = segs[blockIdx.x / numChunks];
+  const int l = threadIdx.x;
+  const int c0 = (blockIdx.x % numChunks) * 16 * CT;
+  const int m = Sg.m;
+  const int lr = l & 15, lk = l >> 4;
+  V *y = bufs.out[Sg.ybuf];
+  for (int rt = 0; rt < m; rt += 16) {
+    Acc re[CT], im[CT];
+    for (int q = 0; q < CT; ++q)
+      for (int r = 0; r < 4; ++r) re[q][r] = im[q][r] = 0;
+    for (int t = Sg.t0; t < Sg.t1; ++t) {
+      const BfkTerm T = terms[t];
+      const V *x = bufs.in[T.xbuf];
+      if (T.aOff < 0) {
+        for (int q = 0; q < CT; ++q) {
+          const int col = c0 + 16 * q + lr;
+          for (int r = 0; r < 4; ++r) {
+            const int i = rt + M::row(l, r);
+            if (i < m && col < nrhs) {
+              V v = x[(T.xOff + i) * (int64_t)nrhs + col];
+              re[q][r] += v.x;
+              im[q][r] += v.y;
+            }
+          }
+        }
+        continue;
+      }
+      const int n = T.n;
+      const int64_t lda = T.lda;
+      const int ai = rt + lr;
+      for (int k0 = 0; k0 < n; k0 += 4) {
+        const int k = k0 + lk;
+        const bool aok = ai < m && k < n;
+        V a = aok ? hi[T.aOff + T.rowOff + ai + k * lda] : czero<V>();
+        V al = (LO && aok) ? lo[T.aOff + T.rowOff + ai + k * lda] : czero<V>();
+        for (int q = 0; q < CT; ++q) {
+          const int col = c0 + 16 * q + lr;
+          V b = (k < n && col < nrhs) ? x[(T.xOff + k) * (int64_t)nrhs + col] : czero<V>();
+          re[q] = M::mma(a.x, b.x, re[q]);
+          re[q] = M::mma(-a.y, b.y, re[q]);
+          im[q] = M::mma(a.x, b.y, im[q]);
+          im[q] = M::mma(a.y, b.x, im[q]);
+          if (LO) {
+            re[q] = M::mma(al.x, b.x, re[q]);
+            re[q] = M::mma(-al.y, b.y, re[q]);
+            im[q] = M::mma(al.x, b.y, im[q]);
+            im[q] = M::mma(al.y, b.x, im[q]);
+          }
+        }
+      }
+    }
+    for (int q = 0; q < CT; ++q) {
+      const int col = c0 + 16 * q + lr;
+      for (int r = 0; r < 4; ++r) {
+        const int i = rt + M::row(l, r);
+        if (i < m && col < nrhs) {
+          V v;
+          v.x = (S)re[q][r];
+          v.y = (S)im[q][r];
+          y[(Sg.yOff + i) * (int64_t)nrhs + col] = v;
+        }
+      }
+    }
   }
 }
 
@@ -585,6 +755,15 @@ void bfk_get_time_sums(double sums[8], double counts[8]) {
   }
 }
 
+int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles) {
+  if (wavesPerBlock != 1 && wavesPerBlock != 2 && wavesPerBlock != 4) return -1;
+  if (mfmaTiles != 0 && mfmaTiles != 1 && mfmaTiles != 2 && mfmaTiles != 4 && mfmaTiles != 8) return -1;
+  g_applyWaves = wavesPerBlock;
+  g_applyUnroll = unroll ? 1 : 0;
+  g_mfmaCT = mfmaTiles;
+  return 0;
+}
+
 void bfk_reset_times(void) {
   for (int i = 0; i < 8; ++i) g_sum[i] = g_cnt[i] = 0;
 }
@@ -690,8 +869,30 @@ int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const
     int64_t n = stageSeg[s + 1] - stageSeg[s];
     if (n == 0) continue;
     if (nrhs == 1) {
-      hipLaunchKernelGGL(k_apply1, dim3((unsigned)n), dim3(64), 0, g_stream, dSegs + stageSeg[s], dTerms,
-                         (const double2 *)dArena, b);
+      const int wv = g_applyWaves;
+      dim3 grid((unsigned)((n + wv - 1) / wv)), block(64 * wv);
+      if (g_applyUnroll)
+        hipLaunchKernelGGL((k_apply1<double2, false, true>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b);
+      else
+        hipLaunchKernelGGL((k_apply1<double2, false, false>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b);
+    } else if (nrhs >= 16 && g_mfmaCT > 0) {
+      const int ct = g_mfmaCT;
+      const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
+      dim3 grid((unsigned)(n * chunks));
+      if (ct == 1)
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 1, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
+      else if (ct == 2)
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 2, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
+      else if (ct == 8)
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 8, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
+      else
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 4, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
     } else {
       hipLaunchKernelGGL(k_applyN, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
                          dSegs + stageSeg[s], dTerms, (const double2 *)dArena, b, nrhs);
@@ -753,65 +954,6 @@ int bfk_lstsq_loaded(int64_t batch, int p, int n, const double *A, const double 
 } /* extern "C" */
 
 /* ====================================================== FP32 apply path */
-
-struct BfkBufsF {
-  const float2 *in[4];
-  float2 *out[4];
-};
-
-__device__ __forceinline__ float2 cfmaf(float2 a, float2 b, float2 c) {
-  return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, c.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, c.y)));
-}
-
-/* factors as FP32 (hi) plus an optional FP32 remainder (lo), FP32
- * vectors, FP32 accumulation */
-__global__ __launch_bounds__(64) void k_apply1_f32(const BfkSeg *__restrict__ segs, const BfkTerm *__restrict__ terms,
-                                                   const float2 *__restrict__ hi, const float2 *__restrict__ lo,
-                                                   BfkBufsF bufs) {
-  const BfkSeg S = segs[blockIdx.x];
-  const int lane = threadIdx.x;
-  const int m = S.m;
-  int R = 64;
-  while (R > 1 && (R >> 1) >= m) R >>= 1;
-  const int G = 64 / R;
-  const int r = lane & (R - 1);
-  const int g = lane / R;
-  float2 *y = bufs.out[S.ybuf] + S.yOff;
-  for (int rb = 0; rb < m; rb += R) {
-    const int i = rb + r;
-    float2 acc = make_float2(0, 0);
-    if (i < m) {
-      for (int t = S.t0; t < S.t1; ++t) {
-        const BfkTerm T = terms[t];
-        const float2 *x = bufs.in[T.xbuf] + T.xOff;
-        if (T.aOff < 0) {
-          if (g == 0) {
-            float2 v = x[i];
-            acc.x += v.x;
-            acc.y += v.y;
-          }
-        } else {
-          const int64_t base = T.aOff + T.rowOff + i;
-          const int64_t lda = T.lda;
-          if (lo) {
-            for (int j = g; j < T.n; j += G) {
-              float2 xv = x[j];
-              acc = cfmaf(lo[base + j * lda], xv, acc);
-              acc = cfmaf(hi[base + j * lda], xv, acc);
-            }
-          } else {
-            for (int j = g; j < T.n; j += G) acc = cfmaf(hi[base + j * lda], x[j], acc);
-          }
-        }
-      }
-    }
-    for (int o = R; o < 64; o <<= 1) {
-      acc.x += __shfl_xor(acc.x, o, 64);
-      acc.y += __shfl_xor(acc.y, o, 64);
-    }
-    if (g == 0 && i < m) y[i] = acc;
-  }
-}
 
 __global__ __launch_bounds__(64) void k_applyN_f32(const BfkSeg *__restrict__ segs, const BfkTerm *__restrict__ terms,
                                                    const float2 *__restrict__ hi, const float2 *__restrict__ lo,
@@ -892,8 +1034,37 @@ int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, c
     int64_t n = stageSeg[s + 1] - stageSeg[s];
     if (n == 0) continue;
     if (nrhs == 1) {
-      hipLaunchKernelGGL(k_apply1_f32, dim3((unsigned)n), dim3(64), 0, g_stream, dSegs + stageSeg[s], dTerms,
-                         (const float2 *)dArenaHi, (const float2 *)dArenaLo, b);
+      const int wv = g_applyWaves;
+      dim3 grid((unsigned)((n + wv - 1) / wv)), block(64 * wv);
+      if (dArenaLo)
+        hipLaunchKernelGGL((k_apply1<float2, true, false>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
+                           dTerms, (const float2 *)dArenaHi, (const float2 *)dArenaLo, b);
+      else if (g_applyUnroll)
+        hipLaunchKernelGGL((k_apply1<float2, false, true>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
+                           dTerms, (const float2 *)dArenaHi, (const float2 *)nullptr, b);
+      else
+        hipLaunchKernelGGL((k_apply1<float2, false, false>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
+                           dTerms, (const float2 *)dArenaHi, (const float2 *)nullptr, b);
+    } else if (nrhs >= 16 && g_mfmaCT > 0) {
+      const int ct = g_mfmaCT;
+      const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
+      dim3 grid((unsigned)(n * chunks));
+      const float2 *H = (const float2 *)dArenaHi, *Lo = (const float2 *)dArenaLo;
+      if (Lo)
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 4, true>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, H, Lo, b, nrhs, chunks);
+      else if (ct == 1)
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 1, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, H, Lo, b, nrhs, chunks);
+      else if (ct == 2)
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 2, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, H, Lo, b, nrhs, chunks);
+      else if (ct == 8)
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 8, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, H, Lo, b, nrhs, chunks);
+      else
+        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 4, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
+                           dTerms, H, Lo, b, nrhs, chunks);
     } else {
       hipLaunchKernelGGL(k_applyN_f32, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
                          dSegs + stageSeg[s], dTerms, (const float2 *)dArenaHi, (const float2 *)dArenaLo, b, nrhs);

@@ -592,6 +592,12 @@ void bfhipGetKernelTimeSums(double sums[8], double counts[8]) { bfk_get_time_sum
 
 void bfhipResetKernelTimes(void) { bfk_reset_times(); }
 
+int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles) {
+  int rc = bfk_set_apply_config(wavesPerBlock, unroll, mfmaTiles);
+  if (rc) bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+  return rc;
+}
+
 void bfhipFacPlanSizes(BfMat const *fac, BfSize *numLeaves, BfSize *numFactors, BfSize *numBlocks) {
   BfMatHipButterfly const *F = (BfMatHipButterfly const *)fac;
   *numLeaves = F->leaves.n;

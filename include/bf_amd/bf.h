@@ -352,6 +352,12 @@ void bfhipGetKernelTimes(double out[8]);
  * measurements per slot (slot 3 is the FP32 apply) */
 void bfhipGetKernelTimeSums(double sums[8], double counts[8]);
 void bfhipResetKernelTimes(void);
+/* new: launch shapes of the apply, for tuning: the 1-RHS apply's waves per
+ * block in {1, 2, 4} (one row segment per wave) and four independent FMA
+ * chains per lane when `unroll`; the many-RHS apply's right-hand-side
+ * tiles of 16 per wave on the matrix cores in {2, 4, 8}, or 0 for the
+ * vector-unit kernel */
+int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
 
 /* new: plan export for parity checks (integers, in the oracle's canonical
  * order): leafTab [L x 6], facTab [F x 4], blkTab [B x 7] */
