@@ -94,12 +94,15 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--rank-const", type=float, default=1.25)
     ap.add_argument("--pinv-rcond", type=float, default=1e-11)
-    ap.add_argument("--mixed-tol", type=float, default=1e-7)
-    ap.add_argument("--mixed-rcond", type=float, default=1e-9)
+    ap.add_argument("--mixed-tol", type=float, default=1e-8)
+    ap.add_argument("--mixed-rcond", type=float, default=1e-11)
+    ap.add_argument("--mixed-rank-const", type=float, default=0.0, help="0 = same as --rank-const")
+    ap.add_argument("--comp", type=int, default=1, help="compensated accumulation in the FP32 applies")
     ap.add_argument("--rows", type=int, default=4096, help="exact dense rows for the error check (0 = skip)")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--mixed-rhs", type=int, default=256)
     ap.add_argument("--mixed-steps", type=int, default=5)
+    ap.add_argument("--mixed-ortho", type=int, default=1, help="orthogonalize the mixed factorization (1) or not (0)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4096)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -114,6 +117,7 @@ def main():
 
     if bf.lib().bfhipDeviceCount() < 1:
         raise SystemExit("no HIP device")
+    bf.lib().bfhipSetApplyCompensation(args.comp)
     n = args.n
     src, tgt, k = curve_problem(n)
     print(f"[bench] N={n} k={k:.1f} leaf=64 tol={args.tol:g} C={args.rank_const} rcond={args.pinv_rcond:g}",
@@ -197,11 +201,21 @@ def main():
         # the mixed form: its own factorization at the looser tolerance,
         # stored and applied in FP32
         t0 = time.perf_counter()
-        mopts = bf.options(tol=args.mixed_tol, rank_const=args.rank_const, pass_through=True,
+        mopts = bf.options(tol=args.mixed_tol, rank_const=args.mixed_rank_const or args.rank_const, pass_through=True,
                            pinv_rcond=args.mixed_rcond)
         Fm = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, mopts)
-        Fm.prepare_mixed("fp32", release_fp64=True)
+        ortho_s = Fm.orthogonalize() if args.mixed_ortho else 0.0
+        Fm.prepare_mixed("fp32")
         factor_m = time.perf_counter() - t0
+        # accuracy of the mixed factorization itself (its FP64 apply), then
+        # drop the FP64 factors
+        errm64_rows = None
+        if args.rows:
+            bf.mul_device(Fm, dx, dy, 1, "fp64")
+            ym = np.empty(n, complex)
+            ym[tt.perm] = dy.to_numpy(np.complex128, n)
+            errm64_rows = float(np.linalg.norm(ym[rows] - yd) / np.linalg.norm(yd))
+        bf.lib().bfhipFacReleaseFp64(Fm.ptr)
         dX32 = bf.DeviceArray.from_numpy(Xt.astype(np.complex64))
         dY32 = bf.DeviceArray(8 * n * nr)
         bf.mul_device(Fm, dX32, dY32, nr, "fp32")
@@ -223,13 +237,22 @@ def main():
             bf.mul_device(Fm, dx1, dy1, 1, "fp32")
         bf.lib().bfhipSynchronize()
         t32_1 = (time.perf_counter() - t0) / args.steps
+        err32_rows = None
+        if args.rows:
+            y32 = np.empty(n, complex)
+            y32[tt.perm] = dy1.to_numpy(np.complex64, n).astype(complex)
+            err32_rows = float(np.linalg.norm(y32[rows] - yd) / np.linalg.norm(yd))
         mixed = {"nrhs": nr, "tol": args.mixed_tol, "pinv_rcond": args.mixed_rcond, "factor_s": factor_m,
+                 "orthogonalize_s": ortho_s, "orthogonalized": bool(args.mixed_ortho),
                  "factor_speedup": factor_s / factor_m, "entries": Fm.stats()["entries"],
                  "fp64_apply_s": t64, "fp32_apply_s": t32, "fp64_rhs_per_s": nr / t64,
                  "fp32_rhs_per_s": nr / t32, "apply_speedup": t64 / t32, "fp32_rel_err_vs_fp64": err32,
+                 "fp32_rel_err_vs_dense_rows": err32_rows, "fp64_rel_err_vs_dense_rows": errm64_rows,
                  "fp32_1rhs_ms": t32_1 * 1e3, "fp64_1rhs_ms": ms_per_step}
-        print(f"[bench] mixed: factor {factor_m:.2f}s; {nr} RHS fp64 {t64 * 1e3:.1f} ms, fp32 {t32 * 1e3:.1f} ms, "
-              f"fp32 err {err32:.2e}; 1 RHS fp32 {t32_1 * 1e3:.2f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] mixed: factor {factor_m:.2f}s (orthogonalize {ortho_s:.2f}s); {nr} RHS fp64 "
+              f"{t64 * 1e3:.1f} ms, fp32 {t32 * 1e3:.1f} ms, fp32 err vs fp64 {err32:.2e}, vs exact rows: "
+              f"1 RHS fp32 {err32_rows}, its fp64 apply {errm64_rows}; 1 RHS fp32 {t32_1 * 1e3:.2f} ms",
+              file=sys.stderr, flush=True)
         del Fm
 
     cpu = None

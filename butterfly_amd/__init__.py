@@ -115,11 +115,14 @@ def lib():
         "bfhipMatMulDevice": (C.c_int, [vp, vp, vp, sz, C.c_int]),
         "bfhipFacPrepareMixed": (C.c_int, [vp, C.c_int]),
         "bfhipFacReleaseFp64": (C.c_int, [vp]),
+        "bfhipFacOrthogonalize": (C.c_int, [vp]),
+        "bfhipFacOrthogonalizeSeconds": (C.c_double, [vp]),
         "bfhipFacGetStats": (None, [vp, dp]),
         "bfhipGetKernelTimes": (None, [dp]),
         "bfhipGetKernelTimeSums": (None, [dp, dp]),
         "bfhipResetKernelTimes": (None, []),
         "bfhipSetApplyConfig": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+        "bfhipSetApplyCompensation": (C.c_int, [C.c_int]),
         "bfhipFacPlanSizes": (None, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
         "bfhipFacPlanExport": (None, [vp, i64p, i64p, i64p]),
         "bfhipFacBlockData": (C.c_int, [vp, sz, dp]),
@@ -328,6 +331,14 @@ class Factorization(Mat):
         if release_fp64 and lib().bfhipFacReleaseFp64(self.ptr):
             check()
             raise BfError("release_fp64 failed")
+
+    def orthogonalize(self):
+        """bfhipFacOrthogonalize: rotate every intermediate basis so the map
+        to the output is orthonormal (same operator); returns the seconds."""
+        if lib().bfhipFacOrthogonalize(self.ptr):
+            check()
+            raise BfError("orthogonalize failed")
+        return lib().bfhipFacOrthogonalizeSeconds(self.ptr)
 
 
 class FacHelm2:

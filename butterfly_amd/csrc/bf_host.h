@@ -23,10 +23,15 @@ void bfk_get_times(double out[8]);
 void bfk_get_time_sums(double sums[8], double counts[8]);
 void bfk_reset_times(void);
 int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles);
+int bfk_set_compensation(int on);
 int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
                double *dArena);
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
               const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena);
+size_t bfk_ortho_units(int rows, int n);
+int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
+              const size_t *groupBytes, const BfkOrthoItem *dItems, double *dArena, const double *dRin,
+              double *dRout);
 int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
               const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs);
 int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
@@ -123,6 +128,8 @@ typedef struct BfMatHipButterfly {
   size_t bufBytes;
 
   double stats[16];
+  int orthogonal;      /* the factors have been orthogonalized */
+  double orthoSeconds; /* wall time of the orthogonalization */
 } BfMatHipButterfly;
 
 /* planner (plan.c) */

@@ -390,6 +390,146 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
   qr_jacobi_solve<64>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, 0.0, X + (size_t)blockIdx.x * p * n);
 }
 
+/* ---------------------------------------------------- orthogonalization */
+
+/* LDS of one orthogonalization job, in 16-byte units: the stacked column
+ * (rows x n, column-major, odd leading dimension), the diagonal of R and
+ * the Householder scales (n each). */
+__host__ __device__ __forceinline__ int ortho_ld(int rows) { return rows | 1; }
+__host__ __device__ __forceinline__ size_t ortho_units(int rows, int n) {
+  return (size_t)ortho_ld(rows) * n + 2 * (size_t)n;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+/* y -= v (v^H y) iv over rows [j, rows) of one column (a Householder
+ * reflector I - iv v v^H) */
+__device__ __forceinline__ void house_apply(const double2 *v, double2 *y, int j, int rows, double iv) {
+  double sr = 0, si = 0;
+  for (int r = j; r < rows; ++r) {
+    double2 vr = v[r], yr = y[r];
+    sr = fma(vr.x, yr.x, fma(vr.y, yr.y, sr));
+    si = fma(vr.x, yr.y, fma(-vr.y, yr.x, si));
+  }
+  sr *= iv;
+  si *= iv;
+  for (int r = j; r < rows; ++r) {
+    double2 vr = v[r], yr = y[r];
+    yr.x -= sr * vr.x - si * vr.y;
+    yr.y -= sr * vr.y + si * vr.x;
+    y[r] = yr;
+  }
+}
+
+/* One column block of one factor per wave: load its blocks, multiply each
+ * by its row's R_in (upper triangular, in place, rows ascending), then for
+ * rOut >= 0 a Householder QR of the stacked column: R goes to rOut and the
+ * blocks are overwritten by the matching rows of Q, formed in place from
+ * the reflectors (last reflector first). */
+__global__ __launch_bounds__(64) void k_ortho(const BfkOrthoJob *__restrict__ jobs, const int32_t *__restrict__ order,
+                                              const BfkOrthoItem *__restrict__ items, double2 *__restrict__ arena,
+                                              const double2 *__restrict__ rIn, double2 *__restrict__ rOut) {
+  extern __shared__ double2 lds[];
+  const BfkOrthoJob J = jobs[order[blockIdx.x]];
+  const int n = J.n, rows = J.rows, ld = ortho_ld(rows);
+  const int lane = threadIdx.x;
+  double2 *M = lds;
+  double2 *dg = M + (size_t)ld * n;
+  double *inv = (double *)(dg + n);
+
+  int off = 0;
+  for (int it = J.i0; it < J.i1; ++it) {
+    const BfkOrthoItem I = items[it];
+    const int m = I.m;
+    const double2 *B = arena + I.aOff;
+    for (int e = lane; e < m * n; e += 64) {
+      const int c = e / m, r = e - c * m;
+      M[c * ld + off + r] = B[e];
+    }
+    __syncthreads();
+    if (I.rIn >= 0) {
+      const double2 *R = rIn + I.rIn;
+      for (int c = lane; c < n; c += 64) {
+        double2 *col = M + c * ld + off;
+        for (int r = 0; r < m; ++r) {
+          double2 acc = make_double2(0, 0);
+          for (int k = r; k < m; ++k) acc = cfma(R[k * m + r], col[k], acc);
+          col[r] = acc;
+        }
+      }
+      __syncthreads();
+    }
+    off += m;
+  }
+
+  if (J.rOut >= 0) {
+    /* Householder QR, reflector j: v = x + ph sig e_j, H = I - v v^H / (sig (sig + |x_j|)) */
+    for (int j = 0; j < n; ++j) {
+      double s = 0;
+      for (int r = j + lane; r < rows; r += 64) {
+        double2 a = M[j * ld + r];
+        s = fma(a.x, a.x, fma(a.y, a.y, s));
+      }
+      s = wave_sum(s);
+      const double sig = sqrt(s);
+      const double2 x0 = M[j * ld + j];
+      const double ax0 = hypot(x0.x, x0.y);
+      const double phx = ax0 > 0 ? x0.x / ax0 : 1.0, phy = ax0 > 0 ? x0.y / ax0 : 0.0;
+      const double iv = sig > 0 ? 1.0 / (sig * (sig + ax0)) : 0.0;
+      __syncthreads();
+      if (lane == 0) {
+        if (sig > 0) M[j * ld + j] = make_double2(x0.x + phx * sig, x0.y + phy * sig);
+        dg[j] = make_double2(-phx * sig, -phy * sig);
+        inv[j] = iv;
+      }
+      __syncthreads();
+      if (iv != 0)
+        for (int c = j + 1 + lane; c < n; c += 64) house_apply(M + j * ld, M + c * ld, j, rows, iv);
+      __syncthreads();
+    }
+    double2 *Ro = rOut + J.rOut;
+    for (int e = lane; e < n * n; e += 64) {
+      const int c = e / n, r = e - c * n;
+      Ro[e] = r < c ? M[c * ld + r] : (r == c ? dg[c] : make_double2(0, 0));
+    }
+    __syncthreads();
+    for (int e = lane; e < n * n; e += 64) {
+      const int c = e / n, r = e - c * n;
+      if (r < c) M[c * ld + r] = make_double2(0, 0);
+    }
+    __syncthreads();
+    for (int j = n - 1; j >= 0; --j) {
+      const double iv = inv[j];
+      if (iv != 0)
+        for (int c = j + 1 + lane; c < n; c += 64) house_apply(M + j * ld, M + c * ld, j, rows, iv);
+      __syncthreads();
+      const double2 v0 = M[j * ld + j];
+      __syncthreads();
+      for (int r = j + lane; r < rows; r += 64) {
+        const double2 vr = M[j * ld + r];
+        const double tx = (vr.x * v0.x + vr.y * v0.y) * iv, ty = (vr.y * v0.x - vr.x * v0.y) * iv;
+        M[j * ld + r] = make_double2((r == j ? 1.0 : 0.0) - tx, -ty);
+      }
+      __syncthreads();
+    }
+  }
+
+  off = 0;
+  for (int it = J.i0; it < J.i1; ++it) {
+    const BfkOrthoItem I = items[it];
+    const int m = I.m;
+    double2 *B = arena + I.aOff;
+    for (int e = lane; e < m * n; e += 64) {
+      const int c = e / m, r = e - c * m;
+      B[e] = M[c * ld + off + r];
+    }
+    off += m;
+  }
+}
+
 /* ---------------------------------------------------------------- apply */
 
 struct BfkBufs {
@@ -420,12 +560,40 @@ template <typename V> struct BufsOf;
 template <> struct BufsOf<double2> { typedef BfkBufs T; };
 template <> struct BufsOf<float2> { typedef BfkBufsF T; };
 
+/* compensated (Kahan) accumulation: s + (-c) carries the running sum to
+ * about one rounding of the final value however long the chain */
+__device__ __forceinline__ void kahan_add(float &s, float &c, float p) {
+  const float y = p - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+__device__ __forceinline__ void kahan_addc(float2 &s, float2 &c, float2 p) {
+  kahan_add(s.x, c.x, p.x);
+  kahan_add(s.y, c.y, p.y);
+}
+/* add an FP32 accumulator tile into an FP64 running sum and clear it */
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+template <class A> __device__ __forceinline__ void fold64(mf_d4 &s, A &p) {
+  s += __builtin_convertvector(p, mf_d4);
+  p = A{};
+}
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
 /* Launch shape of the 1-RHS apply: waves (segments) per block and whether
  * a lane runs four independent FMA chains; set by bfk_set_apply_config. */
 static int g_applyWaves = 1;
 static int g_applyUnroll = 0;
 /* right-hand-side tiles of 16 per wave in the MFMA apply (0 = off) */
 static int g_mfmaCT = 1;
+/* compensated accumulation in the FP32 applies (1 = on) */
+static int g_comp32 = 1;
+/* the MFMA apply folds its partial FP32 tile into the FP64 sum every
+ * (mask + 1) 4-column steps */
+static int g_foldMask = 0;
+__constant__ int g_foldMaskDev = 0;
 
 /* One stage of the apply for one right-hand side.  Each wave of the block
  * owns one output row segment: its lanes split into R rows x G = 64/R
@@ -433,8 +601,10 @@ static int g_mfmaCT = 1;
  * 64); a lane walks its columns of every term (column-major blocks, so the
  * R lanes of a group read R consecutive entries), then the G partial sums
  * meet in a shuffle reduction and the row is written once.  With LO the
- * factors are an FP32 pair hi + lo. */
-template <typename V, bool LO, bool UNROLL>
+ * factors are an FP32 pair hi + lo.  With COMP (FP32 only) every lane
+ * sums its products with Kahan compensation, so the rounding of a stage no
+ * longer grows with the length of its rows. */
+template <typename V, bool LO, bool UNROLL, bool COMP = false>
 __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
                                                 const BfkTerm *__restrict__ terms, const V *__restrict__ hi,
                                                 const V *__restrict__ lo, typename BufsOf<V>::T bufs) {
@@ -456,6 +626,21 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
       for (int t = S.t0; t < S.t1; ++t) {
         const BfkTerm T = terms[t];
         const V *x = bufs.in[T.xbuf] + T.xOff;
+        if constexpr (COMP) {
+          /* a0 = running sum, a1 = compensation */
+          if (T.aOff < 0) {
+            if (g == 0) kahan_addc(a0, a1, x[i]);
+            continue;
+          }
+          const int64_t lda = T.lda;
+          const V *A = hi + T.aOff + T.rowOff + i;
+          for (int j = g; j < T.n; j += G) {
+            V p = cmulf(A[(int64_t)j * lda], x[j]);
+            if (LO) p = cfmaf(lo[T.aOff + T.rowOff + i + (int64_t)j * lda], x[j], p);
+            kahan_addc(a0, a1, p);
+          }
+          continue;
+        }
         if (T.aOff < 0) {
           if (g == 0) a0 = cadd(a0, x[i]);
           continue;
@@ -487,7 +672,13 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
         }
       }
     }
-    V acc = cadd(cadd(a0, a1), cadd(a2, a3));
+    V acc;
+    if constexpr (COMP) {
+      acc.x = a0.x - a1.x;
+      acc.y = a0.y - a1.y;
+    } else {
+      acc = cadd(cadd(a0, a1), cadd(a2, a3));
+    }
     for (int o = R; o < 64; o <<= 1) {
       acc.x += __shfl_xor(acc.x, o, 64);
       acc.y += __shfl_xor(acc.y, o, 64);
@@ -535,7 +726,6 @@ __global__ __launch_bounds__(64) void k_applyN(const BfkSeg *__restrict__ segs, 
 
 /* ------------------------------------------- many right-hand sides: MFMA */
 
-typedef double mf_d4 __attribute__((ext_vector_type(4)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 
 /* 16x16x4 MFMA on one wave, real operands.  Operand lanes: A[l%16][l/16],
@@ -566,27 +756,44 @@ struct MfmaF32 {
  * looping over its row tiles, terms and 4-column steps of each term.  The
  * complex product is four real MFMAs per 16x16x4 step (re*re - im*im,
  * re*im + im*re).  blockIdx.x picks the column chunk so the chunks of one
- * segment run back to back and share its factor blocks in L2. */
-template <class M, int CT, bool LO>
+ * segment share its factor blocks in one L2.  With COMP the matrix cores
+ * accumulate a few 4-column steps into a fresh FP32 tile, which is then
+ * added to an FP64 running sum, so rounding no longer grows with the
+ * length of the rows. */
+template <class M, int CT, bool LO, bool COMP = false>
 __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ segs,
                                                     const BfkTerm *__restrict__ terms,
                                                     const typename M::V *__restrict__ hi,
                                                     const typename M::V *__restrict__ lo,
                                                     typename BufsOf<typename M::V>::T bufs, int nrhs,
-                                                    int numChunks) {
+                                                    int numChunks, int64_t numSegs) {
   typedef typename M::S S;
   typedef typename M::V V;
   typedef typename M::Acc Acc;
-  const BfkSeg Sg = segs[blockIdx.x / numChunks];
+  /* workgroups go to the 8 XCDs round-robin by id: ids b with the same
+   * b % 8 share an XCD (and its L2), so all column chunks of a segment get
+   * ids 8 (group * numChunks + chunk) + xcd and read its blocks from one L2 */
+  const int64_t slot = blockIdx.x >> 3;
+  const int64_t grp = slot / numChunks;
+  const int chunk = (int)(slot - grp * numChunks);
+  const int64_t si = grp * 8 + (blockIdx.x & 7);
+  if (si >= numSegs) return;
+  const BfkSeg Sg = segs[si];
   const int l = threadIdx.x;
-  const int c0 = (blockIdx.x % numChunks) * 16 * CT;
+  const int c0 = chunk * 16 * CT;
   const int m = Sg.m;
   const int lr = l & 15, lk = l >> 4;
   V *y = bufs.out[Sg.ybuf];
   for (int rt = 0; rt < m; rt += 16) {
-    Acc re[CT], im[CT];
+    /* re/im: the tile's sum; with COMP, pre/pim collect up to 16 columns
+     * and cre/cim hold the Kahan compensation of re/im */
+    Acc re[CT], im[CT], pre[CT], pim[CT];
+    mf_d4 re64[CT], im64[CT];
     for (int q = 0; q < CT; ++q)
-      for (int r = 0; r < 4; ++r) re[q][r] = im[q][r] = 0;
+      for (int r = 0; r < 4; ++r) {
+        re[q][r] = im[q][r] = pre[q][r] = pim[q][r] = 0;
+        re64[q][r] = im64[q][r] = 0;
+      }
     for (int t = Sg.t0; t < Sg.t1; ++t) {
       const BfkTerm T = terms[t];
       const V *x = bufs.in[T.xbuf];
@@ -597,11 +804,21 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
             const int i = rt + M::row(l, r);
             if (i < m && col < nrhs) {
               V v = x[(T.xOff + i) * (int64_t)nrhs + col];
-              re[q][r] += v.x;
-              im[q][r] += v.y;
+              if constexpr (COMP) {
+                pre[q][r] += v.x;
+                pim[q][r] += v.y;
+              } else {
+                re[q][r] += v.x;
+                im[q][r] += v.y;
+              }
             }
           }
         }
+        if constexpr (COMP)
+          for (int q = 0; q < CT; ++q) {
+            fold64(re64[q], pre[q]);
+            fold64(im64[q], pim[q]);
+          }
         continue;
       }
       const int n = T.n;
@@ -615,16 +832,25 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
         for (int q = 0; q < CT; ++q) {
           const int col = c0 + 16 * q + lr;
           V b = (k < n && col < nrhs) ? x[(T.xOff + k) * (int64_t)nrhs + col] : czero<V>();
-          re[q] = M::mma(a.x, b.x, re[q]);
-          re[q] = M::mma(-a.y, b.y, re[q]);
-          im[q] = M::mma(a.x, b.y, im[q]);
-          im[q] = M::mma(a.y, b.x, im[q]);
+          Acc &dr = COMP ? pre[q] : re[q];
+          Acc &di = COMP ? pim[q] : im[q];
+          dr = M::mma(a.x, b.x, dr);
+          dr = M::mma(-a.y, b.y, dr);
+          di = M::mma(a.x, b.y, di);
+          di = M::mma(a.y, b.x, di);
           if (LO) {
-            re[q] = M::mma(al.x, b.x, re[q]);
-            re[q] = M::mma(-al.y, b.y, re[q]);
-            im[q] = M::mma(al.x, b.y, im[q]);
-            im[q] = M::mma(al.y, b.x, im[q]);
+            dr = M::mma(al.x, b.x, dr);
+            dr = M::mma(-al.y, b.y, dr);
+            di = M::mma(al.x, b.y, di);
+            di = M::mma(al.y, b.x, di);
           }
+        }
+        if constexpr (COMP) {
+          if (((k0 >> 2) & g_foldMaskDev) == g_foldMaskDev || k0 + 4 >= n)
+            for (int q = 0; q < CT; ++q) {
+              fold64(re64[q], pre[q]);
+              fold64(im64[q], pim[q]);
+            }
         }
       }
     }
@@ -634,8 +860,13 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
         const int i = rt + M::row(l, r);
         if (i < m && col < nrhs) {
           V v;
-          v.x = (S)re[q][r];
-          v.y = (S)im[q][r];
+          if constexpr (COMP) {
+            v.x = (S)re64[q][r];
+            v.y = (S)im64[q][r];
+          } else {
+            v.x = (S)re[q][r];
+            v.y = (S)im[q][r];
+          }
           y[(Sg.yOff + i) * (int64_t)nrhs + col] = v;
         }
       }
@@ -686,6 +917,8 @@ int bfk_init(void) {
     if (hipFuncSetAttribute((const void *)k_lstsq<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)g_ldsCap) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_lstsq_loaded, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_ldsCap) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_ortho, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)g_ldsCap) != hipSuccess) {
       (void)hipGetLastError();
       g_ldsCap = 65536;
@@ -850,6 +1083,29 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
   return time_end(1);
 }
 
+size_t bfk_ortho_units(int rows, int n) { return ortho_units(rows, n); }
+
+/* orthogonalization jobs of one stage; `dOrder` lists job indices grouped
+ * by LDS need as for bfk_lstsq, and every group must fit in LDS */
+int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
+              const size_t *groupBytes, const BfkOrthoItem *dItems, double *dArena, const double *dRin,
+              double *dRout) {
+  if (bfk_init()) return -1;
+  for (int gi = 0; gi < numGroups; ++gi) {
+    int64_t s = groupStart[gi], e = groupStart[gi + 1];
+    if (e <= s) continue;
+    if (groupBytes[gi] > g_ldsCap) return -3;
+    const int64_t maxGrid = 1 << 30;
+    for (int64_t j0 = s; j0 < e; j0 += maxGrid) {
+      int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
+      hipLaunchKernelGGL(k_ortho, dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs, dOrder + j0, dItems,
+                         (double2 *)dArena, (const double2 *)dRin, (double2 *)dRout);
+      BFK_CHECK(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
 /* one apply: `numStages` launches; stage s covers segments
  * [stageSeg[s], stageSeg[s+1]) */
 int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
@@ -880,19 +1136,19 @@ int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const
     } else if (nrhs >= 16 && g_mfmaCT > 0) {
       const int ct = g_mfmaCT;
       const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
-      dim3 grid((unsigned)(n * chunks));
+      dim3 grid((unsigned)(((n + 7) / 8) * 8 * chunks));
       if (ct == 1)
         hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 1, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
       else if (ct == 2)
         hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 2, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
       else if (ct == 8)
         hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 8, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
       else
         hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 4, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks);
+                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
     } else {
       hipLaunchKernelGGL(k_applyN, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
                          dSegs + stageSeg[s], dTerms, (const double2 *)dArena, b, nrhs);
@@ -1005,7 +1261,42 @@ __global__ void k_to_f32(const double2 *__restrict__ a, int64_t n, float2 *__res
   }
 }
 
+template <bool LO, bool COMP>
+static void launch_apply1_f32(dim3 grid, dim3 block, const BfkSeg *segs, int64_t n, const BfkTerm *terms,
+                              const float2 *H, const float2 *Lo, BfkBufsF b) {
+  hipLaunchKernelGGL((k_apply1<float2, LO, false, COMP>), grid, block, 0, g_stream, segs, n, terms, H, Lo, b);
+}
+
+template <bool LO, bool COMP>
+static void launch_mfma_f32(int ct, dim3 grid, const BfkSeg *segs, const BfkTerm *terms, const float2 *H,
+                            const float2 *Lo, BfkBufsF b, int nrhs, int chunks, int64_t numSegs) {
+  if (ct == 1)
+    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 1, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
+                       nrhs, chunks, numSegs);
+  else if (ct == 2)
+    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 2, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
+                       nrhs, chunks, numSegs);
+  else if (ct == 8)
+    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 8, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
+                       nrhs, chunks, numSegs);
+  else
+    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 4, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
+                       nrhs, chunks, numSegs);
+}
+
 extern "C" {
+
+int bfk_set_compensation(int on) {
+  g_comp32 = on ? 1 : 0;
+  int mask = on >= 2 ? on - 2 : 0; /* on = 2 + mask selects a longer fold interval */
+  if (mask != 0 && mask != 1 && mask != 3 && mask != 7) return -1;
+  if (mask != g_foldMask) {
+    if (bfk_init()) return -1;
+    BFK_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_foldMaskDev), &mask, sizeof(int)));
+    g_foldMask = mask;
+  }
+  return 0;
+}
 
 int bfk_to_f32(const double *dArena, int64_t numEntries, float *dHi, float *dLo) {
   if (bfk_init()) return -1;
@@ -1036,35 +1327,30 @@ int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, c
     if (nrhs == 1) {
       const int wv = g_applyWaves;
       dim3 grid((unsigned)((n + wv - 1) / wv)), block(64 * wv);
-      if (dArenaLo)
-        hipLaunchKernelGGL((k_apply1<float2, true, false>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
-                           dTerms, (const float2 *)dArenaHi, (const float2 *)dArenaLo, b);
-      else if (g_applyUnroll)
+      const float2 *H = (const float2 *)dArenaHi, *Lo = (const float2 *)dArenaLo;
+      if (g_comp32) {
+        if (Lo) launch_apply1_f32<true, true>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
+        else launch_apply1_f32<false, true>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
+      } else if (Lo) {
+        launch_apply1_f32<true, false>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
+      } else if (g_applyUnroll) {
         hipLaunchKernelGGL((k_apply1<float2, false, true>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
-                           dTerms, (const float2 *)dArenaHi, (const float2 *)nullptr, b);
-      else
-        hipLaunchKernelGGL((k_apply1<float2, false, false>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
-                           dTerms, (const float2 *)dArenaHi, (const float2 *)nullptr, b);
+                           dTerms, H, Lo, b);
+      } else {
+        launch_apply1_f32<false, false>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
+      }
     } else if (nrhs >= 16 && g_mfmaCT > 0) {
       const int ct = g_mfmaCT;
       const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
-      dim3 grid((unsigned)(n * chunks));
+      dim3 grid((unsigned)(((n + 7) / 8) * 8 * chunks));
       const float2 *H = (const float2 *)dArenaHi, *Lo = (const float2 *)dArenaLo;
-      if (Lo)
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 4, true>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, H, Lo, b, nrhs, chunks);
-      else if (ct == 1)
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 1, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, H, Lo, b, nrhs, chunks);
-      else if (ct == 2)
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 2, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, H, Lo, b, nrhs, chunks);
-      else if (ct == 8)
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 8, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, H, Lo, b, nrhs, chunks);
-      else
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 4, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, H, Lo, b, nrhs, chunks);
+      if (Lo) {
+        if (g_comp32) launch_mfma_f32<true, true>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
+        else launch_mfma_f32<true, false>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
+      } else {
+        if (g_comp32) launch_mfma_f32<false, true>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
+        else launch_mfma_f32<false, false>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
+      }
     } else {
       hipLaunchKernelGGL(k_applyN_f32, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
                          dSegs + stageSeg[s], dTerms, (const float2 *)dArenaHi, (const float2 *)dArenaLo, b, nrhs);

@@ -62,6 +62,27 @@ typedef struct BfkTerm {
   int32_t xbuf;
 } BfkTerm;
 
+/* Orthogonalization of a butterfly (a change of basis of every
+ * intermediate vector; the operator is unchanged).  One job is one column
+ * block of one factor: its blocks (items [i0, i1)) are first multiplied on
+ * the left by the triangular factor R_in of their row (if any), then, when
+ * rOut >= 0, the stacked column (rows x n) is replaced by the Q of its
+ * thin QR and R (n x n, column-major) is written at rOut for the rows of
+ * the previous factor. */
+typedef struct BfkOrthoJob {
+  int64_t rOut;
+  int32_t n, rows;
+  int32_t i0, i1;
+} BfkOrthoJob;
+
+/* One block of an orthogonalization job: m x n column-major at aOff;
+ * rIn >= 0 is the offset of its row's m x m R_in in the input R buffer. */
+typedef struct BfkOrthoItem {
+  int64_t aOff;
+  int64_t rIn;
+  int32_t m, pad;
+} BfkOrthoItem;
+
 #ifdef __cplusplus
 }
 #endif

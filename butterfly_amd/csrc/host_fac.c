@@ -535,6 +535,222 @@ int bfhipFacReleaseFp64(BfMat *fac) {
   return 0;
 }
 
+/* ---- orthogonalization
+ *
+ * For every butterfly leaf, from its last factor back to its first: the
+ * blocks of one column of factor s (s >= 1) are stacked, multiplied on the
+ * left by the triangular factors their rows received from factor s + 1,
+ * and replaced by the Q of a thin QR; the R goes to the row of factor
+ * s - 1 that feeds this column.  Then the map from every intermediate
+ * vector to the output has orthonormal columns, so rounding an
+ * intermediate (or a block) by eps changes the output by about eps
+ * relative, instead of eps times the cancellation of equivalent charges.
+ * Only columns fed by proxy rows (whose blocks are all re-expansions) are
+ * rotated; identity and point-value passes stay as they are. */
+
+typedef BF_VEC(BfkOrthoJob) OrthoJobVec;
+typedef BF_VEC(BfkOrthoItem) OrthoItemVec;
+
+static void ortho_push(OrthoJobVec *jobs, OrthoItemVec *items, const BfkOrthoItem *it, int nit, int n, int64_t rOut) {
+  BfkOrthoJob j;
+  j.rOut = rOut;
+  j.n = n;
+  j.rows = 0;
+  j.i0 = (int32_t)items->n;
+  for (int q = 0; q < nit; ++q) {
+    BF_VEC_PUSH(*items, it[q]);
+    j.rows += it[q].m;
+  }
+  j.i1 = (int32_t)items->n;
+  BF_VEC_PUSH(*jobs, j);
+}
+
+int bfhipFacOrthogonalize(BfMat *fac) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (!F->dArena) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  if (F->orthogonal) return 0;
+  double t0 = bfWallTime();
+  const int Fmax = F->numStages;
+  const size_t cap = bfk_lds_cap() / 16;
+  OrthoJobVec *jobs = calloc((size_t)Fmax, sizeof(OrthoJobVec));
+  OrthoItemVec items;
+  memset(&items, 0, sizeof(items));
+  int64_t *rFill = calloc((size_t)Fmax, sizeof(int64_t));
+  BF_VEC(BfkOrthoItem) col;
+  memset(&col, 0, sizeof(col));
+  BF_VEC(int64_t) pend, npend, cstart, corder;
+  memset(&pend, 0, sizeof(pend));
+  memset(&npend, 0, sizeof(npend));
+  memset(&cstart, 0, sizeof(cstart));
+  memset(&corder, 0, sizeof(corder));
+  BF_VEC(char) rowOK;
+  memset(&rowOK, 0, sizeof(rowOK));
+
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    BfHipLeaf *lf = &F->leaves.a[l];
+    if (!lf->isButterfly) continue;
+    const int nf = lf->numFactors;
+    pend.n = 0;
+    for (int s = nf - 1; s >= 0; --s) {
+      BfHipFactor *fa = &F->factors.a[lf->firstFactor + s];
+      const int g = Fmax - nf + s;
+      const int64_t nbc = fa->numBlockCols, b0 = fa->firstBlock, b1 = fa->firstBlock + fa->numBlocks;
+      /* rows of factor s - 1 that can take a triangular factor: all
+       * re-expansion blocks, each small enough to be rotated alone */
+      rowOK.n = 0;
+      if (s >= 1) {
+        BfHipFactor *fp = &F->factors.a[lf->firstFactor + s - 1];
+        for (int64_t r = 0; r < fp->numBlockRows; ++r) BF_VEC_PUSH(rowOK, 1);
+        for (int64_t b = fp->firstBlock; b < fp->firstBlock + fp->numBlocks; ++b) {
+          BfHipBlock *bl = &F->blocks.a[b];
+          if (bl->kind != BF_BLOCK_LSTSQ || bl->dataOff < 0 || bfk_ortho_units(bl->m, bl->n) > cap)
+            rowOK.a[bl->rowInd] = 0;
+        }
+      }
+      /* blocks of factor s grouped by column (counting sort) */
+      cstart.n = 0;
+      for (int64_t c = 0; c <= nbc; ++c) BF_VEC_PUSH(cstart, 0);
+      for (int64_t b = b0; b < b1; ++b) cstart.a[F->blocks.a[b].colInd + 1]++;
+      for (int64_t c = 0; c < nbc; ++c) cstart.a[c + 1] += cstart.a[c];
+      corder.n = 0;
+      for (int64_t b = b0; b < b1; ++b) BF_VEC_PUSH(corder, 0);
+      for (int64_t b = b0; b < b1; ++b) corder.a[cstart.a[F->blocks.a[b].colInd]++] = b;
+      for (int64_t c = nbc; c > 0; --c) cstart.a[c] = cstart.a[c - 1];
+      cstart.a[0] = 0;
+      npend.n = 0;
+      for (int64_t c = 0; c < nbc; ++c) BF_VEC_PUSH(npend, -1);
+      for (int64_t c = 0; c < nbc; ++c) {
+        col.n = 0;
+        int n = 0, rows = 0, allData = 1, anyPend = 0;
+        for (int64_t q = cstart.a[c]; q < cstart.a[c + 1]; ++q) {
+          BfHipBlock *bl = &F->blocks.a[corder.a[q]];
+          BfkOrthoItem it;
+          it.aOff = bl->dataOff;
+          it.rIn = pend.n ? pend.a[bl->rowInd] : -1;
+          it.m = bl->m;
+          it.pad = 0;
+          n = bl->n;
+          rows += bl->m;
+          if (bl->dataOff < 0) allData = 0;
+          if (it.rIn >= 0) anyPend = 1;
+          BF_VEC_PUSH(col, it);
+        }
+        if (col.n == 0) continue;
+        const int fits = bfk_ortho_units(rows, n) <= cap;
+        if (s >= 1 && rowOK.a[c] && allData && rows >= n && fits) {
+          const int64_t rOut = rFill[g];
+          rFill[g] += (int64_t)n * n;
+          ortho_push(&jobs[g], &items, col.a, (int)col.n, n, rOut);
+          npend.a[c] = rOut;
+        } else if (anyPend) {
+          /* only the blocks whose rows carry a triangular factor change */
+          size_t k = 0;
+          int prow = 0;
+          for (size_t q = 0; q < col.n; ++q)
+            if (col.a[q].rIn >= 0) { col.a[k++] = col.a[q]; prow += col.a[q].m; }
+          if (bfk_ortho_units(prow, n) <= cap) ortho_push(&jobs[g], &items, col.a, (int)k, n, -1);
+          else
+            for (size_t q = 0; q < k; ++q) ortho_push(&jobs[g], &items, &col.a[q], 1, n, -1);
+        }
+      }
+      /* the next (earlier) factor's rows are this factor's columns */
+      pend.n = 0;
+      if (s >= 1)
+        for (int64_t c = 0; c < nbc; ++c) BF_VEC_PUSH(pend, npend.a[c]);
+    }
+  }
+  BF_VEC_FREE(col);
+  BF_VEC_FREE(pend);
+  BF_VEC_FREE(npend);
+  BF_VEC_FREE(cstart);
+  BF_VEC_FREE(corder);
+  BF_VEC_FREE(rowOK);
+
+  /* two R buffers, alternating by stage parity: stage g writes buffer
+   * g % 2, stage g - 1 reads it */
+  int64_t rMax[2] = {0, 0};
+  for (int g = 0; g < Fmax; ++g)
+    if (rFill[g] > rMax[g % 2]) rMax[g % 2] = rFill[g];
+  int rc = 0;
+  double *dR[2] = {NULL, NULL};
+  void *dItems = NULL, *dJobs = NULL, *dOrder = NULL;
+  for (int q = 0; q < 2 && !rc; ++q) {
+    dR[q] = bfk_malloc((size_t)(rMax[q] ? rMax[q] : 1) * 16);
+    if (!dR[q]) rc = -2;
+  }
+  if (!rc) rc = upload(&dItems, items.a, items.n * sizeof(BfkOrthoItem));
+  size_t maxJobs = 0;
+  for (int g = 0; g < Fmax; ++g)
+    if (jobs[g].n > maxJobs) maxJobs = jobs[g].n;
+  if (!rc) {
+    dJobs = bfk_malloc((maxJobs ? maxJobs : 1) * sizeof(BfkOrthoJob));
+    dOrder = bfk_malloc((maxJobs ? maxJobs : 1) * sizeof(int32_t));
+    if (!dJobs || !dOrder) rc = -2;
+  }
+  JobKey *keys = malloc((maxJobs ? maxJobs : 1) * sizeof(JobKey));
+  int32_t *order = malloc((maxJobs ? maxJobs : 1) * sizeof(int32_t));
+  BF_VEC(int64_t) gs;
+  BF_VEC(size_t) gb;
+  memset(&gs, 0, sizeof(gs));
+  memset(&gb, 0, sizeof(gb));
+  for (int g = Fmax - 1; g >= 0 && !rc; --g) {
+    size_t nj = jobs[g].n;
+    if (nj == 0) continue;
+    for (size_t i = 0; i < nj; ++i) {
+      size_t b = bfk_ortho_units(jobs[g].a[i].rows, jobs[g].a[i].n) * 16;
+      keys[i].bytes = (b + 4095) & ~(size_t)4095;
+      if (keys[i].bytes > cap * 16) keys[i].bytes = cap * 16;
+      keys[i].idx = (int32_t)i;
+    }
+    qsort(keys, nj, sizeof(JobKey), cmp_jobkey);
+    gs.n = gb.n = 0;
+    for (size_t i = 0; i < nj; ++i) {
+      order[i] = keys[i].idx;
+      if (i == 0 || keys[i].bytes != keys[i - 1].bytes) {
+        BF_VEC_PUSH(gs, (int64_t)i);
+        BF_VEC_PUSH(gb, keys[i].bytes);
+      }
+    }
+    BF_VEC_PUSH(gs, (int64_t)nj);
+    rc = bfk_h2d(dJobs, jobs[g].a, nj * sizeof(BfkOrthoJob));
+    if (!rc) rc = bfk_h2d(dOrder, order, nj * sizeof(int32_t));
+    if (!rc) rc = bfk_ortho(dJobs, dOrder, (int)gb.n, gs.a, gb.a, dItems, F->dArena, dR[(g + 1) % 2], dR[g % 2]);
+    if (!rc) rc = bfk_sync();
+  }
+  BF_VEC_FREE(gs);
+  BF_VEC_FREE(gb);
+  free(keys);
+  free(order);
+  bfk_free(dJobs);
+  bfk_free(dOrder);
+  bfk_free(dItems);
+  bfk_free(dR[0]);
+  bfk_free(dR[1]);
+  for (int g = 0; g < Fmax; ++g) BF_VEC_FREE(jobs[g]);
+  free(jobs);
+  free(rFill);
+  BF_VEC_FREE(items);
+  if (rc) {
+    bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : BF_ERROR_RUNTIME_ERROR);
+    return rc;
+  }
+  F->orthogonal = 1;
+  F->orthoSeconds = bfWallTime() - t0;
+  return 0;
+}
+
+double bfhipFacOrthogonalizeSeconds(BfMat const *fac) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable) return -1;
+  return ((BfMatHipButterfly const *)fac)->orthoSeconds;
+}
+
 BfMat *bfMatHipButterflyMul(BfMat const *A, BfMat const *Xm) {
   BfMatHipButterfly *F = (BfMatHipButterfly *)A;
   if (!Xm || bfMatGetType(Xm) != BF_TYPE_MAT_DENSE_COMPLEX || Xm->numRows != F->numSrc) {
@@ -597,6 +813,8 @@ int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles) {
   if (rc) bfSetError(BF_ERROR_INVALID_ARGUMENTS);
   return rc;
 }
+
+int bfhipSetApplyCompensation(int on) { return bfk_set_compensation(on); }
 
 void bfhipFacPlanSizes(BfMat const *fac, BfSize *numLeaves, BfSize *numFactors, BfSize *numBlocks) {
   BfMatHipButterfly const *F = (BfMatHipButterfly const *)fac;

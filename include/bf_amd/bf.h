@@ -335,6 +335,17 @@ int bfhipFacPrepareMixed(BfMat *fac, BfhipPrecision prec);
  * has FP32 factors, leaving only the FP32 applies; for callers that keep
  * a mixed factorization resident and need the HBM */
 int bfhipFacReleaseFp64(BfMat *fac);
+/* new (no counterpart): change the basis of every intermediate vector of
+ * the butterfly so that the map from it to the output has orthonormal
+ * columns (a backward pass of thin QRs over the column blocks of each
+ * factor, in FP64 on the device).  The operator is unchanged up to FP64
+ * rounding; reduced-precision applies of the result lose about eps
+ * relative per stage instead of eps times the cancellation of the
+ * equivalent charges.  Call it before bfhipFacPrepareMixed.  Block exports
+ * afterwards return the rotated blocks. */
+int bfhipFacOrthogonalize(BfMat *fac);
+/* new: wall seconds of the last bfhipFacOrthogonalize, -1 on a bad argument */
+double bfhipFacOrthogonalizeSeconds(BfMat const *fac);
 
 /* new (no counterpart): factorization statistics
  * out[0] factor wall seconds, [1] tree+plan seconds, [2] sampling+compression
@@ -358,6 +369,10 @@ void bfhipResetKernelTimes(void);
  * tiles of 16 per wave on the matrix cores in {2, 4, 8}, or 0 for the
  * vector-unit kernel */
 int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
+/* new: compensated (Kahan) accumulation in the FP32 applies, on (1, the
+ * default) or off (0): the FP32 rounding of a stage then stays near one
+ * unit in the last place however long its rows are */
+int bfhipSetApplyCompensation(int on);
 
 /* new: plan export for parity checks (integers, in the oracle's canonical
  * order): leafTab [L x 6], facTab [F x 4], blkTab [B x 7] */

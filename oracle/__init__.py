@@ -46,6 +46,8 @@ def lib():
         "orc_set_rank_const": (None, [d]),
         "orc_set_pinv_rcond": (None, [d]),
         "orc_set_impedance": (None, [C.c_int]),
+        "orc_set_field_form": (None, [C.c_int]),
+        "orc_set_test_oversample": (None, [d]),
         "orc_fac_free": (None, [vp]),
         "orc_fac_perms": (None, [vp, C.POINTER(sz), C.POINTER(sz)]),
         "orc_fac_plan_sizes": (None, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
@@ -151,17 +153,19 @@ class Fac:
     """The oracle's factorization (reference rules, or pass-through)."""
 
     def __init__(self, k, src, tgt, leaf_size=1, tol=1e-15, max_dense=128 * 128, pass_through=False,
-                 plan_only=False, rank_const=1.0, pinv_rcond=0.0):
+                 plan_only=False, rank_const=1.0, pinv_rcond=0.0, field_form=False):
         self.src = np.ascontiguousarray(src, np.float64)
         self.tgt = np.ascontiguousarray(tgt, np.float64)
         lib().orc_set_plan_only(int(plan_only))
         lib().orc_set_rank_const(rank_const)
         lib().orc_set_pinv_rcond(pinv_rcond)
+        lib().orc_set_field_form(int(field_form))
         self.h = lib().orc_fac_make(k, _dp(self.src), len(self.src), _dp(self.tgt), len(self.tgt),
                                     leaf_size, tol, max_dense, int(pass_through))
         lib().orc_set_plan_only(0)
         lib().orc_set_rank_const(1.0)
         lib().orc_set_pinv_rcond(0.0)
+        lib().orc_set_field_form(0)
 
     def perms(self):
         sp = np.zeros(len(self.src), np.uintp)

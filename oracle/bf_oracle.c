@@ -445,7 +445,7 @@ void orc_svd_values(size_t m, size_t n, const double *A, double *sigma) {
 
 /* ------------------------------------------------------------ factorization */
 
-enum { BK_LSTSQ = 0, BK_KERNEL = 1, BK_IDENTITY = 2 };
+enum { BK_LSTSQ = 0, BK_KERNEL = 1, BK_IDENTITY = 2, BK_FIELD = 3 };
 enum { MODE_PROXY = 0, MODE_CHARGES = 1, MODE_VALUES = 2 };
 
 typedef struct {
@@ -588,6 +588,40 @@ static cplx *reexpansion(double k, const OPtSet *orig, const OPtSet *equiv, cons
   return X;
 }
 
+/* Field form: the intermediates of the apply are field samples on the test
+ * circles instead of equivalent charges.  With f = K(test <- equiv) q the
+ * product E Z_L^+ T_L ... T_1 Z_0^+ O_0 of the charge form is re-associated
+ * as (E Z_L^+)(T_L Z_{L-1}^+)...(T_1 Z_0^+) O_0, so both forms are the same
+ * operator with the same truncation.  A field block is
+ *   B = K(X <- equiv) pinv(K(test <- equiv)),
+ * computed through the transpose: the kernel is symmetric, so
+ * B^T = pinv(K(equiv <- test)) K(equiv <- X) is an ordinary least-squares
+ * problem with the same singular values. */
+static int orc_field_form = 0;
+void orc_set_field_form(int v) { orc_field_form = v; }
+
+static cplx *field_block(double k, const OPtSet *X, const OPtSet *equiv, const OPtSet *test) {
+  if (orc_plan_only) return NULL;
+  size_t p = equiv->n, q = test->n, M = X->n;
+  cplx *A = kernel_matrix(k, test, equiv);   /* p x q: K(equiv <- test) */
+  cplx *Bm = kernel_matrix(k, X, equiv);     /* p x M: K(equiv <- X) */
+  cplx *Xt = malloc(q * M * sizeof(cplx));   /* q x M = B^T */
+  lstsq(p, q, A, M, Bm, Xt);
+  cplx *B = malloc(M * q * sizeof(cplx));
+  for (size_t i = 0; i < M; ++i)
+    for (size_t j = 0; j < q; ++j) B[i * q + j] = Xt[j * M + i];
+  free(A); free(Bm); free(Xt);
+  return B;
+}
+
+/* number of test points for p proxies: p under the reference rules, more
+ * when oversampling is set */
+static double orc_test_oversample = 1.0;
+void orc_set_test_oversample(double v) { orc_test_oversample = v; }
+static size_t test_count(size_t p) {
+  return orc_test_oversample > 1.0 ? (size_t)ceil(orc_test_oversample * (double)p) : p;
+}
+
 static cplx *identity(size_t n) {
   if (orc_plan_only) return NULL;
   cplx *E = calloc(n * n, sizeof(cplx));
@@ -637,9 +671,13 @@ static void make_butterfly(OFac *F, OLeaf *leaf, OLevel *Ls, OLevel *Lt, size_t 
       b->m = ns; b->kind = BK_IDENTITY; b->data = identity(ns);
     } else {
       mode[i] = MODE_PROXY;
-      b->m = p; b->kind = BK_LSTSQ;
-      OPtSet orig = node_points(F, 1, s), equiv = node_circle(F, 1, s, p), test = {1, cT, NULL, p};
-      b->data = reexpansion(k, &orig, &equiv, &test);
+      b->m = p;
+      OPtSet orig = node_points(F, 1, s), equiv = node_circle(F, 1, s, p), test = {1, cT, NULL, test_count(p)};
+      if (orc_field_form) {
+        b->kind = BK_KERNEL; b->data = kernel_matrix(k, &orig, &test);
+      } else {
+        b->kind = BK_LSTSQ; b->data = reexpansion(k, &orig, &equiv, &test);
+      }
     }
     msize[i] = b->m;
     rs[i] = b->m; cs[i] = b->n;
@@ -719,16 +757,24 @@ static void make_butterfly(OFac *F, OLeaf *leaf, OLevel *Ls, OLevel *Lt, size_t 
             if ((cm == MODE_CHARGES && rm == MODE_CHARGES) || (cm == MODE_VALUES && rm == MODE_VALUES)) {
               if (b->m != b->n) abort();
               b->kind = BK_IDENTITY; b->data = identity(b->m);
-            } else if (rm == MODE_VALUES) {
+            } else if (orc_field_form && cm == MODE_PROXY) {
+              /* field on the parent target's circle -> field at the child
+               * target's circle (or its points) */
+              b->kind = BK_FIELD;
+              OPtSet X = rm == MODE_VALUES ? node_points(F, 0, tc) : node_circle(F, 0, tc, rows);
+              OPtSet equiv = node_circle(F, 1, skids[c], b->n);
+              OPtSet test = node_circle(F, 0, tPar->nd[tp], b->n);
+              b->data = field_block(k, &X, &equiv, &test);
+            } else if (rm == MODE_VALUES || orc_field_form) {
               b->kind = BK_KERNEL;
-              OPtSet tgt = node_points(F, 0, tc);
+              OPtSet tgt = rm == MODE_VALUES ? node_points(F, 0, tc) : node_circle(F, 0, tc, rows);
               OPtSet src = cm == MODE_CHARGES ? node_points(F, 1, skids[c]) : node_circle(F, 1, skids[c], b->n);
               b->data = kernel_matrix(k, &src, &tgt);
             } else { /* rm == PROXY */
               b->kind = BK_LSTSQ;
               OPtSet orig = cm == MODE_CHARGES ? node_points(F, 1, skids[c]) : node_circle(F, 1, skids[c], b->n);
               OPtSet equiv = node_circle(F, 1, sPar->nd[sp], rows);
-              OPtSet test = node_circle(F, 0, tc, rows);
+              OPtSet test = node_circle(F, 0, tc, test_count(rows));
               b->data = reexpansion(k, &orig, &equiv, &test);
             }
           }
@@ -758,6 +804,10 @@ static void make_butterfly(OFac *F, OLeaf *leaf, OLevel *Ls, OLevel *Lt, size_t 
     OPtSet tgt = node_points(F, 0, t);
     if (mode[i] == MODE_VALUES) {
       b->kind = BK_IDENTITY; b->data = identity(b->m);
+    } else if (orc_field_form && mode[i] == MODE_PROXY) {
+      b->kind = BK_FIELD;
+      OPtSet equiv = node_circle(F, 1, S, b->n), test = node_circle(F, 0, t, b->n);
+      b->data = field_block(k, &tgt, &equiv, &test);
     } else {
       b->kind = BK_KERNEL;
       OPtSet src = mode[i] == MODE_CHARGES ? node_points(F, 1, S) : node_circle(F, 1, S, b->n);

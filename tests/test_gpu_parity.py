@@ -247,6 +247,71 @@ def test_device_resident_apply_and_fp32(gpu_lib):
     assert rel(Y, F @ X) < 1e-3
 
 
+def _columns_of(lt, ft, bt):
+    """(leaf, factor-in-leaf, column) -> list of global block indices, and
+    for each factor the kinds of its rows' blocks."""
+    cols, rowkinds, fi, bi = {}, {}, 0, 0
+    for li, (isb, r0, c0, m, n, nf) in enumerate(lt):
+        if not isb:
+            continue
+        for f in range(nf):
+            nb = ft[fi][3]
+            for b in range(bi, bi + nb):
+                cols.setdefault((li, f, bt[b][1]), []).append(b)
+                rowkinds.setdefault((li, f, bt[b][0]), set()).add(int(bt[b][4]))
+            fi += 1
+            bi += nb
+    return cols, rowkinds
+
+
+def test_orthogonalize_keeps_operator_and_stabilizes_fp32(gpu_lib):
+    """The basis change leaves the FP64 operator unchanged (identity of the
+    products up to FP64 rounding), makes every rotated stacked column block
+    orthonormal (Q^H Q = I), and lowers the error of the FP32 apply."""
+    bf = gpu_lib
+    n = 16384
+    src, tgt, k = curve_problem(n)
+    ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
+    opts = bf.options(tol=1e-7, pass_through=True, rank_const=1.25, pinv_rcond=1e-9)
+    F = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, opts)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y64 = F @ x
+    dx32 = bf.DeviceArray.from_numpy(x.astype(np.complex64))
+    dy32 = bf.DeviceArray(8 * n)
+    F.prepare_mixed("fp32")
+    bf.mul_device(F, dx32, dy32, 1, "fp32")
+    e_before = rel(dy32.to_numpy(np.complex64, n).astype(complex), y64)
+    secs = F.orthogonalize()
+    assert secs >= 0
+    yo = F @ x
+    e_op = rel(yo, y64)
+    F.prepare_mixed("fp32")
+    bf.mul_device(F, dx32, dy32, 1, "fp32")
+    e_after = rel(dy32.to_numpy(np.complex64, n).astype(complex), y64)
+    rows = np.sort(rng.choice(n, 256, replace=False))
+    sp, tp = ts.perm, tt.perm
+    yd = orc.dense_rows(k, tgt[tp[rows]], src[sp], x)
+    e_exact = rel(dy32.to_numpy(np.complex64, n).astype(complex)[rows], yd)
+    print(f"ortho {secs:.3f}s  op change {e_op:.2e}  fp32 err before {e_before:.2e} after {e_after:.2e} "
+          f"vs exact rows {e_exact:.2e}")
+    assert e_op < 1e-11
+    assert e_after < e_before / 3 and e_after < 1e-6
+    assert e_exact < 2e-6
+    # orthonormal stacked columns wherever the column is fed by proxy rows
+    lt, ft, bt = F.plan()
+    cols, rowkinds = _columns_of(lt, ft, bt)
+    keys = [key for key in cols if key[1] >= 1 and rowkinds.get((key[0], key[1] - 1, key[2])) == {0}]
+    keys = [keys[i] for i in rng.choice(len(keys), min(200, len(keys)), replace=False)]
+    worst = 0.0
+    for key in keys:
+        Q = np.vstack([F.block(b, bt[b][2], bt[b][3]) for b in cols[key]])
+        if Q.shape[0] < Q.shape[1]:
+            continue
+        worst = max(worst, np.abs(Q.conj().T @ Q - np.eye(Q.shape[1])).max())
+    assert len(keys) > 0 and worst < 1e-12, worst
+
+
 def test_errors_are_reported_not_raised_on_device(gpu_lib):
     bf = gpu_lib
     src, tgt, k = curve_problem(2048)
