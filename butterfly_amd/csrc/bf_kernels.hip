@@ -727,6 +727,12 @@ __global__ __launch_bounds__(64) void k_applyN(const BfkSeg *__restrict__ segs, 
 /* ------------------------------------------- many right-hand sides: MFMA */
 
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
+#define MFMA_SEG_GROUP 4
+/* workgroups of one many-RHS stage: every (segment, column chunk) pair */
+static inline unsigned mfma_grid(int64_t numSegs, int chunks) {
+  const int64_t g = 8 * MFMA_SEG_GROUP;
+  return (unsigned)(((numSegs + g - 1) / g) * g * chunks);
+}
 
 /* 16x16x4 MFMA on one wave, real operands.  Operand lanes: A[l%16][l/16],
  * B[l/16][l%16].  Result register r of lane l holds D[row(l,r)][l%16]
@@ -771,12 +777,16 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
   typedef typename M::V V;
   typedef typename M::Acc Acc;
   /* workgroups go to the 8 XCDs round-robin by id: ids b with the same
-   * b % 8 share an XCD (and its L2), so all column chunks of a segment get
-   * ids 8 (group * numChunks + chunk) + xcd and read its blocks from one L2 */
+   * b % 8 share an XCD (and its L2).  A group of MFMA_SEG_GROUP adjacent
+   * segments (rows that read the same input blocks) with all their column
+   * chunks gets the ids 8 * slot + xcd, so both the blocks and the inputs
+   * they share are read from one L2. */
   const int64_t slot = blockIdx.x >> 3;
-  const int64_t grp = slot / numChunks;
-  const int chunk = (int)(slot - grp * numChunks);
-  const int64_t si = grp * 8 + (blockIdx.x & 7);
+  const int64_t per = (int64_t)MFMA_SEG_GROUP * numChunks;
+  const int64_t grp = slot / per;
+  const int64_t within = slot - grp * per;
+  const int64_t si = (grp * 8 + (blockIdx.x & 7)) * MFMA_SEG_GROUP + within / numChunks;
+  const int chunk = (int)(within - (within / numChunks) * numChunks);
   if (si >= numSegs) return;
   const BfkSeg Sg = segs[si];
   const int l = threadIdx.x;
@@ -1136,7 +1146,7 @@ int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const
     } else if (nrhs >= 16 && g_mfmaCT > 0) {
       const int ct = g_mfmaCT;
       const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
-      dim3 grid((unsigned)(((n + 7) / 8) * 8 * chunks));
+      dim3 grid(mfma_grid(n, chunks));
       if (ct == 1)
         hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 1, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
                            dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
@@ -1342,7 +1352,7 @@ int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, c
     } else if (nrhs >= 16 && g_mfmaCT > 0) {
       const int ct = g_mfmaCT;
       const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
-      dim3 grid((unsigned)(((n + 7) / 8) * 8 * chunks));
+      dim3 grid(mfma_grid(n, chunks));
       const float2 *H = (const float2 *)dArenaHi, *Lo = (const float2 *)dArenaLo;
       if (Lo) {
         if (g_comp32) launch_mfma_f32<true, true>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);

@@ -120,6 +120,10 @@ int bfApplyPlanBuild(BfMatHipButterfly *F) {
   }
   double facBytes = 0, vecBytes = 0;
   F->segs.n = F->terms.n = F->stageSeg.n = 0;
+  BF_VEC(int64_t) rowFirst, colCount, rowOrder;
+  memset(&rowFirst, 0, sizeof(rowFirst));
+  memset(&colCount, 0, sizeof(colCount));
+  memset(&rowOrder, 0, sizeof(rowOrder));
   for (int s = 0; s < Fmax - 1; ++s) {
     BF_VEC_PUSH(F->stageSeg, (int64_t)F->segs.n);
     for (size_t l = 0; l < F->leaves.n; ++l) {
@@ -131,8 +135,23 @@ int bfApplyPlanBuild(BfMatHipButterfly *F) {
       BfHipFactor *fa = &F->factors.a[fi];
       int64_t *ro = offs_at(F, fa->rowOffStart), *co = offs_at(F, fa->colOffStart);
       vecBytes += 16.0 * (double)(ro[fa->numBlockRows] + co[fa->numBlockCols]);
-      int64_t b = fa->firstBlock, bend = fa->firstBlock + fa->numBlocks;
-      while (b < bend) {
+      const int64_t bbeg = fa->firstBlock, bend = fa->firstBlock + fa->numBlocks;
+      /* rows in the order of their first input column block (counting
+       * sort, stable): the rows that read the same inputs become adjacent
+       * segments, which the many-RHS apply keeps on one XCD */
+      rowFirst.n = 0;
+      for (int64_t b = bbeg; b < bend; ++b)
+        if (b == bbeg || F->blocks.a[b].rowInd != F->blocks.a[b - 1].rowInd) BF_VEC_PUSH(rowFirst, b);
+      const int64_t nrow = (int64_t)rowFirst.n, ncol = fa->numBlockCols;
+      colCount.n = 0;
+      for (int64_t c = 0; c <= ncol; ++c) BF_VEC_PUSH(colCount, 0);
+      for (int64_t r = 0; r < nrow; ++r) colCount.a[F->blocks.a[rowFirst.a[r]].colInd + 1]++;
+      for (int64_t c = 0; c < ncol; ++c) colCount.a[c + 1] += colCount.a[c];
+      rowOrder.n = 0;
+      for (int64_t r = 0; r < nrow; ++r) BF_VEC_PUSH(rowOrder, 0);
+      for (int64_t r = 0; r < nrow; ++r) rowOrder.a[colCount.a[F->blocks.a[rowFirst.a[r]].colInd]++] = rowFirst.a[r];
+      for (int64_t ri = 0; ri < nrow; ++ri) {
+        int64_t b = rowOrder.a[ri];
         int32_t row = F->blocks.a[b].rowInd;
         BfkSeg sg;
         sg.yOff = region[fi] + ro[row];
@@ -161,6 +180,9 @@ int bfApplyPlanBuild(BfMatHipButterfly *F) {
       }
     }
   }
+  BF_VEC_FREE(rowFirst);
+  BF_VEC_FREE(colCount);
+  BF_VEC_FREE(rowOrder);
   /* final stage */
   BF_VEC_PUSH(F->stageSeg, (int64_t)F->segs.n);
   BF_VEC(Contrib) cons;
