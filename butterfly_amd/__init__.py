@@ -48,6 +48,12 @@ class TreeStruct(C.Structure):
     _fields_ = [("vtable", C.c_void_p), ("root", C.c_void_p), ("perm", C.POINTER(PermStruct))]
 
 
+class QuadtreeNodeStruct(C.Structure):
+    _fields_ = [("bboxMin", C.c_double * 2), ("bboxMax", C.c_double * 2), ("split", C.c_double * 2),
+                ("offset", C.c_size_t * 5), ("child", C.c_int32 * 4), ("parent", C.c_int32),
+                ("depth", C.c_int32), ("maxDepthBelow", C.c_int32), ("index", C.c_int32)]
+
+
 class QuadtreeStruct(C.Structure):
     _fields_ = [("super", TreeStruct), ("points", C.c_void_p), ("unitNormals", C.c_void_p),
                 ("nodes", C.c_void_p), ("numNodes", C.c_size_t), ("leafSize", C.c_size_t),
@@ -130,6 +136,26 @@ def lib():
         "bfhipFacBlockGeometry": (C.c_int, [vp, sz, dp]),
         "bfhipHankel0": (C.c_int, [sz, dp, dp]),
         "bfhipLstSqBatched": (C.c_int, [sz, sz, sz, dp, dp, dp]),
+        "bfhipSvdBatched": (C.c_int, [sz, C.POINTER(sz), C.POINTER(sz), dp, C.c_double, C.POINTER(sz), dp, dp]),
+        "bfFacStreamerNew": (vp, []),
+        "bfFacStreamerInit": (None, [vp, vp]),
+        "bfFacStreamerDelete": (None, [C.POINTER(vp)]),
+        "bfFacStreamerGetNumRows": (sz, [vp]),
+        "bfFacStreamerFeed": (C.c_bool, [vp, vp]),
+        "bfFacStreamerIsDone": (C.c_bool, [vp]),
+        "bfFacStreamerGetFac": (vp, [vp]),
+        "bfFacStreamerGetFacSpan": (vp, [vp]),
+        "bfFacStreamerGetCurrentColumnNode": (vp, [vp]),
+        "bfFacStreamerGetRowTreeReversePerm": (vp, [vp]),
+        "bfFacGetMat": (vp, [vp, C.c_int]),
+        "bfFacGetNumBytes": (sz, [vp]),
+        "bfFacSpanGetMat": (vp, [vp, C.c_int]),
+        "bfFacSpanDelete": (None, [C.POINTER(vp)]),
+        "bfFacSpanGetNumFacs": (sz, [vp]),
+        "bfFacGetNumW": (sz, [vp]),
+        "bfFacGetRowNodes": (sz, [vp, C.POINTER(C.c_int32), sz]),
+        "bfFacGetFactorShape": (C.c_int, [vp, sz, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
+        "bfFacGetFactorDense": (C.c_int, [vp, sz, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -388,6 +414,134 @@ def lstsq_batched(A, B):
         check()
         raise BfError(f"lstsq_batched failed ({rc})")
     return X
+
+
+def svd_batched(mats, tol):
+    """bfhipSvdBatched: truncated SVDs A = U W (W = S V^H) on the device;
+    returns [(U, W, k)]."""
+    ms = [np.asarray(a, np.complex128) for a in mats]
+    m = (C.c_size_t * len(ms))(*[a.shape[0] for a in ms])
+    n = (C.c_size_t * len(ms))(*[a.shape[1] for a in ms])
+    A = np.concatenate([a.ravel(order="F") for a in ms]) if ms else np.zeros(1, complex)
+    full = [min(a.shape) for a in ms]
+    U = np.zeros(max(1, sum(a.shape[0] * f for a, f in zip(ms, full))), np.complex128)
+    W = np.zeros(max(1, sum(f * a.shape[1] for a, f in zip(ms, full))), np.complex128)
+    k = (C.c_size_t * len(ms))()
+    if lib().bfhipSvdBatched(len(ms), m, n, _dp(A.view(np.float64)), tol, k, _dp(U.view(np.float64)),
+                             _dp(W.view(np.float64))):
+        check()
+        raise BfError("svd_batched failed")
+    out, uo, wo = [], 0, 0
+    for a, f, kk in zip(ms, full, k):
+        out.append((U[uo:uo + a.shape[0] * kk].reshape(kk, a.shape[0]).T,
+                    W[wo:wo + kk * a.shape[1]].reshape(a.shape[1], kk).T, kk))
+        uo += a.shape[0] * f
+        wo += f * a.shape[1]
+    return out
+
+
+class FacSpecStruct(C.Structure):
+    _fields_ = [("rowTree", C.c_void_p), ("colTree", C.c_void_p), ("rowTreeInitDepth", C.c_size_t),
+                ("tol", C.c_double), ("minNumRows", C.c_size_t), ("minNumCols", C.c_size_t),
+                ("compareRelativeErrors", C.c_bool), ("bailIfBottomedOut", C.c_bool)]
+
+
+class Fac:
+    """A finished streamed factorization (BfFac, owned by its streamer)."""
+
+    def __init__(self, ptr, owner):
+        self.ptr, self._owner = ptr, owner
+
+    @property
+    def num_w(self):
+        return lib().bfFacGetNumW(self.ptr)
+
+    def row_nodes(self):
+        n = lib().bfFacGetRowNodes(self.ptr, None, 0)
+        out = (C.c_int32 * max(n, 1))()
+        lib().bfFacGetRowNodes(self.ptr, out, n)
+        return np.array(out[:n], dtype=np.int64)
+
+    def factor_shape(self, which):
+        r, c, b = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        lib().bfFacGetFactorShape(self.ptr, which, C.byref(r), C.byref(c), C.byref(b))
+        return r.value, c.value, b.value
+
+    def factor_dense(self, which):
+        r, c, _ = self.factor_shape(which)
+        out = np.zeros(2 * r * c)
+        lib().bfFacGetFactorDense(self.ptr, which, _dp(out))
+        return out.view(np.complex128).reshape(r, c)
+
+    def num_bytes(self):
+        return lib().bfFacGetNumBytes(self.ptr)
+
+    def mat(self):
+        """bfFacGetMat: the device product Psi W0 ... W_L."""
+        return Mat(lib().bfFacGetMat(self.ptr, 1), keep=self._owner)
+
+
+class FacStreamer:
+    """bfFacStreamer over two Quadtrees (rows, columns)."""
+
+    def __init__(self, row_tree, col_tree, tol, init_depth=1, min_rows=1, min_cols=1, bail=False):
+        self.row_tree, self.col_tree = row_tree, col_tree
+        self.spec = FacSpecStruct(row_tree.ptr, col_tree.ptr, init_depth, tol, min_rows, min_cols, False, bail)
+        L = lib()
+        self.ptr = L.bfFacStreamerNew()
+        L.bfFacStreamerInit(self.ptr, C.byref(self.spec))
+        check()
+
+    @property
+    def num_rows(self):
+        return lib().bfFacStreamerGetNumRows(self.ptr)
+
+    @property
+    def done(self):
+        return bool(lib().bfFacStreamerIsDone(self.ptr))
+
+    def current_column_node(self):
+        """Index of the current column node in the column tree's node array."""
+        p = lib().bfFacStreamerGetCurrentColumnNode(self.ptr)
+        if not p:
+            return -1
+        base = QuadtreeStruct.from_address(self.col_tree.ptr).super.root
+        return (p - base) // C.sizeof(QuadtreeNodeStruct)
+
+    def feed(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        L = lib()
+        xm = L.bfMatDenseComplexNew()
+        L.bfMatDenseComplexInitViewFromPtr(xm, phi.shape[0], phi.shape[1], phi.ctypes.data)
+        ok = L.bfFacStreamerFeed(self.ptr, xm)
+        L.bfMatDelete(C.byref(C.c_void_p(xm)))
+        if not ok:
+            err = L.bfGetError()
+            if err:
+                raise BfError(f"feed failed (error {err})")
+        return bool(ok)
+
+    def fac(self):
+        p = lib().bfFacStreamerGetFac(self.ptr)
+        if not p:
+            check()
+            raise BfError("no factorization")
+        return Fac(p, self)
+
+    def span_mat(self):
+        L = lib()
+        sp = L.bfFacStreamerGetFacSpan(self.ptr)
+        m = L.bfFacSpanGetMat(sp, 1)
+        L.bfFacSpanDelete(C.byref(C.c_void_p(sp)))
+        return Mat(m, keep=self)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().bfFacStreamerDelete(C.byref(C.c_void_p(self.ptr)))
+                self.ptr = None
+        except Exception:
+            pass
 
 
 class DeviceArray:

@@ -28,6 +28,8 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
                double *dArena);
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
               const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena);
+int bfk_svd_jacobi(int64_t batch, const int64_t *dAOff, const int32_t *dM, const int32_t *dN, double *dA,
+                   const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig, int maxSweeps);
 size_t bfk_ortho_units(int rows, int n);
 int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
               const size_t *groupBytes, const BfkOrthoItem *dItems, double *dArena, const double *dRin,

@@ -390,6 +390,105 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
   qr_jacobi_solve<64>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, 0.0, X + (size_t)blockIdx.x * p * n);
 }
 
+/* ------------------------------------------------ batched one-sided SVD */
+
+/* Singular value decomposition of a batch of m x n complex matrices
+ * (column-major, m >= 1, 1 <= n <= 256), one 256-thread block each, in
+ * place: one-sided (Hestenes) Jacobi rotates pairs of columns until they
+ * are orthogonal, A V = W with orthogonal columns w_j = sigma_j u_j.  On
+ * return A holds W, V (n x n, column-major) the accumulated rotations and
+ * sig[j] = |w_j|.  Pairs follow the round-robin order, one group of g
+ * threads per pair (g a power of two <= 64, so a pair's reduction stays
+ * in one wave). */
+__global__ __launch_bounds__(256) void k_svd_jacobi(const int64_t *__restrict__ aOff, const int32_t *__restrict__ mm,
+                                                    const int32_t *__restrict__ nn, double2 *__restrict__ A,
+                                                    const int64_t *__restrict__ vOff, double2 *__restrict__ V,
+                                                    const int64_t *__restrict__ sOff, double *__restrict__ sig,
+                                                    int maxSweeps) {
+  const int b = blockIdx.x;
+  const int m = mm[b], n = nn[b];
+  double2 *a = A + aOff[b];
+  double2 *v = V + vOff[b];
+  const int tid = threadIdx.x;
+  const double eps = 2.220446049250313e-16;
+  for (int e = tid; e < n * n; e += 256) v[e] = make_double2((e % (n + 1)) == 0 ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  const int P = n + (n & 1);
+  const int npairs = P / 2;
+  int g = 64;
+  while (g > 1 && g * npairs > 256) g >>= 1;
+  const int slots = 256 / g;
+  const int slot = tid / g, sub = tid - slot * g;
+  __shared__ int anyRot;
+  for (int sweep = 0; sweep < maxSweeps && n > 1; ++sweep) {
+    if (tid == 0) anyRot = 0;
+    __syncthreads();
+    for (int step = 0; step < P - 1; ++step) {
+      for (int pb = 0; pb < npairs; pb += slots) {
+        const int pr = pb + slot;
+        int i = -1, j = -1;
+        if (pr < npairs) {
+          const int p0 = rr_row(pr, step, P), p1 = rr_row(P - 1 - pr, step, P);
+          i = min(p0, p1);
+          j = max(p0, p1);
+        }
+        const bool valid = pr < npairs && j < n;
+        double al = 0, be = 0, gx = 0, gy = 0;
+        if (valid) {
+          const double2 *ci = a + (int64_t)i * m, *cj = a + (int64_t)j * m;
+          for (int r = sub; r < m; r += g) {
+            double2 x = ci[r], y = cj[r];
+            al = fma(x.x, x.x, fma(x.y, x.y, al));
+            be = fma(y.x, y.x, fma(y.y, y.y, be));
+            gx = fma(x.x, y.x, fma(x.y, y.y, gx)); /* conj(x) y */
+            gy = fma(x.x, y.y, fma(-x.y, y.x, gy));
+          }
+        }
+        for (int o = 1; o < g; o <<= 1) {
+          al += __shfl_xor(al, o, 64);
+          be += __shfl_xor(be, o, 64);
+          gx += __shfl_xor(gx, o, 64);
+          gy += __shfl_xor(gy, o, 64);
+        }
+        const double ag = sqrt(gx * gx + gy * gy);
+        if (valid && ag > 0 && ag > eps * sqrt(al * be)) {
+          if (sub == 0) anyRot = 1;
+          /* rotate (x, y) <- (c x - s e y, s x + c e y), e = conj(g)/|g| */
+          const double ex = gx / ag, ey = -gy / ag;
+          const double zeta = (be - al) / (2 * ag);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+          const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+          double2 *ci = a + (int64_t)i * m, *cj = a + (int64_t)j * m;
+          for (int r = sub; r < m; r += g) {
+            double2 x = ci[r], y = cj[r];
+            double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x;
+            ci[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
+            cj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
+          }
+          double2 *vi = v + (int64_t)i * n, *vj = v + (int64_t)j * n;
+          for (int r = sub; r < n; r += g) {
+            double2 x = vi[r], y = vj[r];
+            double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x;
+            vi[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
+            vj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!anyRot) break;
+    __syncthreads();
+  }
+  double *s = sig + sOff[b];
+  for (int j = slot; j < n; j += slots) {
+    const double2 *cj = a + (int64_t)j * m;
+    double q = 0;
+    for (int r = sub; r < m; r += g) q = fma(cj[r].x, cj[r].x, fma(cj[r].y, cj[r].y, q));
+    for (int o = 1; o < g; o <<= 1) q += __shfl_xor(q, o, 64);
+    if (sub == 0) s[j] = sqrt(q);
+  }
+}
+
 /* ---------------------------------------------------- orthogonalization */
 
 /* LDS of one orthogonalization job, in 16-byte units: the stacked column
@@ -1091,6 +1190,22 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
     BFK_CHECK(hipFree(work));
   }
   return time_end(1);
+}
+
+/* batched SVD: tables and matrices already in device memory */
+int bfk_svd_jacobi(int64_t batch, const int64_t *dAOff, const int32_t *dM, const int32_t *dN, double *dA,
+                   const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig, int maxSweeps) {
+  if (bfk_init()) return -1;
+  if (batch <= 0) return 0;
+  const int64_t maxGrid = 1 << 30;
+  for (int64_t b0 = 0; b0 < batch; b0 += maxGrid) {
+    int64_t nb = batch - b0 < maxGrid ? batch - b0 : maxGrid;
+    hipLaunchKernelGGL(k_svd_jacobi, dim3((unsigned)nb), dim3(256), 0, g_stream, dAOff + b0, dM + b0, dN + b0,
+                       (double2 *)dA, dVOff + b0, (double2 *)dV, dSOff + b0, dSig, maxSweeps);
+    BFK_CHECK(hipGetLastError());
+  }
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
 }
 
 size_t bfk_ortho_units(int rows, int n) { return ortho_units(rows, n); }

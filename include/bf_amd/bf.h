@@ -69,6 +69,7 @@ typedef enum BfTypes {
   BF_TYPE_QUADTREE = 34,
   BF_TYPE_QUADTREE_NODE = 39,
   BF_TYPE_MAT_HIP_BUTTERFLY = 42,
+  BF_TYPE_MAT_HIP_PRODUCT = 43, /* a streamed factorization on the device */
   BF_TYPE_NONE = 1000
 } BfType;
 
@@ -396,6 +397,73 @@ int bfhipHankel0(BfSize n, BfReal const *x, BfReal *out);
  * host arrays.  Returns 0 on success. */
 int bfhipLstSqBatched(BfSize batch, BfSize p, BfSize n, BfComplex const *A, BfComplex const *B,
                       BfComplex *X);
+
+/* ------------------------------------------- streaming factorization */
+
+/* def.h:57-61 */
+typedef enum BfPolicies {
+  BF_POLICY_VIEW,
+  BF_POLICY_COPY,
+  BF_POLICY_STEAL,
+} BfPolicy;
+
+/* fac.h:6-44.  The trees are quadtrees of this library (BfQuadtree's
+ * BfTree super); rows of fed blocks follow the row tree's order and their
+ * columns the column tree's. */
+typedef struct BfFacSpec {
+  BfTree *rowTree;
+  BfTree *colTree;
+  BfSize rowTreeInitDepth;
+  BfReal tol;
+  BfSize minNumRows;
+  BfSize minNumCols;
+  bool compareRelativeErrors;
+  bool bailIfBottomedOut;
+} BfFacSpec;
+
+typedef struct BfFac BfFac;
+typedef struct BfFacSpan BfFacSpan;
+typedef struct BfFacStreamer BfFacStreamer;
+
+/* fac_streamer.h:9-20.  Feed takes the dense complex block of the current
+ * column leaf (all rows); the SVDs of each round run batched on the
+ * device.  Where the reference stops on a row leaf that does not compress
+ * (unless bailIfBottomedOut), this one passes the leaf's rows through. */
+BfFacStreamer *bfFacStreamerNew(void);
+void bfFacStreamerInit(BfFacStreamer *facStreamer, BfFacSpec const *facSpec);
+void bfFacStreamerDeinit(BfFacStreamer *facStreamer);
+void bfFacStreamerDealloc(BfFacStreamer **facStreamer);
+void bfFacStreamerDelete(BfFacStreamer **facStreamer);
+BfSize bfFacStreamerGetNumRows(BfFacStreamer const *facStreamer);
+bool bfFacStreamerFeed(BfFacStreamer *facStreamer, BfMat *Phi);
+bool bfFacStreamerIsDone(BfFacStreamer const *facStreamer);
+BfFac *bfFacStreamerGetFac(BfFacStreamer const *facStreamer);
+BfFacSpan *bfFacStreamerGetFacSpan(BfFacStreamer const *facStreamer);
+BfQuadtreeNode const *bfFacStreamerGetCurrentColumnNode(BfFacStreamer const *facStreamer);
+BfPerm const *bfFacStreamerGetRowTreeReversePerm(BfFacStreamer const *facStreamer);
+
+/* fac.h:52-56: the product Psi W0 ... W_L as a device matrix
+ * (BF_TYPE_MAT_HIP_PRODUCT, applied with bfMatMul; every policy copies) */
+void bfFacDeinitAndDealloc(BfFac **fac);
+BfMat *bfFacGetMat(BfFac const *fac, BfPolicy policy);
+BfSize bfFacGetNumBytes(BfFac const *fac);
+/* fac_span.h: the partial factorizations side by side; the span does not
+ * own them (they stay with the streamer) */
+BfMat *bfFacSpanGetMat(BfFacSpan const *facSpan, BfPolicy policy);
+void bfFacSpanDelete(BfFacSpan **facSpan);
+/* new: introspection for parity checks */
+BfSize bfFacSpanGetNumFacs(BfFacSpan const *facSpan);
+BfSize bfFacGetNumW(BfFac const *fac);
+BfSize bfFacGetRowNodes(BfFac const *fac, int32_t *out, BfSize cap);
+int bfFacGetFactorShape(BfFac const *fac, BfSize which, BfSize *rows, BfSize *cols, BfSize *numBlocks);
+int bfFacGetFactorDense(BfFac const *fac, BfSize which, BfComplex *out);
+
+/* new (no counterpart; linalg.c:1040-1120 restated): batched truncated
+ * SVDs on the device, A_b = U_b W_b with U_b m_b x k_b orthonormal and
+ * W_b = S V^H (k_b x n_b), k_b = #{sigma >= tol sigma_0}; A, U, W are
+ * packed column-major host arrays, U and W sized for full rank min(m,n). */
+int bfhipSvdBatched(BfSize batch, BfSize const *m, BfSize const *n, BfComplex const *A, BfReal tol, BfSize *k,
+                    BfComplex *U, BfComplex *W);
 
 #ifdef __cplusplus
 }

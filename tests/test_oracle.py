@@ -184,3 +184,63 @@ def test_oracle_pass_through_curve_accuracy():
     rows = np.arange(0, len(tgt), 61)
     yd = orc.dense_rows(k, tgt[rows], src, x)
     assert np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd) < 1e-6
+
+
+# ------------------------------------------------ streaming factorization
+
+def _stream(A, rows_nodes, cols_nodes, **kw):
+    from oracle.streamer import Streamer
+    S = Streamer(rows_nodes, cols_nodes, **kw)
+    while not S.done:
+        c = S.current_column_node()
+        assert S.ct.leaf(c)
+        S.feed(A[:, S.ct.start(c):S.ct.end(c)])
+    return S.fac()
+
+
+def test_streamer_truncation_rule_closed_form():
+    """linalg.c:26-35: keep sigma_k >= tol sigma_0; a diagonal matrix has
+    its diagonal as singular values, so k is a count."""
+    from oracle.streamer import tsvd
+    s = np.array([4.0, 1.0, 3e-3, 4e-3 * 1e-3, 1e-9])
+    A = np.diag(s).astype(complex)
+    for tol, k in [(1e-2, 2), (7.5e-4, 3), (1e-6, 4), (2.4e-10, 5), (1e-12, 5)]:
+        U, W, kk, truncated = tsvd(A, tol)
+        assert kk == k and truncated == (k < 5)
+        assert np.allclose(U @ W, np.diag(np.where(np.arange(5) < k, s, 0)), atol=1e-15)
+
+
+def test_streamer_exact_for_a_low_rank_matrix():
+    """A rank-6 matrix: every compressed row block keeps at most 6 terms and
+    the product reproduces the matrix to rounding."""
+    rng = np.random.default_rng(3)
+    n = 1024
+    src, tgt = rng.random((n, 2)), rng.random((n, 2)) + [2, 0]
+    sp, sn, _ = orc.tree(src, 32)
+    tp, tn, _ = orc.tree(tgt, 16)
+    A = (rng.standard_normal((n, 6)) + 1j * rng.standard_normal((n, 6))) @ \
+        (rng.standard_normal((6, n)) + 1j * rng.standard_normal((6, n)))
+    F = _stream(A, tn, sn, tol=1e-10, init_depth=0, min_rows=20, min_cols=20)
+    assert F.Psi.shape[1] == 6
+    assert np.linalg.norm(F.dense() - A) / np.linalg.norm(A) < 1e-13
+
+
+def test_streamer_reproduces_a_kernel_matrix_to_its_tolerance():
+    """Helmholtz kernel between separated boxes (k = 25): the streamed
+    product is within a small multiple of tol of the matrix and stores far
+    fewer entries; every SVD-made Psi block has orthonormal columns."""
+    rng = np.random.default_rng(4)
+    n, tol = 1024, 1e-8
+    src, tgt = rng.random((n, 2)), rng.random((n, 2)) + [2, 0]
+    sp, sn, _ = orc.tree(src, 64)
+    tp, tn, _ = orc.tree(tgt, 16)
+    A = orc.kernel_matrix(25.0, tgt[tp], src[sp])
+    F = _stream(A, tn, sn, tol=tol, init_depth=1, min_rows=20, min_cols=20)
+    assert np.linalg.norm(F.dense() - A) / np.linalg.norm(A) < 20 * tol
+    assert F.entries() < A.size / 5
+    Psi = F.Psi.toarray()
+    for (a, e) in F.colRanges:
+        blk = Psi[:, a:e]
+        G = blk.conj().T @ blk
+        if not np.allclose(np.abs(blk[np.abs(blk) > 0]), 1.0):  # identity blocks are trivially fine
+            assert np.abs(G - np.eye(e - a)).max() < 1e-12
