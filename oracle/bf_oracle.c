@@ -35,7 +35,10 @@
  * (tests/test_oracle.py).  The order of points inside a quadtree child
  * follows the reference's sift and is pinned by nothing else; the
  * pass-through planner, the relative pseudo-inverse cut and the impedance
- * test have no reference counterpart and are pinned end to end only.
+ * test have no reference counterpart and are pinned end to end only.  Two
+ * experiment switches have no counterpart either: the field form (the same
+ * operator re-associated, pinned by agreeing with the charge form to
+ * rounding) and oversampled test circles (pinned end to end by dense rows).
  */
 #define _DEFAULT_SOURCE 1
 #include <complex.h>

@@ -244,3 +244,43 @@ def test_streamer_reproduces_a_kernel_matrix_to_its_tolerance():
         G = blk.conj().T @ blk
         if not np.allclose(np.abs(blk[np.abs(blk) > 0]), 1.0):  # identity blocks are trivially fine
             assert np.abs(G - np.eye(e - a)).max() < 1e-12
+
+
+def test_field_form_is_the_same_operator():
+    """Field-sample intermediates re-associate the charge-form product
+    E Z_L^+ T_L ... T_1 Z_0^+ O_0 as (E Z_L^+)(T_L Z_{L-1}^+)...O_0: the two
+    oracle factorizations must apply identically up to rounding."""
+    from conftest import curve_problem
+    src, tgt, k = curve_problem(2048)
+    a = orc.Fac(k, src, tgt, 64, 1e-8, 128 * 128, True, rank_const=1.25, pinv_rcond=1e-11)
+    b = orc.Fac(k, src, tgt, 64, 1e-8, 128 * 128, True, rank_const=1.25, pinv_rcond=1e-11, field_form=True)
+    orc.seed(5)
+    x = orc.complex_randn(len(src))
+    ya, yb = a.apply(x)[:, 0], b.apply(x)[:, 0]
+    assert np.linalg.norm(ya - yb) / np.linalg.norm(ya) < 1e-12
+    kinds_b = b.plan()[2][:, 4]
+    assert (kinds_b == 3).any() and not (kinds_b == 0).any()  # field blocks, no charge re-expansions
+
+
+def test_oversampled_test_circles_are_at_least_as_accurate():
+    """Least squares on 2p test points instead of p (same ranks): the
+    operator stays within the factorization's accuracy of the exact rows and
+    is no worse than the square (reference) form."""
+    from conftest import curve_problem
+    src, tgt, k = curve_problem(2048)
+    orc.seed(6)
+    x = orc.complex_randn(len(src))
+    rows = np.arange(0, len(tgt), 17)
+    errs = []
+    for os_ in (1.0, 2.0):
+        orc.lib().orc_set_test_oversample(os_)
+        try:
+            F = orc.Fac(k, src, tgt, 64, 1e-7, 128 * 128, True, rank_const=1.25, pinv_rcond=1e-9)
+        finally:
+            orc.lib().orc_set_test_oversample(1.0)
+        sp, tp = F.perms()
+        y = np.empty(len(tgt), complex)
+        y[tp] = F.apply(x[sp])[:, 0]
+        yd = orc.dense_rows(k, tgt[rows], src, x)
+        errs.append(np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd))
+    assert errs[0] < 1e-5 and errs[1] <= errs[0] * 1.5
