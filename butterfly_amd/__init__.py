@@ -136,6 +136,8 @@ def lib():
         "bfhipFacBlockGeometry": (C.c_int, [vp, sz, dp]),
         "bfhipHankel0": (C.c_int, [sz, dp, dp]),
         "bfhipLstSqBatched": (C.c_int, [sz, sz, sz, dp, dp, dp]),
+        "bfMatSave": (None, [vp, C.c_char_p]),
+        "bfhipFacLoadFile": (vp, [C.c_char_p]),
         "bfhipSvdBatched": (C.c_int, [sz, C.POINTER(sz), C.POINTER(sz), dp, C.c_double, C.POINTER(sz), dp, dp]),
         "bfFacStreamerNew": (vp, []),
         "bfFacStreamerInit": (None, [vp, vp]),
@@ -156,6 +158,10 @@ def lib():
         "bfFacGetRowNodes": (sz, [vp, C.POINTER(C.c_int32), sz]),
         "bfFacGetFactorShape": (C.c_int, [vp, sz, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
         "bfFacGetFactorDense": (C.c_int, [vp, sz, dp]),
+        "bfhipHelm2BieNew": (vp, [vp, vp, vp, dp, vp, sz, C.c_double]),
+        "bfhipBieMulDevice": (C.c_int, [vp, vp, vp, sz, C.c_int]),
+        "bfSolveGMRES": (vp, [vp, vp, vp, C.c_double, sz, C.POINTER(sz), vp]),
+        "bfhipSolveGmresMixed": (C.c_int, [vp, dp, dp, C.c_double, sz, sz, C.POINTER(sz), C.POINTER(sz), dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -357,6 +363,20 @@ class Factorization(Mat):
         if release_fp64 and lib().bfhipFacReleaseFp64(self.ptr):
             check()
             raise BfError("release_fp64 failed")
+
+    def save(self, path):
+        """bfMatSave: the factorization in the bfMatDump framing."""
+        lib().bfMatSave(self.ptr, str(path).encode())
+        check()
+
+    @staticmethod
+    def load(path):
+        """bfhipFacLoadFile: a saved factorization back in HBM."""
+        p = lib().bfhipFacLoadFile(str(path).encode())
+        if not p:
+            check()
+            raise BfError("load failed")
+        return Factorization(p)
 
     def orthogonalize(self):
         """bfhipFacOrthogonalize: rotate every intermediate basis so the map
@@ -601,3 +621,64 @@ def kernel_time_sums():
 
 def reset_kernel_times():
     lib().bfhipResetKernelTimes()
+
+
+class BoundaryOperator(Mat):
+    """bfhipHelm2BieNew: the second-kind operator alpha I + (S + C) W of a
+    closed curve on the device (helm2_bie.c:109-121): S the device
+    factorization `fac` (sources = targets = the curve), C the order-`kr_order`
+    Kapur-Rokhlin corrections along the curve's point order, W the weights
+    (original point order).  Vectors are in tree order; `tree.perm` maps."""
+
+    def __init__(self, fac, helm, tree, weights, kr_order=6, alpha=0.5):
+        self.weights = np.ascontiguousarray(weights, dtype=np.float64)
+        q = QuadtreeStruct.from_address(tree.ptr)
+        perm = C.addressof(q.super.perm.contents)
+        p = lib().bfhipHelm2BieNew(fac.ptr, C.byref(helm.s), tree._pts, _dp(self.weights), perm,
+                                   kr_order, float(alpha))
+        super().__init__(p, keep=(fac, tree))
+
+    def mul_device(self, dx, dy, nrhs, precision="fp64"):
+        """bfhipBieMulDevice: Y = A X on resident device buffers."""
+        rc = lib().bfhipBieMulDevice(self.ptr, dx.ptr, dy.ptr, nrhs, PRECISION[precision])
+        if rc:
+            check()
+            raise BfError(f"boundary apply failed ({rc})")
+
+    def solve(self, b, tol, max_iter, x0=None):
+        """bfSolveGMRES (linalg.c:47-317): unrestarted GMRES per column of b
+        (tree order); returns (x, iterations)."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        squeeze = b.ndim == 1
+        if squeeze:
+            b = b[:, None]
+        L = lib()
+        bm = L.bfMatDenseComplexNew()
+        L.bfMatDenseComplexInitViewFromPtr(bm, b.shape[0], b.shape[1], b.ctypes.data)
+        xm = None
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.complex128).reshape(b.shape)
+            xm = L.bfMatDenseComplexNew()
+            L.bfMatDenseComplexInitViewFromPtr(xm, x0.shape[0], x0.shape[1], x0.ctypes.data)
+        it = C.c_size_t(0)
+        x = L.bfSolveGMRES(self.ptr, bm, xm, float(tol), int(max_iter), C.byref(it), None)
+        L.bfMatDelete(C.byref(C.c_void_p(bm)))
+        if xm:
+            L.bfMatDelete(C.byref(C.c_void_p(xm)))
+        check()
+        out = Mat(x).to_numpy()
+        return (out[:, 0] if squeeze else out), it.value
+
+    def solve_mixed(self, b, tol, restart=50, max_iter=1000):
+        """bfhipSolveGmresMixed: FP64 residuals around GMRES(restart) with the
+        FP32 apply (prepare_mixed must have run on the factorization).
+        Returns (x, {"converged", "iterations", "refinements", "rel_res"})."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros_like(b)
+        it, nref, rel = C.c_size_t(0), C.c_size_t(0), C.c_double(0)
+        rc = lib().bfhipSolveGmresMixed(self.ptr, _dp(b.view(np.float64)), _dp(x.view(np.float64)), float(tol),
+                                        int(restart), int(max_iter), C.byref(it), C.byref(nref), C.byref(rel))
+        if rc < 0:
+            check()
+            raise BfError("mixed GMRES failed")
+        return x, {"converged": rc == 0, "iterations": it.value, "refinements": nref.value, "rel_res": rel.value}

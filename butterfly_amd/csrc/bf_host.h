@@ -28,6 +28,13 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
                double *dArena);
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
               const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena);
+int bfk_pair_green(int64_t n, const double *dA, const double *dB, double k, double *dOut);
+int bfk_scale_rows(int64_t n, int nrhs, const double *dW, const void *dX, void *dU, int fp32);
+int bfk_bie_post(int64_t n, int nrhs, int ncorr, double alpha, const int32_t *dCol, const double *dC,
+                 const void *dX, const void *dU, void *dY, int fp32);
+int bfk_dotc_multi(int64_t n, int nv, const double *dV, int64_t ldv, const double *dW, double *hOut);
+int bfk_axpy_multi(int64_t n, int nv, const double *dV, int64_t ldv, const double *hH, double s, double *dW);
+int bfk_convert(int64_t n, const void *src, void *dst, int to32);
 int bfk_svd_jacobi(int64_t batch, const int64_t *dAOff, const int32_t *dM, const int32_t *dN, double *dA,
                    const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig, int maxSweeps);
 size_t bfk_ortho_units(int rows, int n);
@@ -138,6 +145,8 @@ typedef struct BfMatHipButterfly {
 int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt);
 /* apply plan (fac.c) */
 int bfApplyPlanBuild(BfMatHipButterfly *F);
+/* apply plan and uploads for a plan + arena already in place (fac.c) */
+int bfMatHipButterflyFinish(BfMatHipButterfly *F);
 
 /* matrix vtable (mat.c) */
 struct BfMatVtable {

@@ -390,6 +390,105 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
   qr_jacobi_solve<64>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, 0.0, X + (size_t)blockIdx.x * p * n);
 }
 
+/* ------------------------------------ boundary integral operator, GMRES */
+
+/* G(a_i <- b_i) for a list of point pairs (the quadrature corrections) */
+__global__ void k_pair_green(int64_t n, const double2 *__restrict__ a, const double2 *__restrict__ b, double k,
+                             double2 *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = green(k, a[i], b[i]);
+}
+
+/* u = w .* x, row i scaled by w[i], nrhs columns (row-major) */
+template <typename V>
+__global__ void k_scale_rows(int64_t n, int nrhs, const double *__restrict__ w, const V *__restrict__ x,
+                             V *__restrict__ u) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nrhs) return;
+  const double s = w[e / nrhs];
+  V v = x[e];
+  v.x *= s;
+  v.y *= s;
+  u[e] = v;
+}
+
+/* y = alpha x + y + sum_q c[i][q] u[col[i][q]] (the identity term and the
+ * near-singular corrections of one row, after the butterfly wrote y) */
+template <typename V>
+__global__ void k_bie_post(int64_t n, int nrhs, int ncorr, double alpha, const int32_t *__restrict__ col,
+                           const double2 *__restrict__ c, const V *__restrict__ x, const V *__restrict__ u,
+                           V *__restrict__ y) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nrhs) return;
+  const int64_t i = e / nrhs;
+  const int r = (int)(e - i * nrhs);
+  double ax = alpha * (double)x[e].x + (double)y[e].x, ay = alpha * (double)x[e].y + (double)y[e].y;
+  for (int q = 0; q < ncorr; ++q) {
+    const double2 cq = c[i * ncorr + q];
+    const V uq = u[(int64_t)col[i * ncorr + q] * nrhs + r];
+    ax = fma(cq.x, (double)uq.x, fma(-cq.y, (double)uq.y, ax));
+    ay = fma(cq.x, (double)uq.y, fma(cq.y, (double)uq.x, ay));
+  }
+  V out;
+  out.x = ax;
+  out.y = ay;
+  y[e] = out;
+}
+
+/* Partial sums of h_j = V_j^H w for j < nv: block b writes its share of
+ * every h_j to part[b * nv + j] (summed on the host, in block order). */
+__global__ __launch_bounds__(256) void k_dotc_multi(int64_t n, int nv, const double2 *__restrict__ V, int64_t ldv,
+                                                    const double2 *__restrict__ w, double2 *__restrict__ part) {
+  __shared__ double2 red[256];
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t a = (int64_t)blockIdx.x * chunk, e = a + chunk < n ? a + chunk : n;
+  for (int j = 0; j < nv; ++j) {
+    double sx = 0, sy = 0;
+    for (int64_t i = a + threadIdx.x; i < e; i += 256) {
+      const double2 v = V[j * ldv + i], x = w[i];
+      sx = fma(v.x, x.x, fma(v.y, x.y, sx)); /* conj(v) x */
+      sy = fma(v.x, x.y, fma(-v.y, x.x, sy));
+    }
+    red[threadIdx.x] = make_double2(sx, sy);
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        red[threadIdx.x].x += red[threadIdx.x + o].x;
+        red[threadIdx.x].y += red[threadIdx.x + o].y;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nv + j] = red[0];
+    __syncthreads();
+  }
+}
+
+/* w = s w - V h (nv vectors of V, leading dimension ldv) */
+__global__ void k_axpy_multi(int64_t n, int nv, const double2 *__restrict__ V, int64_t ldv,
+                             const double2 *__restrict__ h, double s, double2 *__restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 acc = w[i];
+  acc.x *= s;
+  acc.y *= s;
+  for (int j = 0; j < nv; ++j) {
+    const double2 v = V[j * ldv + i], c = h[j];
+    acc.x -= c.x * v.x - c.y * v.y;
+    acc.y -= c.x * v.y + c.y * v.x;
+  }
+  w[i] = acc;
+}
+
+__global__ void k_f64_to_f32(int64_t n, const double2 *__restrict__ a, float2 *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+__global__ void k_f32_to_f64(int64_t n, const float2 *__restrict__ a, double2 *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = make_double2((double)a[i].x, (double)a[i].y);
+}
+
 /* ------------------------------------------------ batched one-sided SVD */
 
 /* Singular value decomposition of a batch of m x n complex matrices
@@ -1190,6 +1289,114 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
     BFK_CHECK(hipFree(work));
   }
   return time_end(1);
+}
+
+static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+int bfk_pair_green(int64_t n, const double *dA, const double *dB, double k, double *dOut) {
+  if (bfk_init()) return -1;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_pair_green, dim3(blocks_for(n, 256)), dim3(256), 0, g_stream, n, (const double2 *)dA,
+                     (const double2 *)dB, k, (double2 *)dOut);
+  BFK_CHECK(hipGetLastError());
+  return 0;
+}
+
+int bfk_scale_rows(int64_t n, int nrhs, const double *dW, const void *dX, void *dU, int fp32) {
+  if (bfk_init()) return -1;
+  const int64_t e = n * nrhs;
+  if (e <= 0) return 0;
+  if (fp32)
+    hipLaunchKernelGGL(k_scale_rows<float2>, dim3(blocks_for(e, 256)), dim3(256), 0, g_stream, n, nrhs, dW,
+                       (const float2 *)dX, (float2 *)dU);
+  else
+    hipLaunchKernelGGL(k_scale_rows<double2>, dim3(blocks_for(e, 256)), dim3(256), 0, g_stream, n, nrhs, dW,
+                       (const double2 *)dX, (double2 *)dU);
+  BFK_CHECK(hipGetLastError());
+  return 0;
+}
+
+int bfk_bie_post(int64_t n, int nrhs, int ncorr, double alpha, const int32_t *dCol, const double *dC,
+                 const void *dX, const void *dU, void *dY, int fp32) {
+  if (bfk_init()) return -1;
+  const int64_t e = n * nrhs;
+  if (e <= 0) return 0;
+  if (fp32)
+    hipLaunchKernelGGL(k_bie_post<float2>, dim3(blocks_for(e, 256)), dim3(256), 0, g_stream, n, nrhs, ncorr,
+                       alpha, dCol, (const double2 *)dC, (const float2 *)dX, (const float2 *)dU, (float2 *)dY);
+  else
+    hipLaunchKernelGGL(k_bie_post<double2>, dim3(blocks_for(e, 256)), dim3(256), 0, g_stream, n, nrhs, ncorr,
+                       alpha, dCol, (const double2 *)dC, (const double2 *)dX, (const double2 *)dU, (double2 *)dY);
+  BFK_CHECK(hipGetLastError());
+  return 0;
+}
+
+/* h (host, nv complex) = V^H w; partial sums per block summed on the host
+ * in block order, so the result does not depend on scheduling */
+static double2 *g_part = nullptr, *g_h = nullptr;
+static size_t g_partCap = 0, g_hCap = 0;
+#define DOT_BLOCKS 512
+
+int bfk_dotc_multi(int64_t n, int nv, const double *dV, int64_t ldv, const double *dW, double *hOut) {
+  if (bfk_init()) return -1;
+  if (nv <= 0) return 0;
+  const size_t need = (size_t)DOT_BLOCKS * nv;
+  if (need > g_partCap) {
+    if (g_part) (void)hipFree(g_part);
+    g_part = nullptr;
+    BFK_CHECK(hipMalloc(&g_part, need * sizeof(double2)));
+    g_partCap = need;
+  }
+  hipLaunchKernelGGL(k_dotc_multi, dim3(DOT_BLOCKS), dim3(256), 0, g_stream, n, nv, (const double2 *)dV, ldv,
+                     (const double2 *)dW, g_part);
+  BFK_CHECK(hipGetLastError());
+  double2 *hp = (double2 *)malloc(need * sizeof(double2));
+  if (!hp) return -2;
+  if (hipMemcpyAsync(hp, g_part, need * sizeof(double2), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+      hipStreamSynchronize(g_stream) != hipSuccess) {
+    free(hp);
+    return -1;
+  }
+  for (int j = 0; j < nv; ++j) {
+    double sx = 0, sy = 0;
+    for (int b = 0; b < DOT_BLOCKS; ++b) {
+      sx += hp[(size_t)b * nv + j].x;
+      sy += hp[(size_t)b * nv + j].y;
+    }
+    hOut[2 * j] = sx;
+    hOut[2 * j + 1] = sy;
+  }
+  free(hp);
+  return 0;
+}
+
+/* w = s w - V h, h given on the host */
+int bfk_axpy_multi(int64_t n, int nv, const double *dV, int64_t ldv, const double *hH, double s, double *dW) {
+  if (bfk_init()) return -1;
+  if ((size_t)(nv > 0 ? nv : 1) > g_hCap) {
+    if (g_h) (void)hipFree(g_h);
+    g_h = nullptr;
+    BFK_CHECK(hipMalloc(&g_h, (size_t)(nv > 0 ? nv : 1) * sizeof(double2)));
+    g_hCap = (size_t)(nv > 0 ? nv : 1);
+  }
+  if (nv > 0) BFK_CHECK(hipMemcpyAsync(g_h, hH, (size_t)nv * sizeof(double2), hipMemcpyHostToDevice, g_stream));
+  hipLaunchKernelGGL(k_axpy_multi, dim3(blocks_for(n, 256)), dim3(256), 0, g_stream, n, nv, (const double2 *)dV,
+                     ldv, g_h, s, (double2 *)dW);
+  BFK_CHECK(hipGetLastError());
+  return 0;
+}
+
+int bfk_convert(int64_t n, const void *src, void *dst, int to32) {
+  if (bfk_init()) return -1;
+  if (n <= 0) return 0;
+  if (to32)
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(n, 256)), dim3(256), 0, g_stream, n, (const double2 *)src,
+                       (float2 *)dst);
+  else
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(blocks_for(n, 256)), dim3(256), 0, g_stream, n, (const float2 *)src,
+                       (double2 *)dst);
+  BFK_CHECK(hipGetLastError());
+  return 0;
 }
 
 /* batched SVD: tables and matrices already in device memory */

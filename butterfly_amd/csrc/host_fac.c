@@ -296,6 +296,34 @@ static int upload(void **dst, const void *src, size_t bytes) {
   return 0;
 }
 
+/* the apply of a factorization whose plan and arena are in place (a loaded
+ * one): stage segments and terms, uploaded */
+int bfMatHipButterflyFinish(BfMatHipButterfly *F) {
+  double t0 = bfWallTime();
+  if (bfApplyPlanBuild(F)) return -1;
+  int rc = upload((void **)&F->dSegs, F->segs.a, F->segs.n * sizeof(BfkSeg));
+  if (!rc) rc = upload((void **)&F->dTerms, F->terms.a, F->terms.n * sizeof(BfkTerm));
+  int64_t nL = 0, nK = 0, nI = 0, nD = 0, nB = 0;
+  for (size_t b = 0; b < F->blocks.n; ++b) {
+    nL += F->blocks.a[b].kind == BF_BLOCK_LSTSQ;
+    nK += F->blocks.a[b].kind == BF_BLOCK_KERNEL;
+    nI += F->blocks.a[b].kind == BF_BLOCK_IDENTITY;
+  }
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    nB += F->leaves.a[l].isButterfly;
+    nD += !F->leaves.a[l].isButterfly;
+  }
+  F->stats[3] = (double)F->arenaEntries;
+  F->stats[4] = (double)nL;
+  F->stats[5] = (double)nK;
+  F->stats[6] = (double)nI;
+  F->stats[7] = (double)nD;
+  F->stats[8] = (double)nB;
+  F->stats[9] = (double)F->numStages;
+  F->stats[15] = bfWallTime() - t0;
+  return rc;
+}
+
 static double *tree_coords(BfQuadtree const *t) {
   BfSize n = t->points->size;
   double *xy = malloc(2 * n * sizeof(double));

@@ -28,6 +28,7 @@
 #include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -61,6 +62,7 @@ enum BfError {
  * factorization is appended as a new type) */
 typedef enum BfTypes {
   BF_TYPE_MAT = 0,
+  BF_TYPE_MAT_IDENTITY = 8,
   BF_TYPE_MAT_PRODUCT = 10,
   BF_TYPE_MAT_BLOCK_COO = 15,
   BF_TYPE_MAT_BLOCK_DENSE = 16,
@@ -70,6 +72,7 @@ typedef enum BfTypes {
   BF_TYPE_QUADTREE_NODE = 39,
   BF_TYPE_MAT_HIP_BUTTERFLY = 42,
   BF_TYPE_MAT_HIP_PRODUCT = 43, /* a streamed factorization on the device */
+  BF_TYPE_MAT_HIP_BIE = 44,     /* a boundary integral operator on the device */
   BF_TYPE_NONE = 1000
 } BfType;
 
@@ -457,6 +460,49 @@ BfSize bfFacGetNumW(BfFac const *fac);
 BfSize bfFacGetRowNodes(BfFac const *fac, int32_t *out, BfSize cap);
 int bfFacGetFactorShape(BfFac const *fac, BfSize which, BfSize *rows, BfSize *cols, BfSize *numBlocks);
 int bfFacGetFactorDense(BfFac const *fac, BfSize which, BfComplex *out);
+
+/* ------------------------------------ boundary integral equation, GMRES */
+
+/* new (helm2_bie.c:95-121 assembled on the device): A = alpha I +
+ * (S + C) W with S the device factorization `fac` of the kernel on the
+ * boundary points (square, sources = targets; not owned, keep it alive),
+ * W the quadrature weights (original point order), C the Kapur-Rokhlin
+ * corrections of order krOrder (0, 2 or 6; quadrature.c:12-200, applied
+ * before the weights as helm2_bie.c:98-103 does) along the boundary order
+ * of `points`; `perm` is the tree permutation of fac's quadtree.  Vectors
+ * are in tree order. */
+BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points, BfReal const *weights,
+                        BfPerm const *perm, BfSize krOrder, BfReal alpha);
+/* new: y = A x on device vectors (n x nrhs, FP64 or FP32; FP32 needs
+ * bfhipFacPrepareMixed on the factorization) */
+int bfhipBieMulDevice(BfMat const *op, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec);
+/* linalg.c:47-317: GMRES without restarts, maxNumIter Krylov vectors, one
+ * right-hand side at a time, stopping at |r| <= tol |b|; A a device
+ * boundary operator, B (and X0) host dense complex; the basis lives in HBM
+ * (FP64, classical Gram-Schmidt twice).  M (left preconditioning) must be
+ * NULL here. */
+BfMat *bfSolveGMRES(BfMat const *A, BfMat const *B, BfMat *X0, BfReal tol, BfSize maxNumIter, BfSize *numIter,
+                    BfMat const *M);
+/* new: mixed precision GMRES: FP64 residuals and refinement, inner
+ * GMRES(restart) with the FP32 apply; host b, x (tree order); returns 0
+ * when |b - A x| <= tol |b| (FP64 apply), 1 if maxIter ran out */
+int bfhipSolveGmresMixed(BfMat const *A, BfComplex const *b, BfComplex *x, BfReal tol, BfSize restart,
+                         BfSize maxIter, BfSize *numIter, BfSize *numRefine, BfReal *relRes);
+
+/* mat.c:63-73: write a device factorization in the bfMatDump framing (the
+ * type, then its payload: the leaves as BF_TYPE_MAT_PRODUCT of
+ * BF_TYPE_MAT_BLOCK_DIAG / BF_TYPE_MAT_BLOCK_COO factors of dense complex
+ * and identity blocks, mat_product.c:123-140, mat_block_diag.c:241-265,
+ * mat_block_coo.c:260-290); the layout is in host_io.c.  The reference
+ * defines no Dump for dense complex blocks; here they are their size and
+ * row-major entries. */
+void bfMatDump(BfMat const *mat, FILE *fp);
+void bfMatSave(BfMat const *mat, char const *path);
+/* new (no counterpart): read a factorization written by bfMatDump /
+ * bfMatSave back into HBM (no factorization kernels run); for callers that
+ * reuse a factorization across runs */
+BfMat *bfhipFacLoad(FILE *fp);
+BfMat *bfhipFacLoadFile(char const *path);
 
 /* new (no counterpart; linalg.c:1040-1120 restated): batched truncated
  * SVDs on the device, A_b = U_b W_b with U_b m_b x k_b orthonormal and

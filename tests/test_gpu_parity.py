@@ -318,3 +318,24 @@ def test_errors_are_reported_not_raised_on_device(gpu_lib):
     F, ts, tt = _factor_both(src, tgt, k, leaf=64, tol=1e-8, pass_through=True)
     with pytest.raises(bf.BfError):
         F @ np.zeros(len(src) + 1, complex)
+
+
+def test_save_and_load_round_trip(gpu_lib, tmp_path):
+    """bfMatSave / bfhipFacLoadFile: the loaded factorization has the same
+    plan and applies bit-identically, before and after orthogonalization."""
+    bf = gpu_lib
+    src, tgt, k = curve_problem(4096)
+    F, ts, tt = _factor_both(src, tgt, k, leaf=64, tol=1e-8, pass_through=True)
+    path = tmp_path / "fac.bf"
+    F.save(path)
+    G = bf.Factorization.load(path)
+    for a, b in zip(F.plan(), G.plan()):
+        assert np.array_equal(a, b)
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((len(src), 3)) + 1j * rng.standard_normal((len(src), 3))
+    assert np.array_equal(F @ x, G @ x)
+    F.orthogonalize()
+    G.orthogonalize()
+    assert np.array_equal(F @ x, G @ x)
+    with pytest.raises(bf.BfError):
+        bf.Factorization.load(tmp_path / "missing.bf")

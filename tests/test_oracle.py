@@ -284,3 +284,30 @@ def test_oversampled_test_circles_are_at_least_as_accurate():
         yd = orc.dense_rows(k, tgt[rows], src, x)
         errs.append(np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd))
     assert errs[0] < 1e-5 and errs[1] <= errs[0] * 1.5
+
+
+def test_kapur_rokhlin_single_layer_on_a_circle():
+    """Fourier modes are eigenfunctions of the single layer on a circle:
+    S e^{in t} = (i pi R/2) J_n(kR) H_n(kR) e^{in t}.  The trapezoid rule
+    with the zero diagonal converges at first order; Kapur-Rokhlin order 2
+    and order 6 must show their orders when N doubles (a wrong or missing
+    gamma drops the rate to first order)."""
+    from scipy.special import hankel1, jv
+    from oracle.quadrature import kr_correct
+    R, k, n_mode = 1.3, 7.0, 3
+    lam = 1j * np.pi * R / 2 * jv(n_mode, k * R) * hankel1(n_mode, k * R)
+    errs = {}
+    for N in (256, 512):
+        th = 2 * np.pi * np.arange(N) / N
+        pts = np.stack([R * np.cos(th), R * np.sin(th)], 1)
+        K = orc.kernel_matrix(k, pts, pts)
+        sigma = np.exp(1j * n_mode * th)
+        w = np.full(N, 2 * np.pi * R / N)
+        for order in (0, 2, 6):
+            y = kr_correct(K, order) @ (w * sigma)
+            errs[order, N] = np.linalg.norm(y - lam * sigma) / np.linalg.norm(lam * sigma)
+    rate = {o: np.log2(errs[o, 256] / errs[o, 512]) for o in (0, 2, 6)}
+    assert 0.5 < rate[0] < 1.5, rate
+    assert 2.0 < rate[2] < 3.5, rate
+    assert rate[6] > 5.5, rate
+    assert errs[6, 512] < 1e-6, errs
