@@ -73,6 +73,10 @@ __device__ __forceinline__ double2 green(double k, double2 t, double2 s) {
   return make_double2(gr, gi);
 }
 
+/* the same, as a call: keeps the H0 coefficients from being hoisted into
+ * registers for the whole of a kernel that samples only in its prologue */
+__device__ __attribute__((noinline)) double2 green_call(double k, double2 t, double2 s) { return green(k, t, s); }
+
 /* ---------------------------------------------------------- sampling */
 
 #define SAMPLE_THREADS 256
@@ -321,6 +325,314 @@ __device__ void qr_jacobi_solve(double2 *M, int p, int n, int ld, double2 *w, do
   }
 }
 
+/* ---- the same solve for one wave on LDS, tuned for the small blocks that
+ * make up almost all of a factorization (p <= 32): every loop issues its
+ * LDS loads in batches before it waits, lane groups reduce with DPP lane
+ * swaps instead of LDS shuffles, the round-robin needs no integer modulo,
+ * and each rotation costs two FP64 divides instead of five. */
+
+/* x from the lane CTRL selects inside the lane's row of 16 (DPP) */
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+/* partner lanes for groups of 2, 4, 8 and 16: quad_perm [1,0,3,2],
+ * quad_perm [2,3,0,1], row_half_mirror, row_mirror.  After the k-th swap
+ * every lane of an aligned group of 2^k lanes holds the group's sum. */
+#define DPP_X1 0xB1
+#define DPP_X2 0x4E
+#define DPP_HM 0x141
+#define DPP_M 0x140
+
+/* sum over aligned groups of Lp lanes (Lp a power of two <= 64); every lane
+ * of the wave must execute it */
+__device__ __forceinline__ double group_sum(double v, int Lp) {
+  if (Lp >= 2) v += dpp_d<DPP_X1>(v);
+  if (Lp >= 4) v += dpp_d<DPP_X2>(v);
+  if (Lp >= 8) v += dpp_d<DPP_HM>(v);
+  if (Lp >= 16) v += dpp_d<DPP_M>(v);
+  if (Lp >= 32) v += __shfl_xor(v, 16, 64);
+  if (Lp >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+__device__ __forceinline__ void argmax_step(double &v, int &idx, double ov, int oi) {
+  if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+}
+/* max over the wave with the smallest index among maximal entries */
+__device__ __forceinline__ void wave_argmax(double &v, int &idx) {
+  argmax_step(v, idx, dpp_d<DPP_X1>(v), dpp_i<DPP_X1>(idx));
+  argmax_step(v, idx, dpp_d<DPP_X2>(v), dpp_i<DPP_X2>(idx));
+  argmax_step(v, idx, dpp_d<DPP_HM>(v), dpp_i<DPP_HM>(idx));
+  argmax_step(v, idx, dpp_d<DPP_M>(v), dpp_i<DPP_M>(idx));
+  argmax_step(v, idx, __shfl_xor(v, 16, 64), __shfl_xor(idx, 16, 64));
+  argmax_step(v, idx, __shfl_xor(v, 32, 64), __shfl_xor(idx, 32, 64));
+}
+
+/* round-robin (circle method) partner slot of position pos at step `step`
+ * for P (even) rows: position 0 stays, the others rotate */
+__device__ __forceinline__ int rr_row_fast(int pos, int step, int P) {
+  if (pos == 0) return 0;
+  int x = pos - 1 + step;
+  if (x >= P - 1) x -= P - 1;
+  return 1 + x;
+}
+
+/* |M[r][c]|^2 summed over rows [r0, p), loads issued four at a time */
+__device__ __forceinline__ double col_norm2(const double2 *__restrict__ M, int ld, int c, int r0, int p) {
+  double s0 = 0, s1 = 0;
+  int r = r0;
+  for (; r + 3 < p; r += 4) {
+    const double2 a0 = M[r * ld + c], a1 = M[(r + 1) * ld + c], a2 = M[(r + 2) * ld + c], a3 = M[(r + 3) * ld + c];
+    s0 = fma(a0.x, a0.x, fma(a0.y, a0.y, s0));
+    s1 = fma(a1.x, a1.x, fma(a1.y, a1.y, s1));
+    s0 = fma(a2.x, a2.x, fma(a2.y, a2.y, s0));
+    s1 = fma(a3.x, a3.x, fma(a3.y, a3.y, s1));
+  }
+  for (; r < p; ++r) {
+    const double2 a = M[r * ld + c];
+    s0 = fma(a.x, a.x, fma(a.y, a.y, s0));
+  }
+  return s0 + s1;
+}
+
+/* |x[c]|^2 summed over c in [0, n) */
+__device__ __forceinline__ double row_norm2(const double2 *__restrict__ x, int n) {
+  double s0 = 0, s1 = 0;
+  int c = 0;
+  for (; c + 3 < n; c += 4) {
+    const double2 a0 = x[c], a1 = x[c + 1], a2 = x[c + 2], a3 = x[c + 3];
+    s0 = fma(a0.x, a0.x, fma(a0.y, a0.y, s0));
+    s1 = fma(a1.x, a1.x, fma(a1.y, a1.y, s1));
+    s0 = fma(a2.x, a2.x, fma(a2.y, a2.y, s0));
+    s1 = fma(a3.x, a3.x, fma(a3.y, a3.y, s1));
+  }
+  for (; c < n; ++c) s0 = fma(x[c].x, x[c].x, fma(x[c].y, x[c].y, s0));
+  return s0 + s1;
+}
+
+__device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, int n, int ld, double2 *__restrict__ w,
+                                               double2 *__restrict__ v, int *__restrict__ piv, double rcond,
+                                               double2 *__restrict__ out) {
+  const int lane = threadIdx.x;
+  const int W = p + n;
+  const double eps = 2.220446049250313e-16;
+
+  for (int i = lane; i < p; i += 64) piv[i] = i;
+  __syncthreads();
+
+  /* ---- 1. pivoted Householder QR (lane = column) */
+  for (int j = 0; j < p; ++j) {
+    double best = -1.0;
+    int bidx = p;
+    for (int c = j + lane; c < p; c += 64) {
+      const double s = col_norm2(M, ld, c, j, p);
+      if (s > best) { best = s; bidx = c; }
+    }
+    wave_argmax(best, bidx);
+    const int q = __builtin_amdgcn_readfirstlane(bidx);
+    if (q != j) {
+      for (int r = lane; r < p; r += 64) {
+        const double2 t = M[r * ld + j];
+        M[r * ld + j] = M[r * ld + q];
+        M[r * ld + q] = t;
+      }
+      if (lane == 0) { const int t = piv[j]; piv[j] = piv[q]; piv[q] = t; }
+    }
+    __syncthreads();
+    const double sig = sqrt(best);
+    if (sig > 0) {
+      const double2 x0 = M[j * ld + j];
+      const double ax0 = hypot(x0.x, x0.y);
+      const double phx = ax0 > 0 ? x0.x / ax0 : 1.0, phy = ax0 > 0 ? x0.y / ax0 : 0.0;
+      for (int r = j + lane; r < p; r += 64) {
+        double2 a = M[r * ld + j];
+        if (r == j) a = make_double2(x0.x + phx * sig, x0.y + phy * sig);
+        v[r] = a;
+      }
+      __syncthreads();
+      const double inv = 1.0 / (sig * (sig + ax0));
+      for (int c = j + 1 + lane; c < W; c += 64) {
+        double sr = 0, si = 0, tr = 0, ti = 0; /* v^H y in two chains */
+        int r = j;
+        for (; r + 1 < p; r += 2) {
+          const double2 v0 = v[r], y0 = M[r * ld + c], v1 = v[r + 1], y1 = M[(r + 1) * ld + c];
+          sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+          si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+          tr = fma(v1.x, y1.x, fma(v1.y, y1.y, tr));
+          ti = fma(v1.x, y1.y, fma(-v1.y, y1.x, ti));
+        }
+        if (r < p) {
+          const double2 v0 = v[r], y0 = M[r * ld + c];
+          sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+          si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+        }
+        sr = (sr + tr) * inv;
+        si = (si + ti) * inv;
+        r = j;
+        for (; r + 3 < p; r += 4) {
+          double2 y[4], vv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) { vv[u] = v[r + u]; y[u] = M[(r + u) * ld + c]; }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            y[u].x -= sr * vv[u].x - si * vv[u].y;
+            y[u].y -= sr * vv[u].y + si * vv[u].x;
+            M[(r + u) * ld + c] = y[u];
+          }
+        }
+        for (; r < p; ++r) {
+          const double2 vr = v[r];
+          double2 y = M[r * ld + c];
+          y.x -= sr * vr.x - si * vr.y;
+          y.y -= sr * vr.y + si * vr.x;
+          M[r * ld + c] = y;
+        }
+      }
+      __syncthreads();
+      for (int r = j + lane; r < p; r += 64)
+        M[r * ld + j] = (r == j) ? make_double2(-phx * sig, -phy * sig) : make_double2(0, 0);
+    }
+    __syncthreads();
+  }
+
+  /* ---- 2. one-sided Jacobi on the rows of R */
+  const int P = p + (p & 1);
+  const int npairs = P / 2;
+  int Lp = 64;
+  while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
+  const int pairsPerPass = 64 / Lp;
+  const int slot = lane / Lp;
+  const int sub = lane - slot * Lp;
+  const double thr = 4.0 * eps * sqrt((double)p);
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    int rotated = 0;
+    for (int step = 0; step < P - 1; ++step) {
+      for (int pb = 0; pb < npairs; pb += pairsPerPass) {
+        const int pr = pb + slot;
+        int i = 0, j = 0;
+        bool valid = false;
+        if (pr < npairs) {
+          const int a = rr_row_fast(pr, step, P), b = rr_row_fast(P - 1 - pr, step, P);
+          i = min(a, b);
+          j = max(a, b);
+          valid = j < p;
+        }
+        double al = 0, be = 0, gx = 0, gy = 0;
+        if (valid) {
+          const double2 *ri = M + i * ld, *rj = M + j * ld;
+          int c = sub;
+          for (; c + Lp < p; c += 2 * Lp) {
+            const double2 x0 = ri[c], y0 = rj[c], x1 = ri[c + Lp], y1 = rj[c + Lp];
+            al = fma(x0.x, x0.x, fma(x0.y, x0.y, al));
+            be = fma(y0.x, y0.x, fma(y0.y, y0.y, be));
+            gx = fma(x0.x, y0.x, fma(x0.y, y0.y, gx)); /* x conj(y) */
+            gy = fma(x0.y, y0.x, fma(-x0.x, y0.y, gy));
+            al = fma(x1.x, x1.x, fma(x1.y, x1.y, al));
+            be = fma(y1.x, y1.x, fma(y1.y, y1.y, be));
+            gx = fma(x1.x, y1.x, fma(x1.y, y1.y, gx));
+            gy = fma(x1.y, y1.x, fma(-x1.x, y1.y, gy));
+          }
+          if (c < p) {
+            const double2 x0 = ri[c], y0 = rj[c];
+            al = fma(x0.x, x0.x, fma(x0.y, x0.y, al));
+            be = fma(y0.x, y0.x, fma(y0.y, y0.y, be));
+            gx = fma(x0.x, y0.x, fma(x0.y, y0.y, gx));
+            gy = fma(x0.y, y0.x, fma(-x0.x, y0.y, gy));
+          }
+        }
+        al = group_sum(al, Lp);
+        be = group_sum(be, Lp);
+        gx = group_sum(gx, Lp);
+        gy = group_sum(gy, Lp);
+        const double ag2 = gx * gx + gy * gy;
+        if (valid && ag2 > 0 && ag2 > thr * thr * al * be) {
+          rotated = 1;
+          const double ag = sqrt(ag2);
+          const double rg = 1.0 / ag;
+          const double ex = gx * rg, ey = gy * rg;
+          const double zeta = (be - al) * (0.5 * rg);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
+          const double cs = 1.0 / sqrt(fma(t, t, 1.0)), sn = cs * t;
+          const double sx = sn * ex, sy = sn * ey;
+          double2 *ri = M + i * ld, *rj = M + j * ld;
+          int c = sub;
+          for (; c + 3 * Lp < W; c += 4 * Lp) {
+            double2 x[4], y[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { x[u] = ri[c + u * Lp]; y[u] = rj[c + u * Lp]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const double ux = sx * y[u].x - sy * y[u].y, uy = sx * y[u].y + sy * y[u].x; /* s e y */
+              ri[c + u * Lp] = make_double2(cs * x[u].x - ux, cs * x[u].y - uy);
+              rj[c + u * Lp] = make_double2(fma(sn, x[u].x, cs * (ex * y[u].x - ey * y[u].y)),
+                                            fma(sn, x[u].y, cs * (ex * y[u].y + ey * y[u].x)));
+            }
+          }
+          for (; c < W; c += Lp) {
+            const double2 x0 = ri[c], y0 = rj[c];
+            const double ux = sx * y0.x - sy * y0.y, uy = sx * y0.y + sy * y0.x;
+            ri[c] = make_double2(cs * x0.x - ux, cs * x0.y - uy);
+            rj[c] = make_double2(fma(sn, x0.x, cs * (ex * y0.x - ey * y0.y)),
+                                 fma(sn, x0.y, cs * (ex * y0.y + ey * y0.x)));
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!__any(rotated)) break;
+  }
+
+  /* ---- 3. truncation and X */
+  double smax = 0;
+  for (int i = lane; i < p; i += 64) {
+    const double s = row_norm2(M + i * ld, p);
+    w[i].x = s;
+    smax = fmax(smax, s);
+  }
+  int dummy = 0;
+  wave_argmax(smax, dummy);
+  __syncthreads();
+  const double tol = fmax((double)p * eps * sqrt(smax) + eps, rcond * sqrt(smax));
+  for (int i = lane; i < p; i += 64) {
+    const double s = sqrt(w[i].x);
+    w[i].y = (s >= tol) ? 1.0 / w[i].x : 0.0;
+  }
+  __syncthreads();
+  const int cnt = p * n;
+  int c = lane / p, a = lane - (lane / p) * p;
+  const int dc = 64 / p, da = 64 - dc * p;
+  for (int e = lane; e < cnt; e += 64) {
+    double ax = 0, ay = 0, bx = 0, by = 0;
+    int i = 0;
+    for (; i + 1 < p; i += 2) {
+      const double w0 = w[i].y, w1 = w[i + 1].y;
+      const double2 x0 = M[i * ld + a], y0 = M[i * ld + p + c], x1 = M[(i + 1) * ld + a], y1 = M[(i + 1) * ld + p + c];
+      ax = fma(w0, x0.x * y0.x + x0.y * y0.y, ax); /* conj(x) y */
+      ay = fma(w0, x0.x * y0.y - x0.y * y0.x, ay);
+      bx = fma(w1, x1.x * y1.x + x1.y * y1.y, bx);
+      by = fma(w1, x1.x * y1.y - x1.y * y1.x, by);
+    }
+    if (i < p) {
+      const double w0 = w[i].y;
+      const double2 x0 = M[i * ld + a], y0 = M[i * ld + p + c];
+      ax = fma(w0, x0.x * y0.x + x0.y * y0.y, ax);
+      ay = fma(w0, x0.x * y0.y - x0.y * y0.x, ay);
+    }
+    out[(size_t)c * p + piv[a]] = make_double2(ax + bx, ay + by);
+    /* next e = e + 64: advance (c, a) without dividing */
+    c += dc;
+    a += da;
+    if (a >= p) { a -= p; ++c; }
+  }
+}
+
 struct LstsqView {
   double2 *M, *w, *v, *pts;
   double *red;
@@ -342,8 +654,10 @@ __device__ __forceinline__ LstsqView lstsq_view(double2 *base, int p, int n) {
 /* One job per block of T threads.  The working set lives in LDS, or for
  * jobs too large for LDS (about one in 10^4 at N = 2^20) in a per-block
  * slice of a global scratch buffer. */
+/* five waves per SIMD: the solve fits in 96 VGPRs without spills, and the
+ * LDS of the common 8-12 KiB jobs allows about that many per CU anyway */
 template <int T, bool GLOBAL>
-__global__ __launch_bounds__(T) void k_lstsq(const BfkLstsqJob *__restrict__ jobs, const int32_t *__restrict__ order,
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_lstsq(const BfkLstsqJob *__restrict__ jobs, const int32_t *__restrict__ order,
                                              const double2 *__restrict__ srcXY, double k,
                                              double2 *__restrict__ arena, double2 *__restrict__ gwork,
                                              int64_t gstride, double rcond) {
@@ -365,10 +679,13 @@ __global__ __launch_bounds__(T) void k_lstsq(const BfkLstsqJob *__restrict__ job
     int i = e / W;
     int c = e - i * W;
     double2 s = c < p ? ep[c] : op[c - p];
-    L.M[i * ld + c] = green(k, tp[i], s);
+    L.M[i * ld + c] = green_call(k, tp[i], s);
   }
   __syncthreads();
-  qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, rcond, arena + J.out);
+  if (T == 64 && !GLOBAL)
+    qr_jacobi_wave(L.M, p, n, ld, L.w, L.v, L.piv, rcond, arena + J.out);
+  else
+    qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, rcond, arena + J.out);
 }
 
 /* the same solve on matrices given in global memory (row-major A p x p,
@@ -387,7 +704,7 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
     L.M[i * ld + c] = c < p ? a[i * p + c] : b[i * n + (c - p)];
   }
   __syncthreads();
-  qr_jacobi_solve<64>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, 0.0, X + (size_t)blockIdx.x * p * n);
+  qr_jacobi_wave(L.M, p, n, ld, L.w, L.v, L.piv, 0.0, X + (size_t)blockIdx.x * p * n);
 }
 
 /* ------------------------------------ boundary integral operator, GMRES */
