@@ -502,6 +502,93 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
     __syncthreads();
   }
 
+  /* ---- 1b. when no singular value can fall below the cut, the
+   * pseudo-inverse is the inverse and X = R^-1 Q^H B.  sigma_max <= |R|_F
+   * bounds the cut from above and sigma_min >= 1 / |R^-1|_F bounds the
+   * smallest singular value from below; Z = R^-1 goes into the (zero)
+   * strict lower triangle of M, transposed, with its diagonal in w. */
+  {
+    double rf = 0;
+    for (int c = lane; c < p; c += 64) rf += col_norm2(M, ld, c, 0, c + 1);
+    rf = group_sum(rf, 64);
+    for (int i = lane; i < p; i += 64) {
+      const double2 d = M[i * ld + i];
+      const double m2 = d.x * d.x + d.y * d.y;
+      v[i] = make_double2(d.x / m2, -d.y / m2);
+    }
+    __syncthreads();
+    double zf = 0;
+    for (int j = lane; j < p; j += 64) {
+      const double2 zjj = v[j];
+      w[j] = zjj;
+      zf = zjj.x * zjj.x + zjj.y * zjj.y;
+      double2 *zj = M + j * ld; /* z_kj at zj[k] for k < j */
+      for (int i = j - 1; i >= 0; --i) {
+        const double2 *ri = M + i * ld;
+        const double2 rij = ri[j];
+        double sx = rij.x * zjj.x - rij.y * zjj.y, sy = rij.x * zjj.y + rij.y * zjj.x;
+        double tx = 0, ty = 0;
+        int k = i + 1;
+        for (; k + 1 < j; k += 2) {
+          const double2 a0 = ri[k], b0 = zj[k], a1 = ri[k + 1], b1 = zj[k + 1];
+          sx = fma(a0.x, b0.x, fma(-a0.y, b0.y, sx));
+          sy = fma(a0.x, b0.y, fma(a0.y, b0.x, sy));
+          tx = fma(a1.x, b1.x, fma(-a1.y, b1.y, tx));
+          ty = fma(a1.x, b1.y, fma(a1.y, b1.x, ty));
+        }
+        if (k < j) {
+          const double2 a0 = ri[k], b0 = zj[k];
+          sx = fma(a0.x, b0.x, fma(-a0.y, b0.y, sx));
+          sy = fma(a0.x, b0.y, fma(a0.y, b0.x, sy));
+        }
+        sx += tx;
+        sy += ty;
+        const double2 di = v[i];
+        const double2 z = make_double2(-(sx * di.x - sy * di.y), -(sx * di.y + sy * di.x));
+        zj[i] = z;
+        zf = fma(z.x, z.x, fma(z.y, z.y, zf));
+      }
+    }
+    zf = group_sum(zf, 64);
+    const double sigMaxUp = sqrt(rf);
+    const double cutUp = fmax((double)p * eps * sigMaxUp + eps, rcond * sigMaxUp);
+    const bool direct = zf > 0 && zf <= 1.7976931348623157e308 && 1.0 / sqrt(zf) >= cutUp;
+    __syncthreads();
+    if (direct) {
+      const int cnt = p * n;
+      int c = lane / p, a = lane - (lane / p) * p;
+      const int dc = 64 / p, da = 64 - dc * p;
+      for (int e = lane; e < cnt; e += 64) {
+        const double2 za = w[a], b0 = M[a * ld + p + c];
+        double ax = za.x * b0.x - za.y * b0.y, ay = za.x * b0.y + za.y * b0.x;
+        double bx = 0, by = 0;
+        int i = a + 1;
+        for (; i + 1 < p; i += 2) {
+          const double2 z0 = M[i * ld + a], y0 = M[i * ld + p + c], z1 = M[(i + 1) * ld + a],
+                        y1 = M[(i + 1) * ld + p + c];
+          ax = fma(z0.x, y0.x, fma(-z0.y, y0.y, ax));
+          ay = fma(z0.x, y0.y, fma(z0.y, y0.x, ay));
+          bx = fma(z1.x, y1.x, fma(-z1.y, y1.y, bx));
+          by = fma(z1.x, y1.y, fma(z1.y, y1.x, by));
+        }
+        if (i < p) {
+          const double2 z0 = M[i * ld + a], y0 = M[i * ld + p + c];
+          ax = fma(z0.x, y0.x, fma(-z0.y, y0.y, ax));
+          ay = fma(z0.x, y0.y, fma(z0.y, y0.x, ay));
+        }
+        out[(size_t)c * p + piv[a]] = make_double2(ax + bx, ay + by);
+        c += dc;
+        a += da;
+        if (a >= p) { a -= p; ++c; }
+      }
+      return;
+    }
+    /* back to R: clear the strict lower triangle again */
+    for (int i = lane; i < p; i += 64)
+      for (int k = 0; k < i; ++k) M[i * ld + k] = make_double2(0, 0);
+    __syncthreads();
+  }
+
   /* ---- 2. one-sided Jacobi on the rows of R */
   const int P = p + (p & 1);
   const int npairs = P / 2;
