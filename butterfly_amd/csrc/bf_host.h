@@ -15,6 +15,7 @@ size_t bfk_lds_cap(void);
 size_t bfk_lstsq_lds_bytes(int p, int n);
 void *bfk_malloc(size_t bytes);
 void bfk_free(void *p);
+void bfk_free_async(void *p);
 int bfk_h2d(void *dst, const void *src, size_t bytes);
 int bfk_d2h(void *dst, const void *src, size_t bytes);
 int bfk_memset(void *dst, size_t bytes);

@@ -37,6 +37,10 @@
 
 static hipStream_t g_stream = nullptr;
 static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+/* the compression launches return before they finish: their events are
+ * read at the next bfk_sync / bfk_get_times */
+static hipEvent_t g_evL0 = nullptr, g_evL1 = nullptr;
+static int g_lPending = 0;
 static size_t g_ldsCap = 0;
 static double g_times[8];   /* last measured ms per category */
 static double g_sum[8];     /* accumulated ms per category since the last reset */
@@ -1522,6 +1526,8 @@ int bfk_init(void) {
   BFK_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   BFK_CHECK(hipEventCreate(&g_ev0));
   BFK_CHECK(hipEventCreate(&g_ev1));
+  BFK_CHECK(hipEventCreate(&g_evL0));
+  BFK_CHECK(hipEventCreate(&g_evL1));
   hipDeviceProp_t prop;
   BFK_CHECK(hipGetDeviceProperties(&prop, 0));
   g_ldsCap = prop.sharedMemPerBlock;
@@ -1563,6 +1569,12 @@ void bfk_free(void *p) {
   if (p) (void)hipFree(p);
 }
 
+/* released once the work queued before it on the library's stream is done;
+ * the host does not wait */
+void bfk_free_async(void *p) {
+  if (p && bfk_init() == 0) (void)hipFreeAsync(p, g_stream);
+}
+
 int bfk_h2d(void *dst, const void *src, size_t bytes) {
   if (bfk_init()) return -1;
   BFK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
@@ -1583,17 +1595,31 @@ int bfk_memset(void *dst, size_t bytes) {
   return 0;
 }
 
-int bfk_sync(void) {
-  if (bfk_init()) return -1;
-  BFK_CHECK(hipStreamSynchronize(g_stream));
+static int collect_pending(void) {
+  if (!g_lPending) return 0;
+  g_lPending = 0;
+  BFK_CHECK(hipEventSynchronize(g_evL1));
+  float ms = 0;
+  BFK_CHECK(hipEventElapsedTime(&ms, g_evL0, g_evL1));
+  g_times[1] = ms;
+  g_sum[1] += ms;
+  g_cnt[1] += 1;
   return 0;
 }
 
+int bfk_sync(void) {
+  if (bfk_init()) return -1;
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  return collect_pending();
+}
+
 void bfk_get_times(double out[8]) {
+  (void)collect_pending();
   for (int i = 0; i < 8; ++i) out[i] = g_times[i];
 }
 
 void bfk_get_time_sums(double sums[8], double counts[8]) {
+  (void)collect_pending();
   for (int i = 0; i < 8; ++i) {
     sums[i] = g_sum[i];
     counts[i] = g_cnt[i];
@@ -1655,7 +1681,8 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
 int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
               const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena) {
   if (bfk_init()) return -1;
-  if (time_begin()) return -1;
+  if (collect_pending()) return -1;
+  BFK_CHECK(hipEventRecord(g_evL0, g_stream));
   int64_t gs = -1, ge = -1;
   size_t gmax = 0;
   for (int gi = 0; gi < numGroups; ++gi) {
@@ -1678,21 +1705,24 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
     }
   }
   if (gs >= 0) {
+    /* the scratch slices are reused chunk after chunk in stream order and
+     * released in stream order, so the host does not wait here */
     const int64_t stride = (int64_t)(gmax / sizeof(double2));
     const int64_t chunk = 4096;
     const int64_t nbig = ge - gs;
     double2 *work = nullptr;
-    BFK_CHECK(hipMalloc(&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax));
+    BFK_CHECK(hipMallocAsync((void **)&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax, g_stream));
     for (int64_t j0 = gs; j0 < ge; j0 += chunk) {
       int64_t nj = ge - j0 < chunk ? ge - j0 : chunk;
       hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, g_stream, dJobs, dOrder + j0,
                          (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride, rcond);
       BFK_CHECK(hipGetLastError());
     }
-    BFK_CHECK(hipStreamSynchronize(g_stream));
-    BFK_CHECK(hipFree(work));
+    BFK_CHECK(hipFreeAsync(work, g_stream));
   }
-  return time_end(1);
+  BFK_CHECK(hipEventRecord(g_evL1, g_stream));
+  g_lPending = 1;
+  return 0;
 }
 
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }

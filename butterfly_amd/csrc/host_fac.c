@@ -335,11 +335,28 @@ static double *tree_coords(BfQuadtree const *t) {
   return xy;
 }
 
-typedef struct { size_t bytes; int32_t idx; } JobKey;
-static int cmp_jobkey(const void *a, const void *b) {
-  const JobKey *x = a, *y = b;
-  if (x->bytes != y->bytes) return x->bytes < y->bytes ? -1 : 1;
-  return x->idx < y->idx ? -1 : x->idx > y->idx;
+typedef BF_VEC(size_t) BfSizeVec;
+
+/* Stable counting sort of jobs by LDS class (class c launches with c * 4 KiB):
+ * order[] lists the jobs class by class, in plan order inside a class;
+ * gs gets each non-empty class's first position (plus a final nj) and gb
+ * its byte size. */
+static void group_by_class(const uint32_t *cls, size_t nj, int32_t *order, BfI64Vec *gs, BfSizeVec *gb) {
+  uint32_t maxCls = 0;
+  for (size_t i = 0; i < nj; ++i)
+    if (cls[i] > maxCls) maxCls = cls[i];
+  size_t *start = calloc((size_t)maxCls + 2, sizeof(size_t));
+  for (size_t i = 0; i < nj; ++i) ++start[cls[i] + 1];
+  for (size_t c = 0; c <= maxCls; ++c) start[c + 1] += start[c];
+  gs->n = gb->n = 0;
+  for (size_t c = 0; c <= maxCls; ++c)
+    if (start[c + 1] > start[c]) {
+      BF_VEC_PUSH(*gs, (int64_t)start[c]);
+      BF_VEC_PUSH(*gb, c << 12);
+    }
+  BF_VEC_PUSH(*gs, (int64_t)nj);
+  for (size_t i = 0; i < nj; ++i) order[start[cls[i]]++] = (int32_t)i;
+  free(start);
 }
 
 static int run_factor_kernels(BfMatHipButterfly *F) {
@@ -355,33 +372,23 @@ static int run_factor_kernels(BfMatHipButterfly *F) {
   /* re-expansion jobs grouped by the LDS they need (sizes rounded up to
    * 4 KiB so a launch serves many jobs) */
   size_t nj = F->lstsqJobs.n;
-  JobKey *keys = malloc((nj ? nj : 1) * sizeof(JobKey));
-  for (size_t i = 0; i < nj; ++i) {
-    size_t b = bfk_lstsq_lds_bytes(F->lstsqJobs.a[i].p, F->lstsqJobs.a[i].n);
-    keys[i].bytes = (b + 4095) & ~(size_t)4095;
-    keys[i].idx = (int32_t)i;
-  }
-  qsort(keys, nj, sizeof(JobKey), cmp_jobkey);
+  uint32_t *cls = malloc((nj ? nj : 1) * sizeof(uint32_t));
+  for (size_t i = 0; i < nj; ++i)
+    cls[i] = (uint32_t)((bfk_lstsq_lds_bytes(F->lstsqJobs.a[i].p, F->lstsqJobs.a[i].n) + 4095) >> 12);
   int32_t *order = malloc((nj ? nj : 1) * sizeof(int32_t));
-  BF_VEC(int64_t) gs;
-  BF_VEC(size_t) gb;
+  BfI64Vec gs;
+  BfSizeVec gb;
   memset(&gs, 0, sizeof(gs));
   memset(&gb, 0, sizeof(gb));
-  for (size_t i = 0; i < nj; ++i) {
-    order[i] = keys[i].idx;
-    if (i == 0 || keys[i].bytes != keys[i - 1].bytes) {
-      BF_VEC_PUSH(gs, (int64_t)i);
-      BF_VEC_PUSH(gb, keys[i].bytes);
-    }
-  }
-  BF_VEC_PUSH(gs, (int64_t)nj);
-  free(keys);
+  group_by_class(cls, nj, order, &gs, &gb);
+  free(cls);
   rc = upload(&dL, F->lstsqJobs.a, nj * sizeof(BfkLstsqJob));
   if (!rc) rc = upload(&dOrder, order, nj * sizeof(int32_t));
   if (!rc && nj)
     rc = bfk_lstsq(dL, dOrder, (int)gb.n, gs.a, gb.a, F->dSrcXY, F->helm.k, F->opts.pinvRcond, F->dArena);
-  bfk_free(dL);
-  bfk_free(dOrder);
+  /* the launches are still running: release their tables in stream order */
+  bfk_free_async(dL);
+  bfk_free_async(dOrder);
   free(order);
   BF_VEC_FREE(gs);
   BF_VEC_FREE(gb);
@@ -423,9 +430,12 @@ BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const
   free(sxy);
   free(txy);
   if (!rc) rc = run_factor_kernels(F);
+  /* the apply plan is host work: build it while the compression runs */
+  double ta = bfWallTime();
+  if (!rc) rc = bfApplyPlanBuild(F) ? -1 : 0;
+  double tApplyPlan = bfWallTime() - ta;
   if (!rc) rc = bfk_sync();
   double t2 = bfWallTime();
-  if (!rc) rc = bfApplyPlanBuild(F) ? -1 : 0;
   if (!rc) rc = upload((void **)&F->dSegs, F->segs.a, F->segs.n * sizeof(BfkSeg));
   if (!rc) rc = upload((void **)&F->dTerms, F->terms.a, F->terms.n * sizeof(BfkTerm));
   if (rc) {
@@ -464,7 +474,7 @@ BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const
   F->stats[10] = h0;
   F->stats[13] = ktimes[0];
   F->stats[14] = ktimes[1];
-  F->stats[15] = t3 - t2;
+  F->stats[15] = tApplyPlan + (t3 - t2);
   return &F->super;
 }
 
@@ -744,31 +754,21 @@ int bfhipFacOrthogonalize(BfMat *fac) {
     dOrder = bfk_malloc((maxJobs ? maxJobs : 1) * sizeof(int32_t));
     if (!dJobs || !dOrder) rc = -2;
   }
-  JobKey *keys = malloc((maxJobs ? maxJobs : 1) * sizeof(JobKey));
+  uint32_t *cls = malloc((maxJobs ? maxJobs : 1) * sizeof(uint32_t));
   int32_t *order = malloc((maxJobs ? maxJobs : 1) * sizeof(int32_t));
-  BF_VEC(int64_t) gs;
-  BF_VEC(size_t) gb;
+  BfI64Vec gs;
+  BfSizeVec gb;
   memset(&gs, 0, sizeof(gs));
   memset(&gb, 0, sizeof(gb));
+  const uint32_t capCls = (uint32_t)((cap * 16) >> 12);
   for (int g = Fmax - 1; g >= 0 && !rc; --g) {
     size_t nj = jobs[g].n;
     if (nj == 0) continue;
     for (size_t i = 0; i < nj; ++i) {
-      size_t b = bfk_ortho_units(jobs[g].a[i].rows, jobs[g].a[i].n) * 16;
-      keys[i].bytes = (b + 4095) & ~(size_t)4095;
-      if (keys[i].bytes > cap * 16) keys[i].bytes = cap * 16;
-      keys[i].idx = (int32_t)i;
+      uint32_t c = (uint32_t)((bfk_ortho_units(jobs[g].a[i].rows, jobs[g].a[i].n) * 16 + 4095) >> 12);
+      cls[i] = c > capCls ? capCls : c;
     }
-    qsort(keys, nj, sizeof(JobKey), cmp_jobkey);
-    gs.n = gb.n = 0;
-    for (size_t i = 0; i < nj; ++i) {
-      order[i] = keys[i].idx;
-      if (i == 0 || keys[i].bytes != keys[i - 1].bytes) {
-        BF_VEC_PUSH(gs, (int64_t)i);
-        BF_VEC_PUSH(gb, keys[i].bytes);
-      }
-    }
-    BF_VEC_PUSH(gs, (int64_t)nj);
+    group_by_class(cls, nj, order, &gs, &gb);
     rc = bfk_h2d(dJobs, jobs[g].a, nj * sizeof(BfkOrthoJob));
     if (!rc) rc = bfk_h2d(dOrder, order, nj * sizeof(int32_t));
     if (!rc) rc = bfk_ortho(dJobs, dOrder, (int)gb.n, gs.a, gb.a, dItems, F->dArena, dR[(g + 1) % 2], dR[g % 2]);
@@ -776,7 +776,7 @@ int bfhipFacOrthogonalize(BfMat *fac) {
   }
   BF_VEC_FREE(gs);
   BF_VEC_FREE(gb);
-  free(keys);
+  free(cls);
   free(order);
   bfk_free(dJobs);
   bfk_free(dOrder);
