@@ -7,6 +7,9 @@
  */
 #define _DEFAULT_SOURCE 1
 #include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <unistd.h>
 
 #include "bf_host.h"
 
@@ -17,10 +20,17 @@ typedef struct {
   double *cx, *cy, *cr; /* bounding circles per node */
 } TreeView;
 
+/* one top-level block of the plan: a dense leaf or a separated pair */
+typedef struct {
+  int32_t s, t, separated;
+} PlanItem;
+typedef BF_VEC(PlanItem) PlanItemVec;
+
 typedef struct {
   BfMatHipButterfly *F;
   TreeView src, tgt;
   double k, tol, C;
+  PlanItemVec *items; /* when set, the recursion only lists the top-level blocks */
 } Planner;
 
 typedef BF_VEC(int32_t) Level;
@@ -428,6 +438,17 @@ static void make_separated(Planner *P, int32_t s, int32_t t) {
   free_levels(tl, ntl);
 }
 
+static void emit(Planner *P, int32_t s, int32_t t, int sep) {
+  if (P->items) {
+    PlanItem it = {s, t, sep};
+    BF_VEC_PUSH(*P->items, it);
+  } else if (sep) {
+    make_separated(P, s, t);
+  } else {
+    make_dense(P, s, t);
+  }
+}
+
 /* facHelm2MakeMultilevel_rec (fac_helm2.c:860-941) */
 static void multilevel_rec(Planner *P, const int32_t *sNodes, int ns, const int32_t *tNodes, int nt) {
   const TreeView *sv = &P->src, *tv = &P->tgt;
@@ -435,22 +456,132 @@ static void multilevel_rec(Planner *P, const int32_t *sNodes, int ns, const int3
     for (int j = 0; j < ns; ++j) {
       int32_t s = sNodes[j], t = tNodes[i];
       if ((BfSize)(npts(tv, t) * npts(sv, s)) < P->F->opts.maxDenseSize) {
-        make_dense(P, s, t);
+        emit(P, s, t, 0);
       } else if (separated(sv, s, tv, t)) {
-        make_separated(P, s, t);
+        emit(P, s, t, 1);
       } else {
         /* a leaf stands in for its own child list; two leaves that are
          * still not separated become one dense block */
         int32_t sc[4], tc[4];
         int nsc = children(sv, s, sc, 1), ntc = children(tv, t, tc, 1);
         if (nsc == 1 && ntc == 1 && sc[0] == s && tc[0] == t) {
-          make_dense(P, s, t);
+          emit(P, s, t, 0);
           continue;
         }
         multilevel_rec(P, sc, nsc, tc, ntc);
       }
     }
   }
+}
+
+/* ---- parallel build: the top-level blocks are independent, so they are
+ * built by a pool of threads into per-chunk tables (chunks of consecutive
+ * blocks, handed out in order), which are then concatenated in block order
+ * with their indices and arena offsets shifted.  The result is the same
+ * plan, entry for entry, as building the blocks one after another. */
+
+typedef struct {
+  BfMatHipButterfly T; /* local tables: leaves, factors, blocks, offs, jobs, arena size */
+  size_t i0, i1;       /* items */
+  /* bases in the merged tables */
+  int64_t leafBase, factorBase, blockBase, offsBase, lBase, kBase, arenaBase;
+} PlanChunk;
+
+typedef struct {
+  const Planner *P;
+  const PlanItemVec *items;
+  PlanChunk *chunks;
+  size_t numChunks;
+  BfMatHipButterfly *F;
+  atomic_size_t next;
+  int phase; /* 0 build, 1 merge */
+} PlanPool;
+
+static void build_chunk(PlanPool *pool, PlanChunk *c) {
+  Planner L = *pool->P;
+  L.items = NULL;
+  L.F = &c->T;
+  c->T.helm = pool->F->helm;
+  c->T.opts = pool->F->opts;
+  for (size_t i = c->i0; i < c->i1; ++i) emit(&L, pool->items->a[i].s, pool->items->a[i].t, pool->items->a[i].separated);
+}
+
+#define COPY_TABLE(dst, src, base) memcpy((dst).a + (base), (src).a, (src).n * sizeof(*(src).a))
+
+static void merge_chunk(PlanPool *pool, PlanChunk *c) {
+  BfMatHipButterfly *F = pool->F, *T = &c->T;
+  COPY_TABLE(F->leaves, T->leaves, c->leafBase);
+  for (size_t i = 0; i < T->leaves.n; ++i) {
+    BfHipLeaf *l = &F->leaves.a[c->leafBase + (int64_t)i];
+    l->firstFactor += c->factorBase;
+    if (l->denseOff >= 0) l->denseOff += c->arenaBase;
+  }
+  COPY_TABLE(F->factors, T->factors, c->factorBase);
+  for (size_t i = 0; i < T->factors.n; ++i) {
+    BfHipFactor *f = &F->factors.a[c->factorBase + (int64_t)i];
+    f->firstBlock += c->blockBase;
+    f->rowOffStart += c->offsBase;
+    f->colOffStart += c->offsBase;
+  }
+  COPY_TABLE(F->blocks, T->blocks, c->blockBase);
+  for (size_t i = 0; i < T->blocks.n; ++i) {
+    BfHipBlock *b = &F->blocks.a[c->blockBase + (int64_t)i];
+    if (b->dataOff >= 0) b->dataOff += c->arenaBase;
+    if (b->job >= 0) b->job += (int32_t)(b->kind == BF_BLOCK_LSTSQ ? c->lBase : c->kBase);
+  }
+  COPY_TABLE(F->offs, T->offs, c->offsBase);
+  COPY_TABLE(F->lstsqJobs, T->lstsqJobs, c->lBase);
+  for (size_t i = 0; i < T->lstsqJobs.n; ++i) F->lstsqJobs.a[c->lBase + (int64_t)i].out += c->arenaBase;
+  COPY_TABLE(F->kernelJobs, T->kernelJobs, c->kBase);
+  for (size_t i = 0; i < T->kernelJobs.n; ++i) F->kernelJobs.a[c->kBase + (int64_t)i].out += c->arenaBase;
+  BF_VEC_FREE(T->leaves);
+  BF_VEC_FREE(T->factors);
+  BF_VEC_FREE(T->blocks);
+  BF_VEC_FREE(T->offs);
+  BF_VEC_FREE(T->lstsqJobs);
+  BF_VEC_FREE(T->kernelJobs);
+}
+
+static void *plan_worker(void *arg) {
+  PlanPool *pool = arg;
+  for (;;) {
+    size_t c = atomic_fetch_add(&pool->next, 1);
+    if (c >= pool->numChunks) break;
+    if (pool->phase == 0)
+      build_chunk(pool, &pool->chunks[c]);
+    else
+      merge_chunk(pool, &pool->chunks[c]);
+  }
+  return NULL;
+}
+
+static void run_pool(PlanPool *pool, int phase, int nthreads) {
+  pool->phase = phase;
+  atomic_store(&pool->next, 0);
+  pthread_t th[64];
+  int started = 0;
+  for (int i = 1; i < nthreads; ++i)
+    if (pthread_create(&th[started], NULL, plan_worker, pool) == 0) ++started;
+  plan_worker(pool);
+  for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
+}
+
+#define RESERVE(v, total)                                              \
+  do {                                                                 \
+    if ((v).cap < (size_t)(total)) {                                   \
+      void *na_ = realloc((v).a, ((size_t)(total) ? (size_t)(total) : 1) * sizeof(*(v).a)); \
+      if (!na_) abort();                                               \
+      (v).a = na_;                                                     \
+      (v).cap = (size_t)(total);                                       \
+    }                                                                  \
+    (v).n = (size_t)(total);                                           \
+  } while (0)
+
+static int plan_threads(void) {
+  long n = sysconf(_SC_NPROCESSORS_ONLN);
+  if (n < 1) n = 1;
+  if (n > 16) n = 16;
+  return (int)n;
 }
 
 int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt) {
@@ -472,7 +603,71 @@ int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *t
   int n2s = 0, n2t = 0;
   for (BfSize i = src->levelOffset[2]; i < src->levelOffset[3]; ++i) s2[n2s++] = (int32_t)i;
   for (BfSize i = tgt->levelOffset[2]; i < tgt->levelOffset[3]; ++i) t2[n2t++] = (int32_t)i;
+  /* list the top-level blocks in plan order, then build them */
+  PlanItemVec items;
+  memset(&items, 0, sizeof(items));
+  P.items = &items;
   multilevel_rec(&P, s2, n2s, t2, n2t);
+  P.items = NULL;
+  /* chunks of about equal point count, so one huge block does not sit
+   * in a chunk with many others */
+  const int nthreads = plan_threads();
+  double totalW = 0;
+  for (size_t i = 0; i < items.n; ++i)
+    totalW += (double)(npts(&P.src, items.a[i].s) + npts(&P.tgt, items.a[i].t));
+  const double target = totalW / (double)(64 * nthreads) + 1;
+  size_t nc = 0;
+  PlanChunk *chunks = calloc(items.n ? items.n : 1, sizeof(PlanChunk));
+  for (size_t i = 0; i < items.n;) {
+    double w = 0;
+    size_t j = i;
+    while (j < items.n && (j == i || w < target)) {
+      w += (double)(npts(&P.src, items.a[j].s) + npts(&P.tgt, items.a[j].t));
+      ++j;
+    }
+    chunks[nc].i0 = i;
+    chunks[nc].i1 = j;
+    ++nc;
+    i = j;
+  }
+  PlanPool pool;
+  memset(&pool, 0, sizeof(pool));
+  pool.P = &P;
+  pool.items = &items;
+  pool.chunks = chunks;
+  pool.numChunks = nc;
+  pool.F = F;
+  run_pool(&pool, 0, nthreads);
+  int64_t nl = (int64_t)F->leaves.n, nf = (int64_t)F->factors.n, nb = (int64_t)F->blocks.n;
+  int64_t no = (int64_t)F->offs.n, njl = (int64_t)F->lstsqJobs.n, njk = (int64_t)F->kernelJobs.n;
+  int64_t na = F->arenaEntries;
+  for (size_t c = 0; c < nc; ++c) {
+    BfMatHipButterfly *T = &chunks[c].T;
+    chunks[c].leafBase = nl;
+    chunks[c].factorBase = nf;
+    chunks[c].blockBase = nb;
+    chunks[c].offsBase = no;
+    chunks[c].lBase = njl;
+    chunks[c].kBase = njk;
+    chunks[c].arenaBase = na;
+    nl += (int64_t)T->leaves.n;
+    nf += (int64_t)T->factors.n;
+    nb += (int64_t)T->blocks.n;
+    no += (int64_t)T->offs.n;
+    njl += (int64_t)T->lstsqJobs.n;
+    njk += (int64_t)T->kernelJobs.n;
+    na += T->arenaEntries;
+  }
+  RESERVE(F->leaves, nl);
+  RESERVE(F->factors, nf);
+  RESERVE(F->blocks, nb);
+  RESERVE(F->offs, no);
+  RESERVE(F->lstsqJobs, njl);
+  RESERVE(F->kernelJobs, njk);
+  F->arenaEntries = na;
+  run_pool(&pool, 1, nthreads);
+  free(chunks);
+  BF_VEC_FREE(items);
   tree_view_free(&P.src);
   tree_view_free(&P.tgt);
   return 0;
