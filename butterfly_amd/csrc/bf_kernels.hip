@@ -1006,24 +1006,43 @@ __host__ __device__ __forceinline__ size_t ortho_units(int rows, int n) {
   return (size_t)ortho_ld(rows) * n + 2 * (size_t)n;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return group_sum(v, 64); }
 
 /* y -= v (v^H y) iv over rows [j, rows) of one column (a Householder
- * reflector I - iv v v^H) */
-__device__ __forceinline__ void house_apply(const double2 *v, double2 *y, int j, int rows, double iv) {
-  double sr = 0, si = 0;
-  for (int r = j; r < rows; ++r) {
-    double2 vr = v[r], yr = y[r];
-    sr = fma(vr.x, yr.x, fma(vr.y, yr.y, sr));
-    si = fma(vr.x, yr.y, fma(-vr.y, yr.x, si));
+ * reflector I - iv v v^H); loads are issued in batches before the waits */
+__device__ __forceinline__ void house_apply(const double2 *__restrict__ v, double2 *__restrict__ y, int j, int rows,
+                                            double iv) {
+  double sr = 0, si = 0, tr = 0, ti = 0;
+  int r = j;
+  for (; r + 1 < rows; r += 2) {
+    const double2 v0 = v[r], y0 = y[r], v1 = v[r + 1], y1 = y[r + 1];
+    sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+    si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+    tr = fma(v1.x, y1.x, fma(v1.y, y1.y, tr));
+    ti = fma(v1.x, y1.y, fma(-v1.y, y1.x, ti));
   }
-  sr *= iv;
-  si *= iv;
-  for (int r = j; r < rows; ++r) {
-    double2 vr = v[r], yr = y[r];
+  if (r < rows) {
+    const double2 v0 = v[r], y0 = y[r];
+    sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+    si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+  }
+  sr = (sr + tr) * iv;
+  si = (si + ti) * iv;
+  r = j;
+  for (; r + 3 < rows; r += 4) {
+    double2 vv[4], yy[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { vv[u] = v[r + u]; yy[u] = y[r + u]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      yy[u].x -= sr * vv[u].x - si * vv[u].y;
+      yy[u].y -= sr * vv[u].y + si * vv[u].x;
+      y[r + u] = yy[u];
+    }
+  }
+  for (; r < rows; ++r) {
+    const double2 vr = v[r];
+    double2 yr = y[r];
     yr.x -= sr * vr.x - si * vr.y;
     yr.y -= sr * vr.y + si * vr.x;
     y[r] = yr;
@@ -1061,9 +1080,19 @@ __global__ __launch_bounds__(64) void k_ortho(const BfkOrthoJob *__restrict__ jo
       for (int c = lane; c < n; c += 64) {
         double2 *col = M + c * ld + off;
         for (int r = 0; r < m; ++r) {
-          double2 acc = make_double2(0, 0);
-          for (int k = r; k < m; ++k) acc = cfma(R[k * m + r], col[k], acc);
-          col[r] = acc;
+          double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+          int k = r;
+          for (; k + 3 < m; k += 4) {
+            const double2 r0 = R[k * m + r], r1 = R[(k + 1) * m + r], r2 = R[(k + 2) * m + r],
+                          r3 = R[(k + 3) * m + r];
+            const double2 c0 = col[k], c1 = col[k + 1], c2 = col[k + 2], c3 = col[k + 3];
+            a0 = cfma(r0, c0, a0);
+            a1 = cfma(r1, c1, a1);
+            a0 = cfma(r2, c2, a0);
+            a1 = cfma(r3, c3, a1);
+          }
+          for (; k < m; ++k) a0 = cfma(R[k * m + r], col[k], a0);
+          col[r] = make_double2(a0.x + a1.x, a0.y + a1.y);
         }
       }
       __syncthreads();
