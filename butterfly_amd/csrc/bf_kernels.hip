@@ -36,6 +36,11 @@
   } while (0)
 
 static hipStream_t g_stream = nullptr;
+/* side streams for the compression launches (default priority, like
+ * g_stream); they fork from and join back into g_stream */
+#define BFK_SIDE_STREAMS 3
+static hipStream_t g_side[BFK_SIDE_STREAMS] = {nullptr, nullptr, nullptr};
+static hipEvent_t g_fork = nullptr, g_join[BFK_SIDE_STREAMS] = {nullptr, nullptr, nullptr};
 static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 /* the compression launches return before they finish: their events are
  * read at the next bfk_sync / bfk_get_times */
@@ -1553,6 +1558,11 @@ int bfk_init(void) {
   }
   BFK_CHECK(hipSetDevice(0));
   BFK_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  for (int i = 0; i < BFK_SIDE_STREAMS; ++i) {
+    BFK_CHECK(hipStreamCreateWithFlags(&g_side[i], hipStreamNonBlocking));
+    BFK_CHECK(hipEventCreateWithFlags(&g_join[i], hipEventDisableTiming));
+  }
+  BFK_CHECK(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
   BFK_CHECK(hipEventCreate(&g_ev0));
   BFK_CHECK(hipEventCreate(&g_ev1));
   BFK_CHECK(hipEventCreate(&g_evL0));
@@ -1712,42 +1722,61 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
   if (bfk_init()) return -1;
   if (collect_pending()) return -1;
   BFK_CHECK(hipEventRecord(g_evL0, g_stream));
+  /* A class of a few large blocks runs for milliseconds at a low
+   * occupancy (its latency is one block's), so the classes are spread over
+   * g_stream and the side streams, largest blocks first, and the long tail
+   * overlaps the classes that hold most of the blocks. */
+  BFK_CHECK(hipEventRecord(g_fork, g_stream));
+  hipStream_t st[BFK_SIDE_STREAMS + 1];
+  st[0] = g_stream;
+  for (int i = 0; i < BFK_SIDE_STREAMS; ++i) {
+    st[i + 1] = g_side[i];
+    BFK_CHECK(hipStreamWaitEvent(g_side[i], g_fork, 0));
+  }
+  const int NS = BFK_SIDE_STREAMS + 1;
+  int next = 0;
   int64_t gs = -1, ge = -1;
   size_t gmax = 0;
-  for (int gi = 0; gi < numGroups; ++gi) {
-    int64_t s = groupStart[gi], e = groupStart[gi + 1];
-    if (e <= s) continue;
-    if (groupBytes[gi] > g_ldsCap) {
+  for (int gi = 0; gi < numGroups; ++gi)
+    if (groupStart[gi + 1] > groupStart[gi] && groupBytes[gi] > g_ldsCap) {
       /* groups are sorted by size, so the oversized ones are contiguous */
-      if (gs < 0) gs = s;
-      ge = e;
+      if (gs < 0) gs = groupStart[gi];
+      ge = groupStart[gi + 1];
       if (groupBytes[gi] > gmax) gmax = groupBytes[gi];
-      continue;
     }
+  if (gs >= 0) {
+    /* the scratch slices are reused chunk after chunk in stream order and
+     * released in stream order, so the host does not wait here */
+    hipStream_t sb = st[next++ % NS];
+    const int64_t stride = (int64_t)(gmax / sizeof(double2));
+    const int64_t chunk = 4096;
+    const int64_t nbig = ge - gs;
+    double2 *work = nullptr;
+    BFK_CHECK(hipMallocAsync((void **)&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax, sb));
+    for (int64_t j0 = gs; j0 < ge; j0 += chunk) {
+      int64_t nj = ge - j0 < chunk ? ge - j0 : chunk;
+      hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, sb, dJobs, dOrder + j0,
+                         (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride, rcond);
+      BFK_CHECK(hipGetLastError());
+    }
+    BFK_CHECK(hipFreeAsync(work, sb));
+  }
+  for (int gi = numGroups - 1; gi >= 0; --gi) {
+    const int64_t s = groupStart[gi], e = groupStart[gi + 1];
+    if (e <= s || groupBytes[gi] > g_ldsCap) continue;
+    hipStream_t sl = st[next++ % NS];
     const int64_t maxGrid = 1 << 30;
     for (int64_t j0 = s; j0 < e; j0 += maxGrid) {
       int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
-      hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs,
+      hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), groupBytes[gi], sl, dJobs,
                          dOrder + j0, (const double2 *)dSrcXY, k, (double2 *)dArena, (double2 *)nullptr,
                          (int64_t)0, rcond);
       BFK_CHECK(hipGetLastError());
     }
   }
-  if (gs >= 0) {
-    /* the scratch slices are reused chunk after chunk in stream order and
-     * released in stream order, so the host does not wait here */
-    const int64_t stride = (int64_t)(gmax / sizeof(double2));
-    const int64_t chunk = 4096;
-    const int64_t nbig = ge - gs;
-    double2 *work = nullptr;
-    BFK_CHECK(hipMallocAsync((void **)&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax, g_stream));
-    for (int64_t j0 = gs; j0 < ge; j0 += chunk) {
-      int64_t nj = ge - j0 < chunk ? ge - j0 : chunk;
-      hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, g_stream, dJobs, dOrder + j0,
-                         (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride, rcond);
-      BFK_CHECK(hipGetLastError());
-    }
-    BFK_CHECK(hipFreeAsync(work, g_stream));
+  for (int i = 0; i < BFK_SIDE_STREAMS; ++i) {
+    BFK_CHECK(hipEventRecord(g_join[i], g_side[i]));
+    BFK_CHECK(hipStreamWaitEvent(g_stream, g_join[i], 0));
   }
   BFK_CHECK(hipEventRecord(g_evL1, g_stream));
   g_lPending = 1;
