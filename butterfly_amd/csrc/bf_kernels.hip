@@ -426,6 +426,121 @@ __device__ __forceinline__ double row_norm2(const double2 *__restrict__ x, int n
   return s0 + s1;
 }
 
+/* 1/sqrt(x) for normal x > 0: the hardware estimate and two Newton steps
+ * (y <- y (3/2 - x y^2 / 2)), accurate to an ulp or two */
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+/* One-sided Jacobi on the rows of R (p rows, W columns with the right-hand
+ * part), round-robin pairs, LP lanes per pair (a power of two fixed at
+ * compile time so the strides fold into the LDS offsets).  A rotation of
+ * rows i < j with g = r_i . conj(r_j), a = |r_i|^2, b = |r_j|^2:
+ *   e = g/|g|, zeta = (b - a)/(2|g|), t = sign(zeta)/(|zeta| + sqrt(1 + zeta^2)),
+ *   c = 1/sqrt(1 + t^2), s = c t;  r_i <- c r_i - s e r_j,  r_j <- s r_i + c e r_j. */
+template <int LP>
+__device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int W, int ld) {
+  const int lane = threadIdx.x;
+  const double eps = 2.220446049250313e-16;
+  const int P = p + (p & 1);
+  const int npairs = P / 2;
+  constexpr int pairsPerPass = 64 / LP;
+  const int slot = lane / LP;
+  const int sub = lane - slot * LP;
+  const double thr = 4.0 * eps * sqrt((double)p);
+  const double thr2 = thr * thr;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    int rotated = 0;
+    for (int step = 0; step < P - 1; ++step) {
+      for (int pb = 0; pb < npairs; pb += pairsPerPass) {
+        const int pr = pb + slot;
+        int i = 0, j = 0;
+        bool valid = false;
+        if (pr < npairs) {
+          const int a = rr_row_fast(pr, step, P), b = rr_row_fast(P - 1 - pr, step, P);
+          i = min(a, b);
+          j = max(a, b);
+          valid = j < p;
+        }
+        double al = 0, be = 0, gx = 0, gy = 0;
+        if (valid) {
+          const double2 *ri = M + i * ld + sub, *rj = M + j * ld + sub;
+          const int cnt = (p - sub + LP - 1) / LP; /* columns sub, sub + LP, ... below p */
+          int q = 0;
+          for (; q + 1 < cnt; q += 2) {
+            const double2 x0 = ri[q * LP], y0 = rj[q * LP], x1 = ri[(q + 1) * LP], y1 = rj[(q + 1) * LP];
+            al = fma(x0.x, x0.x, fma(x0.y, x0.y, al));
+            be = fma(y0.x, y0.x, fma(y0.y, y0.y, be));
+            gx = fma(x0.x, y0.x, fma(x0.y, y0.y, gx)); /* x conj(y) */
+            gy = fma(x0.y, y0.x, fma(-x0.x, y0.y, gy));
+            al = fma(x1.x, x1.x, fma(x1.y, x1.y, al));
+            be = fma(y1.x, y1.x, fma(y1.y, y1.y, be));
+            gx = fma(x1.x, y1.x, fma(x1.y, y1.y, gx));
+            gy = fma(x1.y, y1.x, fma(-x1.x, y1.y, gy));
+          }
+          if (q < cnt) {
+            const double2 x0 = ri[q * LP], y0 = rj[q * LP];
+            al = fma(x0.x, x0.x, fma(x0.y, x0.y, al));
+            be = fma(y0.x, y0.x, fma(y0.y, y0.y, be));
+            gx = fma(x0.x, y0.x, fma(x0.y, y0.y, gx));
+            gy = fma(x0.y, y0.x, fma(-x0.x, y0.y, gy));
+          }
+        }
+        al = group_sum(al, LP);
+        be = group_sum(be, LP);
+        gx = group_sum(gx, LP);
+        gy = group_sum(gy, LP);
+        const double ag2 = gx * gx + gy * gy;
+        /* rows below 1e-145 in norm lie far under any cut; they are not rotated */
+        if (valid && ag2 > 1e-290 && ag2 > thr2 * al * be) {
+          rotated = 1;
+          const double rg = rsqrt_nr(ag2);
+          const double ex = gx * rg, ey = gy * rg;
+          const double zeta = (be - al) * (0.5 * rg);
+          const double az = fabs(zeta);
+          double t;
+          if (az > 1e150) {
+            t = 0.5 / zeta;
+          } else {
+            const double q1 = fma(zeta, zeta, 1.0);
+            t = (zeta >= 0 ? 1.0 : -1.0) / (az + q1 * rsqrt_nr(q1));
+          }
+          const double cs = rsqrt_nr(fma(t, t, 1.0)), sn = cs * t;
+          const double sx = sn * ex, sy = sn * ey, cx = cs * ex, cy = cs * ey;
+          double2 *ri = M + i * ld + sub, *rj = M + j * ld + sub;
+          const int cnt = (W - sub + LP - 1) / LP;
+          int q = 0;
+          for (; q + 3 < cnt; q += 4) {
+            double2 x[4], y[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { x[u] = ri[(q + u) * LP]; y[u] = rj[(q + u) * LP]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              ri[(q + u) * LP] = make_double2(fma(cs, x[u].x, -fma(sx, y[u].x, -sy * y[u].y)),
+                                              fma(cs, x[u].y, -fma(sx, y[u].y, sy * y[u].x)));
+              rj[(q + u) * LP] = make_double2(fma(sn, x[u].x, fma(cx, y[u].x, -cy * y[u].y)),
+                                              fma(sn, x[u].y, fma(cx, y[u].y, cy * y[u].x)));
+            }
+          }
+          for (; q < cnt; ++q) {
+            const double2 x0 = ri[q * LP], y0 = rj[q * LP];
+            ri[q * LP] = make_double2(fma(cs, x0.x, -fma(sx, y0.x, -sy * y0.y)),
+                                      fma(cs, x0.y, -fma(sx, y0.y, sy * y0.x)));
+            rj[q * LP] = make_double2(fma(sn, x0.x, fma(cx, y0.x, -cy * y0.y)),
+                                      fma(sn, x0.y, fma(cx, y0.y, cy * y0.x)));
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!__any(rotated)) break;
+  }
+}
+
 __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, int n, int ld, double2 *__restrict__ w,
                                                double2 *__restrict__ v, int *__restrict__ piv, double rcond,
                                                double2 *__restrict__ out) {
@@ -599,90 +714,19 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
   }
 
   /* ---- 2. one-sided Jacobi on the rows of R */
-  const int P = p + (p & 1);
-  const int npairs = P / 2;
-  int Lp = 64;
-  while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
-  const int pairsPerPass = 64 / Lp;
-  const int slot = lane / Lp;
-  const int sub = lane - slot * Lp;
-  const double thr = 4.0 * eps * sqrt((double)p);
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    int rotated = 0;
-    for (int step = 0; step < P - 1; ++step) {
-      for (int pb = 0; pb < npairs; pb += pairsPerPass) {
-        const int pr = pb + slot;
-        int i = 0, j = 0;
-        bool valid = false;
-        if (pr < npairs) {
-          const int a = rr_row_fast(pr, step, P), b = rr_row_fast(P - 1 - pr, step, P);
-          i = min(a, b);
-          j = max(a, b);
-          valid = j < p;
-        }
-        double al = 0, be = 0, gx = 0, gy = 0;
-        if (valid) {
-          const double2 *ri = M + i * ld, *rj = M + j * ld;
-          int c = sub;
-          for (; c + Lp < p; c += 2 * Lp) {
-            const double2 x0 = ri[c], y0 = rj[c], x1 = ri[c + Lp], y1 = rj[c + Lp];
-            al = fma(x0.x, x0.x, fma(x0.y, x0.y, al));
-            be = fma(y0.x, y0.x, fma(y0.y, y0.y, be));
-            gx = fma(x0.x, y0.x, fma(x0.y, y0.y, gx)); /* x conj(y) */
-            gy = fma(x0.y, y0.x, fma(-x0.x, y0.y, gy));
-            al = fma(x1.x, x1.x, fma(x1.y, x1.y, al));
-            be = fma(y1.x, y1.x, fma(y1.y, y1.y, be));
-            gx = fma(x1.x, y1.x, fma(x1.y, y1.y, gx));
-            gy = fma(x1.y, y1.x, fma(-x1.x, y1.y, gy));
-          }
-          if (c < p) {
-            const double2 x0 = ri[c], y0 = rj[c];
-            al = fma(x0.x, x0.x, fma(x0.y, x0.y, al));
-            be = fma(y0.x, y0.x, fma(y0.y, y0.y, be));
-            gx = fma(x0.x, y0.x, fma(x0.y, y0.y, gx));
-            gy = fma(x0.y, y0.x, fma(-x0.x, y0.y, gy));
-          }
-        }
-        al = group_sum(al, Lp);
-        be = group_sum(be, Lp);
-        gx = group_sum(gx, Lp);
-        gy = group_sum(gy, Lp);
-        const double ag2 = gx * gx + gy * gy;
-        if (valid && ag2 > 0 && ag2 > thr * thr * al * be) {
-          rotated = 1;
-          const double ag = sqrt(ag2);
-          const double rg = 1.0 / ag;
-          const double ex = gx * rg, ey = gy * rg;
-          const double zeta = (be - al) * (0.5 * rg);
-          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
-          const double cs = 1.0 / sqrt(fma(t, t, 1.0)), sn = cs * t;
-          const double sx = sn * ex, sy = sn * ey;
-          double2 *ri = M + i * ld, *rj = M + j * ld;
-          int c = sub;
-          for (; c + 3 * Lp < W; c += 4 * Lp) {
-            double2 x[4], y[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) { x[u] = ri[c + u * Lp]; y[u] = rj[c + u * Lp]; }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const double ux = sx * y[u].x - sy * y[u].y, uy = sx * y[u].y + sy * y[u].x; /* s e y */
-              ri[c + u * Lp] = make_double2(cs * x[u].x - ux, cs * x[u].y - uy);
-              rj[c + u * Lp] = make_double2(fma(sn, x[u].x, cs * (ex * y[u].x - ey * y[u].y)),
-                                            fma(sn, x[u].y, cs * (ex * y[u].y + ey * y[u].x)));
-            }
-          }
-          for (; c < W; c += Lp) {
-            const double2 x0 = ri[c], y0 = rj[c];
-            const double ux = sx * y0.x - sy * y0.y, uy = sx * y0.y + sy * y0.x;
-            ri[c] = make_double2(cs * x0.x - ux, cs * x0.y - uy);
-            rj[c] = make_double2(fma(sn, x0.x, cs * (ex * y0.x - ey * y0.y)),
-                                 fma(sn, x0.y, cs * (ex * y0.y + ey * y0.x)));
-          }
-        }
-        __syncthreads();
-      }
+  {
+    const int npairs = (p + (p & 1)) / 2;
+    int Lp = 64;
+    while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
+    switch (Lp) {
+      case 8: jacobi_rows<8>(M, p, W, ld); break;
+      case 4: jacobi_rows<4>(M, p, W, ld); break;
+      case 2: jacobi_rows<2>(M, p, W, ld); break;
+      case 1: jacobi_rows<1>(M, p, W, ld); break;
+      case 16: jacobi_rows<16>(M, p, W, ld); break;
+      case 32: jacobi_rows<32>(M, p, W, ld); break;
+      default: jacobi_rows<64>(M, p, W, ld); break;
     }
-    if (!__any(rotated)) break;
   }
 
   /* ---- 3. truncation and X */
