@@ -1652,10 +1652,23 @@ void bfk_free(void *p) {
   if (p) (void)hipFree(p);
 }
 
-/* released once the work queued before it on the library's stream is done;
- * the host does not wait */
+/* released at the next bfk_sync, after the work queued before it is done;
+ * the host does not wait here (hipFree would wait for the device) */
+#define BFK_MAX_DEFERRED 16
+static void *g_deferred[BFK_MAX_DEFERRED];
+static int g_numDeferred = 0;
 void bfk_free_async(void *p) {
-  if (p && bfk_init() == 0) (void)hipFreeAsync(p, g_stream);
+  if (!p) return;
+  if (g_numDeferred == BFK_MAX_DEFERRED) {
+    (void)hipStreamSynchronize(g_stream);
+    for (int i = 0; i < g_numDeferred; ++i) (void)hipFree(g_deferred[i]);
+    g_numDeferred = 0;
+  }
+  g_deferred[g_numDeferred++] = p;
+}
+static void free_deferred(void) {
+  for (int i = 0; i < g_numDeferred; ++i) (void)hipFree(g_deferred[i]);
+  g_numDeferred = 0;
 }
 
 int bfk_h2d(void *dst, const void *src, size_t bytes) {
@@ -1693,6 +1706,7 @@ static int collect_pending(void) {
 int bfk_sync(void) {
   if (bfk_init()) return -1;
   BFK_CHECK(hipStreamSynchronize(g_stream));
+  free_deferred();
   return collect_pending();
 }
 
@@ -1796,14 +1810,14 @@ int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, co
     const int64_t chunk = 4096;
     const int64_t nbig = ge - gs;
     double2 *work = nullptr;
-    BFK_CHECK(hipMallocAsync((void **)&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax, sb));
+    BFK_CHECK(hipMalloc((void **)&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax));
     for (int64_t j0 = gs; j0 < ge; j0 += chunk) {
       int64_t nj = ge - j0 < chunk ? ge - j0 : chunk;
       hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, sb, dJobs, dOrder + j0,
                          (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride, rcond);
       BFK_CHECK(hipGetLastError());
     }
-    BFK_CHECK(hipFreeAsync(work, sb));
+    bfk_free_async(work);
   }
   for (int gi = numGroups - 1; gi >= 0; --gi) {
     const int64_t s = groupStart[gi], e = groupStart[gi + 1];
