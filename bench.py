@@ -53,6 +53,57 @@ def dense_rows_threaded(k, tgt_rows, src, x, threads):
     return out
 
 
+def gmres_section(bf, n, tol, rank_const, rcond, restart, solve_tol):
+    """Second-kind boundary equation I/2 + (S + C)W on the curve (n points,
+    10 per wavelength, order-6 corrections), right-hand side a plane wave in
+    direction 30 degrees: FP64 GMRES (no restart, `restart` Krylov vectors)
+    against the mixed solver (FP64 residuals around GMRES(restart) with the
+    FP32 apply of the orthogonalized factorization)."""
+    th = 2 * np.pi * np.arange(n) / n
+    r = 1 + 0.3 * np.cos(5 * th)
+    dr = -1.5 * np.sin(5 * th)
+    pts = np.stack([r * np.cos(th), r * np.sin(th)], 1)
+    w = np.sqrt(r * r + dr * dr) * (2 * np.pi / n)
+    k = 2 * np.pi * n / (10 * CURVE_L)
+    t0 = time.perf_counter()
+    tree = bf.Quadtree(pts, 64)
+    helm = bf.Helm2(k)
+    F = bf.FacHelm2.make_multilevel(helm, tree, tree, bf.options(tol=tol, rank_const=rank_const, pass_through=True,
+                                                                 pinv_rcond=rcond))
+    A = bf.BoundaryOperator(F, helm, tree, w, kr_order=6, alpha=0.5)
+    factor_s = time.perf_counter() - t0
+    d = np.array([np.cos(np.pi / 6), np.sin(np.pi / 6)])
+    b = np.exp(1j * k * (pts @ d))[tree.perm]
+    A.solve(b, solve_tol, restart)  # warm-up
+    bf.lib().bfhipSynchronize()
+    t0 = time.perf_counter()
+    x64, it64 = A.solve(b, solve_tol, restart)
+    t64 = time.perf_counter() - t0
+    r64 = float(np.linalg.norm(A @ x64 - b) / np.linalg.norm(b))
+    t0 = time.perf_counter()
+    ortho_s = F.orthogonalize()
+    F.prepare_mixed("fp32")
+    prep_s = time.perf_counter() - t0
+    A.solve_mixed(b, solve_tol, restart=restart)  # warm-up
+    t0 = time.perf_counter()
+    xm, info = A.solve_mixed(b, solve_tol, restart=restart)
+    tm = time.perf_counter() - t0
+    rm = float(np.linalg.norm(A @ xm - b) / np.linalg.norm(b))
+    out = {"workload": f"I/2 + (S + C)W on r=1+0.3cos(5t), N={n}, k={k:.1f} (10 ppw), KR order 6, plane wave at 30 deg, "
+                       f"factor tol {tol:g}, C={rank_const}, rcond {rcond:g}",
+           "N": n, "factor_s": factor_s, "solve_tol": solve_tol, "restart": restart,
+           "fp64_iterations": it64, "fp64_solve_s": t64, "fp64_rel_residual": r64,
+           "mixed_prepare_s": prep_s, "mixed_orthogonalize_s": ortho_s, "mixed_iterations": info["iterations"],
+           "mixed_refinements": info["refinements"], "mixed_solve_s": tm, "mixed_rel_residual": rm,
+           "mixed_vs_fp64_solution": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+           "solve_speedup": t64 / tm}
+    print(f"[bench] gmres N={n}: factor {factor_s:.2f}s; fp64 {it64} it {t64 * 1e3:.1f} ms res {r64:.1e}; mixed "
+          f"{info['iterations']} it / {info['refinements']} refinements {tm * 1e3:.1f} ms res {rm:.1e} "
+          f"(prepare {prep_s:.2f}s)", file=sys.stderr, flush=True)
+    del A, F
+    return out
+
+
 def cpu_baseline(n_sample, tol, rank_const, rcond, n_full_entries):
     """Oracle (single-threaded C restatement) factor + apply on the same
     geometry at n_sample points; the apply rate is scaled to the full size
@@ -103,6 +154,9 @@ def main():
     ap.add_argument("--mixed-rhs", type=int, default=256)
     ap.add_argument("--mixed-steps", type=int, default=5)
     ap.add_argument("--mixed-ortho", type=int, default=1, help="orthogonalize the mixed factorization (1) or not (0)")
+    ap.add_argument("--gmres-n", type=int, default=1 << 18, help="boundary-equation GMRES section size (0 = skip)")
+    ap.add_argument("--gmres-restart", type=int, default=50)
+    ap.add_argument("--gmres-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4096)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -255,6 +309,11 @@ def main():
               file=sys.stderr, flush=True)
         del Fm
 
+    gm = None
+    if args.gmres_n:
+        gm = gmres_section(bf, args.gmres_n, args.tol, args.rank_const, args.pinv_rcond, args.gmres_restart,
+                           args.gmres_tol)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.rank_const, args.pinv_rcond, st["entries"])
@@ -292,6 +351,7 @@ def main():
                      "kernel": "k_apply1 (all stages of one apply)", "bytes_per_apply": bytes_per_apply,
                      "event_ms_per_apply": apply_ms},
         "mixed": mixed,
+        "gmres": gm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

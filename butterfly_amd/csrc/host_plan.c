@@ -408,10 +408,27 @@ static void make_separated(Planner *P, int32_t s, int32_t t) {
         for (int c = 0; c < nk; ++c) BF_VEC_PUSH(Lt[j + 1], kids[c]);
       }
     }
-    BfHipLeaf *l = new_leaf(P->F, P, s, t, 1);
+    BfMatHipButterfly *F = P->F;
+    const size_t nl = F->leaves.n, nfac = F->factors.n, nb = F->blocks.n, no = F->offs.n;
+    const size_t njl = F->lstsqJobs.n, njk = F->kernelJobs.n;
+    const int64_t na = F->arenaEntries;
+    BfHipLeaf *l = new_leaf(F, P, s, t, 1);
     make_butterfly(P, l, Ls, Lt, D, 1);
     free_levels(Ls, D + 1);
     free_levels(Lt, D + 1);
+    /* a butterfly that stores more entries than the block itself (a small
+     * target facing a large source across a narrow gap, where the proxy
+     * count k r1 r2 / d exceeds the points) is replaced by the dense block */
+    if (F->arenaEntries - na > npts(sv, s) * npts(tv, t)) {
+      F->leaves.n = nl;
+      F->factors.n = nfac;
+      F->blocks.n = nb;
+      F->offs.n = no;
+      F->lstsqJobs.n = njl;
+      F->kernelJobs.n = njk;
+      F->arenaEntries = na;
+      make_dense(P, s, t);
+    }
     return;
   }
 

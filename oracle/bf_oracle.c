@@ -909,6 +909,27 @@ static void make_separated(OFac *F, int s, int t) {
   make_butterfly(F, l, Ls, Lt, D, modes);
   for (size_t j = 0; j <= D; ++j) { free(Ls[j].nd); free(Lt[j].nd); }
   free(Ls); free(Lt);
+  if (modes) {
+    /* the pass-through planner keeps a butterfly only if it stores no more
+     * entries than the dense block */
+    size_t stored = 0;
+    for (size_t f = 0; f < l->numFactors; ++f)
+      for (size_t b = 0; b < l->factors[f].numBlocks; ++b)
+        if (l->factors[f].blocks[b].kind != BK_IDENTITY) stored += l->factors[f].blocks[b].m * l->factors[f].blocks[b].n;
+    if (stored > l->m * l->n) {
+      for (size_t f = 0; f < l->numFactors; ++f) {
+        OFactor *fa = &l->factors[f];
+        for (size_t b = 0; b < fa->numBlocks; ++b) free(fa->blocks[b].data);
+        free(fa->blocks); free(fa->rowOffset); free(fa->colOffset);
+      }
+      free(l->factors);
+      l->factors = NULL;
+      l->numFactors = 0;
+      l->isButterfly = 0;
+      OPtSet src = node_points(F, 1, s), tgt = node_points(F, 0, t);
+      l->dense = kernel_matrix(F->k, &src, &tgt);
+    }
+  }
 }
 
 static void multilevel_rec(OFac *F, const int *sNodes, size_t ns, const int *tNodes, size_t nt) {

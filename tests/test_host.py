@@ -75,10 +75,14 @@ def _plan_pair(src, tgt, k, leaf, tol, pt):
     return F, O
 
 
-@pytest.mark.parametrize("case", ["box_ref", "box_pass", "curve_pass", "curve_ref", "curve_pass_c125"])
+@pytest.mark.parametrize("case", ["box_ref", "box_pass", "curve_pass", "curve_ref", "curve_pass_c125", "self_pass_c125"])
 def test_plan_matches_oracle(case):
     if case.startswith("box"):
         src, tgt, k = box_problem(4096)
+    elif case.startswith("self"):
+        # the curve against itself: near pairs across narrow gaps
+        src, _, k = curve_problem(4096)
+        tgt = src
     else:
         src, tgt, k = curve_problem(4096)
     leaf, tol, pt = (1, 1e-15, False) if case.endswith("ref") else (64, 1e-8, True)
@@ -87,6 +91,41 @@ def test_plan_matches_oracle(case):
     F = bf.FacHelm2.plan_only(bf.Helm2(k), ts, tt, bf.options(tol=tol, pass_through=pt, rank_const=C))
     O = orc.Fac(k, src, tgt, leaf, tol, 128 * 128, pt, plan_only=True, rank_const=C)
     for a, b in zip(F.plan(), O.plan()):
+        assert a.shape == b.shape and np.array_equal(a, b)
+
+
+def _stored_per_leaf(lt, ft, bt):
+    """Stored entries of every top-level leaf (identity blocks store none)."""
+    out, f, b = [], 0, 0
+    for row in lt:
+        if not row[0]:
+            out.append(row[3] * row[4])
+            continue
+        st = 0
+        for _ in range(row[5]):
+            nb = ft[f, 3]
+            blk = bt[b:b + nb]
+            st += int((blk[:, 2] * blk[:, 3])[blk[:, 4] != 2].sum())
+            b += nb
+            f += 1
+        out.append(st)
+    return np.array(out)
+
+
+def test_pass_through_keeps_only_compressing_butterflies():
+    """The pass-through planner replaces a butterfly that would store more
+    than its dense block (on the curve against itself a few such pairs
+    occur at N = 16384) by the dense block; no leaf stores more than m n."""
+    src, _, k = curve_problem(16384)
+    t = bf.Quadtree(src, 64)
+    F = bf.FacHelm2.plan_only(bf.Helm2(k), t, t, bf.options(tol=1e-8, pass_through=True, rank_const=1.25))
+    lt, ft, bt = F.plan()
+    st = _stored_per_leaf(lt, ft, bt)
+    assert (st <= lt[:, 3] * lt[:, 4]).all()
+    lsq = bt[bt[:, 4] == 0]
+    assert lsq[:, 2].max() < 128
+    O = orc.Fac(k, src, src, 64, 1e-8, 128 * 128, True, plan_only=True, rank_const=1.25)
+    for a, b in zip((lt, ft, bt), O.plan()):
         assert a.shape == b.shape and np.array_equal(a, b)
 
 
