@@ -104,6 +104,121 @@ def gmres_section(bf, n, tol, rank_const, rcond, restart, solve_tol):
     return out
 
 
+def batch_section(bf, count, nrhs, steps, rows):
+    """Independent problems: sources uniform in [0,1]^2, targets in
+    [2,3]x[0,1] (seeded), N log-uniform in [2^12, 2^18], k for 8 points per
+    wavelength, leaf size from {32, 64, 128}, tol 1e-7; factored one by one
+    and as one block-diagonal operator, then `nrhs` right-hand sides per
+    problem applied in FP64 and (orthogonalized) FP32."""
+    rng = np.random.default_rng(3)
+    probs = []
+    for _ in range(count):
+        n = int(round(2 ** rng.uniform(12, 18)))
+        leaf = int(rng.choice([32, 64, 128]))
+        src = rng.uniform(0, 1, (n, 2))
+        tgt = rng.uniform(0, 1, (n, 2))
+        tgt[:, 0] += 2
+        probs.append((src, tgt, 2 * np.pi * np.sqrt(n) / 8, leaf))
+    trees = [(bf.Quadtree(s_, leaf), bf.Quadtree(t_, leaf)) for s_, t_, _, leaf in probs]
+    helms = [bf.Helm2(k) for _, _, k, _ in probs]
+    opts = [bf.options(tol=1e-7, rank_const=1.25, pass_through=True, pinv_rcond=1e-11) for _ in probs]
+    nsrc = sum(len(p_[0]) for p_ in probs)
+    ntgt = sum(len(p_[1]) for p_ in probs)
+    X = rng.standard_normal((nsrc, nrhs)) + 1j * rng.standard_normal((nsrc, nrhs))
+    # one at a time
+    bf.lib().bfhipSynchronize()
+    t0 = time.perf_counter()
+    singles = [bf.FacHelm2.make_multilevel(h, ts, tt, o) for h, (ts, tt), o in zip(helms, trees, opts)]
+    seq_factor = time.perf_counter() - t0
+    xs, c0 = [], 0
+    for s_, _, _, _ in probs:
+        xs.append(bf.DeviceArray.from_numpy(np.ascontiguousarray(X[c0:c0 + len(s_)])))
+        c0 += len(s_)
+    ys = [bf.DeviceArray(16 * len(t_) * nrhs) for _, t_, _, _ in probs]
+    for F, dx, dy in zip(singles, xs, ys):
+        bf.mul_device(F, dx, dy, nrhs, "fp64")
+    bf.lib().bfhipSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for F, dx, dy in zip(singles, xs, ys):
+            bf.mul_device(F, dx, dy, nrhs, "fp64")
+    bf.lib().bfhipSynchronize()
+    seq_apply = (time.perf_counter() - t0) / steps
+    Yseq = np.concatenate([dy.to_numpy(np.complex128, (len(t_), nrhs)) for dy, (_, t_, _, _) in zip(ys, probs)])
+    err_rows = None
+    if rows:
+        import oracle as orc
+        errs = []
+        for i in range(0, count, max(1, count // 4)):
+            s_, t_, k, _ = probs[i]
+            ts, tt = trees[i]
+            r = np.sort(rng.choice(len(t_), size=min(rows, len(t_)), replace=False))
+            x = np.empty(len(s_), complex)
+            c0 = sum(len(p_[0]) for p_ in probs[:i])
+            x[ts.perm] = X[c0:c0 + len(s_), 0]
+            yd = orc.dense_rows(k, t_[r], s_, x)
+            y = np.empty(len(t_), complex)
+            r0 = sum(len(p_[1]) for p_ in probs[:i])
+            y[tt.perm] = Yseq[r0:r0 + len(t_), 0]
+            errs.append(float(np.linalg.norm(y[r] - yd) / np.linalg.norm(yd)))
+        err_rows = max(errs)
+    print(f"[bench] batch {count} problems ({nsrc} points): one by one factor {seq_factor:.2f}s, "
+          f"{nrhs}-RHS fp64 apply {seq_apply * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    del singles, xs, ys
+    # all at once
+    t0 = time.perf_counter()
+    B = bf.FacHelm2.make_batched(helms, [t[0] for t in trees], [t[1] for t in trees], opts)
+    batch_factor = time.perf_counter() - t0
+    bst = B.stats()
+    print(f"[bench] batch factor breakdown: plan {bst['plan_s']:.2f}s, kernels+apply plan {bst['kernels_s']:.2f}s "
+          f"(compress {bst['compress_ms']:.0f} ms, sample {bst['sample_ms']:.0f} ms), apply plan "
+          f"{bst['apply_plan_s']:.2f}s", file=sys.stderr, flush=True)
+    dX = bf.DeviceArray.from_numpy(X)
+    dY = bf.DeviceArray(16 * ntgt * nrhs)
+    bf.mul_device(B, dX, dY, nrhs, "fp64")
+    bf.lib().bfhipSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bf.mul_device(B, dX, dY, nrhs, "fp64")
+    bf.lib().bfhipSynchronize()
+    batch_apply = (time.perf_counter() - t0) / steps
+    Y64 = dY.to_numpy(np.complex128, (ntgt, nrhs))
+    t0 = time.perf_counter()
+    B.orthogonalize()
+    B.prepare_mixed("fp32")
+    mixed_prep = time.perf_counter() - t0
+    dX32 = bf.DeviceArray.from_numpy(X.astype(np.complex64))
+    dY32 = bf.DeviceArray(8 * ntgt * nrhs)
+    bf.mul_device(B, dX32, dY32, nrhs, "fp32")
+    bf.lib().bfhipSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bf.mul_device(B, dX32, dY32, nrhs, "fp32")
+    bf.lib().bfhipSynchronize()
+    batch_apply32 = (time.perf_counter() - t0) / steps
+    Y32 = dY32.to_numpy(np.complex64, (ntgt, nrhs)).astype(complex)
+    out = {"workload": f"{count} independent problems: sources U[0,1]^2 -> targets U[2,3]x[0,1], N log-uniform in "
+                       f"[2^12, 2^18] (total {nsrc}), k = 2 pi sqrt(N)/8 (8 ppw), leaf from {{32,64,128}}, tol 1e-7, "
+                       f"C=1.25, rcond 1e-11, {nrhs} RHS each",
+           "problems": count, "total_points": nsrc, "entries": B.stats()["entries"],
+           "factor_one_by_one_s": seq_factor, "factor_batched_s": batch_factor,
+           "factor_batched_breakdown": {"plan_s": bst["plan_s"], "kernels_and_apply_plan_s": bst["kernels_s"],
+                                        "compress_ms": bst["compress_ms"], "sample_ms": bst["sample_ms"],
+                                        "apply_plan_s": bst["apply_plan_s"]},
+           "factor_speedup": seq_factor / batch_factor,
+           "apply_fp64_one_by_one_ms": seq_apply * 1e3, "apply_fp64_batched_ms": batch_apply * 1e3,
+           "apply_fp32_batched_ms": batch_apply32 * 1e3, "mixed_prepare_s": mixed_prep,
+           "batched_vs_one_by_one": float(np.linalg.norm(Y64 - Yseq) / np.linalg.norm(Yseq)),
+           "fp32_rel_err_vs_fp64": float(np.linalg.norm(Y32 - Y64) / np.linalg.norm(Y64)),
+           "fp64_rel_err_vs_dense_rows_max": err_rows}
+    print(f"[bench] batch {count} problems ({nsrc} points): factor one by one {seq_factor:.2f}s, batched "
+          f"{batch_factor:.2f}s; {nrhs}-RHS apply fp64 one by one {seq_apply * 1e3:.1f} ms, batched "
+          f"{batch_apply * 1e3:.1f} ms, fp32 {batch_apply32 * 1e3:.1f} ms (err {out['fp32_rel_err_vs_fp64']:.1e}); "
+          f"fp64 vs dense rows {err_rows}", file=sys.stderr, flush=True)
+    del B
+    return out
+
+
 def cpu_baseline(n_sample, tol, rank_const, rcond, n_full_entries):
     """Oracle (single-threaded C restatement) factor + apply on the same
     geometry at n_sample points; the apply rate is scaled to the full size
@@ -157,6 +272,8 @@ def main():
     ap.add_argument("--gmres-n", type=int, default=1 << 18, help="boundary-equation GMRES section size (0 = skip)")
     ap.add_argument("--gmres-restart", type=int, default=50)
     ap.add_argument("--gmres-tol", type=float, default=1e-10)
+    ap.add_argument("--batch", type=int, default=64, help="independent-problems section size (0 = skip)")
+    ap.add_argument("--batch-rhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4096)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -314,6 +431,10 @@ def main():
         gm = gmres_section(bf, args.gmres_n, args.tol, args.rank_const, args.pinv_rcond, args.gmres_restart,
                            args.gmres_tol)
 
+    bt = None
+    if args.batch:
+        bt = batch_section(bf, args.batch, args.batch_rhs, max(1, args.mixed_steps), 256 if args.rows else 0)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.rank_const, args.pinv_rcond, st["entries"])
@@ -352,6 +473,7 @@ def main():
                      "event_ms_per_apply": apply_ms},
         "mixed": mixed,
         "gmres": gm,
+        "batch": bt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -102,6 +102,7 @@ def lib():
         "bfFacHelm2MakeMultilevel": (vp, [vp, vp, vp]),
         "bfFacHelm2MakeMultilevelWithOptions": (vp, [vp, vp, vp, vp]),
         "bfhipFacHelm2Plan": (vp, [vp, vp, vp, vp]),
+        "bfhipFacHelm2MakeBatched": (vp, [sz, vp, C.POINTER(vp), C.POINTER(vp), vp]),
         "bfMatGetType": (C.c_int, [vp]),
         "bfMatGetNumRows": (sz, [vp]),
         "bfMatGetNumCols": (sz, [vp]),
@@ -399,6 +400,23 @@ class FacHelm2:
             p = L.bfFacHelm2MakeMultilevelWithOptions(C.byref(helm.s), src_tree.ptr, tgt_tree.ptr, C.byref(opts))
         check()
         return Factorization(p, keep=(src_tree, tgt_tree))
+
+    @staticmethod
+    def make_batched(helms, src_trees, tgt_trees, opts_list):
+        """bfhipFacHelm2MakeBatched: independent problems as one
+        block-diagonal operator (inputs and outputs stacked in order)."""
+        n = len(helms)
+        if not (len(src_trees) == len(tgt_trees) == len(opts_list) == n):
+            raise ValueError("one helm, source tree, target tree and options per problem")
+        H = (Helm2Struct * n)(*[h.s for h in helms])
+        S = (C.c_void_p * n)(*[t.ptr for t in src_trees])
+        T = (C.c_void_p * n)(*[t.ptr for t in tgt_trees])
+        O = (FacOptionsStruct * n)(*opts_list)
+        p = lib().bfhipFacHelm2MakeBatched(n, C.cast(H, C.c_void_p), S, T, C.cast(O, C.c_void_p))
+        check()
+        if not p:
+            raise BfError("batched factorization failed")
+        return Factorization(p, keep=(tuple(src_trees), tuple(tgt_trees)))
 
     @staticmethod
     def plan_only(helm, src_tree, tgt_tree, opts=None):

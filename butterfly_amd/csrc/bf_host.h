@@ -17,6 +17,7 @@ void *bfk_malloc(size_t bytes);
 void bfk_free(void *p);
 void bfk_free_async(void *p);
 int bfk_h2d(void *dst, const void *src, size_t bytes);
+int bfk_h2d_side(void *dst, const void *src, size_t bytes);
 int bfk_d2h(void *dst, const void *src, size_t bytes);
 int bfk_memset(void *dst, size_t bytes);
 int bfk_sync(void);
@@ -27,8 +28,10 @@ int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles);
 int bfk_set_compensation(int on);
 int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
                double *dArena);
-int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
-              const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena);
+int bfk_lstsq_begin(size_t maxOversizeBytes, int64_t numOversize);
+int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t numJobs, size_t ldsBytes,
+                     const double *dSrcXY, double *dArena);
+int bfk_lstsq_end(void);
 int bfk_pair_green(int64_t n, const double *dA, const double *dB, double k, double *dOut);
 int bfk_scale_rows(int64_t n, int nrhs, const double *dW, const void *dX, void *dU, int fp32);
 int bfk_bie_post(int64_t n, int nrhs, int ncorr, double alpha, const int32_t *dCol, const double *dC,
@@ -144,6 +147,9 @@ typedef struct BfMatHipButterfly {
 
 /* planner (plan.c) */
 int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt);
+/* plans one more problem straight into F as its next diagonal block */
+int bfPlanBuildAt(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt, int64_t srcPts,
+                  int64_t tgtPts);
 /* apply plan (fac.c) */
 int bfApplyPlanBuild(BfMatHipButterfly *F);
 /* apply plan and uploads for a plan + arena already in place (fac.c) */

@@ -41,6 +41,9 @@ static hipStream_t g_stream = nullptr;
 #define BFK_SIDE_STREAMS 3
 static hipStream_t g_side[BFK_SIDE_STREAMS] = {nullptr, nullptr, nullptr};
 static hipEvent_t g_fork = nullptr, g_join[BFK_SIDE_STREAMS] = {nullptr, nullptr, nullptr};
+static int g_nextStream = 0;
+/* uploads that must not queue behind the kernels of g_stream */
+static hipStream_t g_copy = nullptr;
 static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 /* the compression launches return before they finish: their events are
  * read at the next bfk_sync / bfk_get_times */
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *_
     for (int64_t e = threadIdx.x; e < cnt; e += SAMPLE_THREADS) {
       int j = (int)(e / m);
       int i = (int)(e - (int64_t)j * m);
-      out[e] = green(k, tgtXY[J.tgtOff + i], spts[j]);
+      out[e] = green(J.k, tgtXY[J.tgtOff + i], spts[j]);
     }
   }
 }
@@ -802,7 +805,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_l
                                              double2 *__restrict__ arena, double2 *__restrict__ gwork,
                                              int64_t gstride, double rcond) {
   extern __shared__ double2 lds[];
-  const BfkLstsqJob J = jobs[order[blockIdx.x]];
+  const BfkLstsqJob J = jobs[order ? order[blockIdx.x] : blockIdx.x];
   const int p = J.p, n = J.n, ld = p + n + 1;
   LstsqView L = lstsq_view(GLOBAL ? gwork + (int64_t)blockIdx.x * gstride : lds, p, n);
   double2 *tp = L.pts, *ep = tp + p, *op = ep + p;
@@ -819,13 +822,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_l
     int i = e / W;
     int c = e - i * W;
     double2 s = c < p ? ep[c] : op[c - p];
-    L.M[i * ld + c] = green_call(k, tp[i], s);
+    L.M[i * ld + c] = green_call(J.k, tp[i], s);
   }
   __syncthreads();
   if (T == 64 && !GLOBAL)
-    qr_jacobi_wave(L.M, p, n, ld, L.w, L.v, L.piv, rcond, arena + J.out);
+    qr_jacobi_wave(L.M, p, n, ld, L.w, L.v, L.piv, J.rcond, arena + J.out);
   else
-    qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, rcond, arena + J.out);
+    qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, J.rcond, arena + J.out);
 }
 
 /* the same solve on matrices given in global memory (row-major A p x p,
@@ -1607,6 +1610,7 @@ int bfk_init(void) {
     BFK_CHECK(hipEventCreateWithFlags(&g_join[i], hipEventDisableTiming));
   }
   BFK_CHECK(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
+  BFK_CHECK(hipStreamCreateWithFlags(&g_copy, hipStreamNonBlocking));
   BFK_CHECK(hipEventCreate(&g_ev0));
   BFK_CHECK(hipEventCreate(&g_ev1));
   BFK_CHECK(hipEventCreate(&g_evL0));
@@ -1671,10 +1675,25 @@ static void free_deferred(void) {
   g_numDeferred = 0;
 }
 
+/* pageable uploads in pieces of 256 MiB: one multi-GB pageable copy ran at
+ * a quarter of the rate of the same bytes in pieces */
 int bfk_h2d(void *dst, const void *src, size_t bytes) {
   if (bfk_init()) return -1;
-  BFK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  const size_t piece = (size_t)256 << 20;
+  for (size_t off = 0; off < bytes; off += piece) {
+    const size_t n = bytes - off < piece ? bytes - off : piece;
+    BFK_CHECK(hipMemcpyAsync((char *)dst + off, (const char *)src + off, n, hipMemcpyHostToDevice, g_stream));
+  }
   BFK_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+/* host -> device on the copy stream: waits for this copy only, not for
+ * kernels queued on the compute streams */
+int bfk_h2d_side(void *dst, const void *src, size_t bytes) {
+  if (bfk_init()) return -1;
+  BFK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_copy));
+  BFK_CHECK(hipStreamSynchronize(g_copy));
   return 0;
 }
 
@@ -1770,68 +1789,73 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
   return time_end(0);
 }
 
-/* re-expansion blocks; `dOrder` lists job indices grouped by LDS need,
- * `groupStart/groupBytes` give [numGroups] contiguous ranges of dOrder
- * and the bytes each job of the range needs.  Groups that fit in LDS run
- * one wave per job; the rest run 256 threads per job on global scratch,
- * all of them in one launch per chunk of 4096 jobs. */
-int bfk_lstsq(const BfkLstsqJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
-              const size_t *groupBytes, const double *dSrcXY, double k, double rcond, double *dArena) {
+/* Re-expansion blocks: bfk_lstsq_begin, then any number of
+ * bfk_lstsq_launch calls (each over consecutive jobs of one LDS class,
+ * already in device memory), then bfk_lstsq_end.  The launches return at
+ * once; the time from begin to end is read at the next bfk_sync. */
+/* scratch for the jobs above the LDS capacity: allocated once per
+ * compression (hipMalloc between launches would stall them), reused chunk
+ * after chunk by launches that all go to g_stream */
+static double2 *g_lstsqWork = nullptr;
+static size_t g_lstsqWorkStride = 0; /* bytes per job slot */
+#define BFK_SCRATCH_CHUNK 4096
+
+int bfk_lstsq_begin(size_t maxOversizeBytes, int64_t numOversize) {
   if (bfk_init()) return -1;
   if (collect_pending()) return -1;
+  g_lstsqWork = nullptr;
+  g_lstsqWorkStride = 0;
+  if (numOversize > 0 && maxOversizeBytes > 0) {
+    const int64_t slots = numOversize < BFK_SCRATCH_CHUNK ? numOversize : BFK_SCRATCH_CHUNK;
+    BFK_CHECK(hipMalloc((void **)&g_lstsqWork, (size_t)slots * maxOversizeBytes));
+    g_lstsqWorkStride = maxOversizeBytes;
+  }
   BFK_CHECK(hipEventRecord(g_evL0, g_stream));
-  /* A class of a few large blocks runs for milliseconds at a low
-   * occupancy (its latency is one block's), so the classes are spread over
-   * g_stream and the side streams, largest blocks first, and the long tail
-   * overlaps the classes that hold most of the blocks. */
+  /* the side streams start after everything queued so far on g_stream */
   BFK_CHECK(hipEventRecord(g_fork, g_stream));
-  hipStream_t st[BFK_SIDE_STREAMS + 1];
-  st[0] = g_stream;
-  for (int i = 0; i < BFK_SIDE_STREAMS; ++i) {
-    st[i + 1] = g_side[i];
-    BFK_CHECK(hipStreamWaitEvent(g_side[i], g_fork, 0));
-  }
-  const int NS = BFK_SIDE_STREAMS + 1;
-  int next = 0;
-  int64_t gs = -1, ge = -1;
-  size_t gmax = 0;
-  for (int gi = 0; gi < numGroups; ++gi)
-    if (groupStart[gi + 1] > groupStart[gi] && groupBytes[gi] > g_ldsCap) {
-      /* groups are sorted by size, so the oversized ones are contiguous */
-      if (gs < 0) gs = groupStart[gi];
-      ge = groupStart[gi + 1];
-      if (groupBytes[gi] > gmax) gmax = groupBytes[gi];
-    }
-  if (gs >= 0) {
-    /* the scratch slices are reused chunk after chunk in stream order and
-     * released in stream order, so the host does not wait here */
-    hipStream_t sb = st[next++ % NS];
-    const int64_t stride = (int64_t)(gmax / sizeof(double2));
-    const int64_t chunk = 4096;
-    const int64_t nbig = ge - gs;
-    double2 *work = nullptr;
-    BFK_CHECK(hipMalloc((void **)&work, (size_t)(nbig < chunk ? nbig : chunk) * gmax));
-    for (int64_t j0 = gs; j0 < ge; j0 += chunk) {
-      int64_t nj = ge - j0 < chunk ? ge - j0 : chunk;
-      hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, sb, dJobs, dOrder + j0,
-                         (const double2 *)dSrcXY, k, (double2 *)dArena, work, stride, rcond);
+  for (int i = 0; i < BFK_SIDE_STREAMS; ++i) BFK_CHECK(hipStreamWaitEvent(g_side[i], g_fork, 0));
+  g_nextStream = 0;
+  return 0;
+}
+
+/* One launch over `numJobs` jobs of one LDS class (ldsBytes per job):
+ * dOrder[0..numJobs) indexes dJobs, or with dOrder NULL the jobs are
+ * consecutive.  Launches rotate over g_stream and the side streams: a class of a
+ * few large blocks runs for milliseconds at one block's latency and low
+ * occupancy, and overlaps the classes that hold most of the blocks.  Jobs
+ * above the LDS capacity run 256 threads each on the scratch of
+ * bfk_lstsq_begin, chunk after chunk on g_stream. */
+int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t numJobs, size_t ldsBytes,
+                     const double *dSrcXY, double *dArena) {
+  if (numJobs <= 0) return 0;
+  if (ldsBytes > g_ldsCap) {
+    if (!g_lstsqWork || ldsBytes > g_lstsqWorkStride) return -3; /* bfk_lstsq_begin sized the scratch */
+    const int64_t stride = (int64_t)(g_lstsqWorkStride / sizeof(double2));
+    for (int64_t j0 = 0; j0 < numJobs; j0 += BFK_SCRATCH_CHUNK) {
+      int64_t nj = numJobs - j0 < BFK_SCRATCH_CHUNK ? numJobs - j0 : BFK_SCRATCH_CHUNK;
+      hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, g_stream,
+                         dOrder ? dJobs : dJobs + j0, dOrder ? dOrder + j0 : (const int32_t *)nullptr,
+                         (const double2 *)dSrcXY, 0.0, (double2 *)dArena, g_lstsqWork, stride, 0.0);
       BFK_CHECK(hipGetLastError());
     }
-    bfk_free_async(work);
+    return 0;
   }
-  for (int gi = numGroups - 1; gi >= 0; --gi) {
-    const int64_t s = groupStart[gi], e = groupStart[gi + 1];
-    if (e <= s || groupBytes[gi] > g_ldsCap) continue;
-    hipStream_t sl = st[next++ % NS];
-    const int64_t maxGrid = 1 << 30;
-    for (int64_t j0 = s; j0 < e; j0 += maxGrid) {
-      int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
-      hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), groupBytes[gi], sl, dJobs,
-                         dOrder + j0, (const double2 *)dSrcXY, k, (double2 *)dArena, (double2 *)nullptr,
-                         (int64_t)0, rcond);
-      BFK_CHECK(hipGetLastError());
-    }
+  hipStream_t st = g_nextStream == 0 ? g_stream : g_side[g_nextStream - 1];
+  g_nextStream = (g_nextStream + 1) % (BFK_SIDE_STREAMS + 1);
+  const int64_t maxGrid = 1 << 30;
+  for (int64_t j0 = 0; j0 < numJobs; j0 += maxGrid) {
+    int64_t nj = numJobs - j0 < maxGrid ? numJobs - j0 : maxGrid;
+    hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), ldsBytes, st, dOrder ? dJobs : dJobs + j0,
+                       dOrder ? dOrder + j0 : (const int32_t *)nullptr, (const double2 *)dSrcXY, 0.0,
+                       (double2 *)dArena, (double2 *)nullptr, (int64_t)0, 0.0);
+    BFK_CHECK(hipGetLastError());
   }
+  return 0;
+}
+
+int bfk_lstsq_end(void) {
+  bfk_free_async(g_lstsqWork);
+  g_lstsqWork = nullptr;
   for (int i = 0; i < BFK_SIDE_STREAMS; ++i) {
     BFK_CHECK(hipEventRecord(g_join[i], g_side[i]));
     BFK_CHECK(hipStreamWaitEvent(g_stream, g_join[i], 0));

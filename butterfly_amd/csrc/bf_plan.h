@@ -16,11 +16,14 @@ extern "C" {
  * test: p points on circle (tcx, tcy, tr); equiv: p points on circle
  * (ecx, ecy, er); orig: n points on circle (ocx, ocy, orr) when
  * origOff < 0, else the source points [origOff, origOff + n) in
- * source-tree order.  X is written column-major at `out`. */
+ * source-tree order.  X is written column-major at `out`.  Each job
+ * carries its wavenumber k and pseudo-inverse cut rcond, so the jobs of
+ * independent problems can share a launch. */
 typedef struct BfkLstsqJob {
   double ecx, ecy, er;
   double tcx, tcy, tr;
   double ocx, ocy, orr;
+  double k, rcond;
   int64_t origOff;
   int64_t out;
   int32_t p, n;
@@ -32,6 +35,7 @@ typedef struct BfkLstsqJob {
  * points [srcOff, srcOff + n).  Written column-major at `out`. */
 typedef struct BfkKernelJob {
   double scx, scy, sr;
+  double k;
   int64_t srcOff;
   int64_t tgtOff;
   int64_t out;

@@ -7,6 +7,8 @@
 #define _DEFAULT_SOURCE 1
 #include <math.h>
 
+#include <pthread.h>
+
 #include "bf_host.h"
 
 static BfType hipbf_get_type(BfMat const *m) {
@@ -341,37 +343,97 @@ typedef BF_VEC(size_t) BfSizeVec;
  * order[] lists the jobs class by class, in plan order inside a class;
  * gs gets each non-empty class's first position (plus a final nj) and gb
  * its byte size. */
-static void group_by_class(const uint32_t *cls, size_t nj, int32_t *order, BfI64Vec *gs, BfSizeVec *gb) {
+static void group_by_class(const uint32_t *cls, size_t nj, int32_t *order, BfI64Vec *gs, BfSizeVec *gb,
+                           int descending) {
   uint32_t maxCls = 0;
   for (size_t i = 0; i < nj; ++i)
     if (cls[i] > maxCls) maxCls = cls[i];
+  /* key: the class, or its mirror for descending order */
   size_t *start = calloc((size_t)maxCls + 2, sizeof(size_t));
-  for (size_t i = 0; i < nj; ++i) ++start[cls[i] + 1];
+#define KEY(c) (descending ? maxCls - (c) : (c))
+  for (size_t i = 0; i < nj; ++i) ++start[KEY(cls[i]) + 1];
   for (size_t c = 0; c <= maxCls; ++c) start[c + 1] += start[c];
   gs->n = gb->n = 0;
-  for (size_t c = 0; c <= maxCls; ++c)
-    if (start[c + 1] > start[c]) {
-      BF_VEC_PUSH(*gs, (int64_t)start[c]);
-      BF_VEC_PUSH(*gb, c << 12);
+  for (size_t q = 0; q <= maxCls; ++q)
+    if (start[q + 1] > start[q]) {
+      BF_VEC_PUSH(*gs, (int64_t)start[q]);
+      BF_VEC_PUSH(*gb, (size_t)KEY(q) << 12);
     }
   BF_VEC_PUSH(*gs, (int64_t)nj);
-  for (size_t i = 0; i < nj; ++i) order[start[cls[i]]++] = (int32_t)i;
+  for (size_t i = 0; i < nj; ++i) order[start[KEY(cls[i])]++] = (int32_t)i;
+#undef KEY
   free(start);
 }
 
-static int run_factor_kernels(BfMatHipButterfly *F) {
+/* the apply plan, built on a helper thread while the main thread uploads
+ * and launches the compression */
+typedef struct {
+  BfMatHipButterfly *F;
   int rc;
+  double seconds;
+} ApplyPlanTask;
+
+static void *apply_plan_task(void *arg) {
+  ApplyPlanTask *t = arg;
+  double t0 = bfWallTime();
+  t->rc = bfApplyPlanBuild(t->F) ? -1 : 0;
+  t->seconds = bfWallTime() - t0;
+  return NULL;
+}
+
+typedef struct {
+  const BfkLstsqJob *src;
+  const int32_t *order;
+  BfkLstsqJob *dst;
+  size_t i0, i1;
+} GatherTask;
+
+static void *gather_task(void *arg) {
+  GatherTask *g = arg;
+  for (size_t i = g->i0; i < g->i1; ++i) g->dst[i] = g->src[g->order[i]];
+  return NULL;
+}
+
+/* dst[i] = src[order[i]] on up to 16 threads */
+static void gather_jobs(const BfkLstsqJob *src, const int32_t *order, BfkLstsqJob *dst, size_t n) {
+  enum { NT = 16 };
+  GatherTask t[NT];
+  pthread_t th[NT];
+  int ok[NT] = {0};
+  const int nt = n < ((size_t)1 << 16) ? 1 : NT;
+  for (int q = 0; q < nt; ++q) {
+    t[q].src = src;
+    t[q].order = order;
+    t[q].dst = dst;
+    t[q].i0 = n * (size_t)q / (size_t)nt;
+    t[q].i1 = n * (size_t)(q + 1) / (size_t)nt;
+  }
+  for (int q = 1; q < nt; ++q) {
+    ok[q] = pthread_create(&th[q], NULL, gather_task, &t[q]) == 0;
+    if (!ok[q]) gather_task(&t[q]);
+  }
+  gather_task(&t[0]);
+  for (int q = 1; q < nt; ++q)
+    if (ok[q]) pthread_join(th[q], NULL);
+}
+
+/* Sampling, then the compression: the re-expansion jobs are grouped by
+ * the LDS they need (4 KiB classes, launched largest first over four
+ * streams) and uploaded in chunks that are launched as they arrive; the
+ * apply plan is built on a helper thread meanwhile.  Returns with the
+ * kernels running (bfk_sync waits). */
+static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
+  int rc;
+  *tApplyPlan = 0;
   size_t arenaBytes = (size_t)F->arenaEntries * 16;
   F->dArena = bfk_malloc(arenaBytes);
   if (!F->dArena) return -2;
-  void *dK = NULL, *dL = NULL, *dOrder = NULL;
+  void *dK = NULL;
   if ((rc = upload(&dK, F->kernelJobs.a, F->kernelJobs.n * sizeof(BfkKernelJob)))) return rc;
   rc = bfk_sample(dK, (int64_t)F->kernelJobs.n, F->dSrcXY, F->dTgtXY, F->helm.k, F->dArena);
   bfk_free(dK);
   if (rc) return rc;
-  /* re-expansion jobs grouped by the LDS they need (sizes rounded up to
-   * 4 KiB so a launch serves many jobs) */
-  size_t nj = F->lstsqJobs.n;
+  const size_t nj = F->lstsqJobs.n;
   uint32_t *cls = malloc((nj ? nj : 1) * sizeof(uint32_t));
   for (size_t i = 0; i < nj; ++i)
     cls[i] = (uint32_t)((bfk_lstsq_lds_bytes(F->lstsqJobs.a[i].p, F->lstsqJobs.a[i].n) + 4095) >> 12);
@@ -380,60 +442,69 @@ static int run_factor_kernels(BfMatHipButterfly *F) {
   BfSizeVec gb;
   memset(&gs, 0, sizeof(gs));
   memset(&gb, 0, sizeof(gb));
-  group_by_class(cls, nj, order, &gs, &gb);
+  group_by_class(cls, nj, order, &gs, &gb, 1);
   free(cls);
-  rc = upload(&dL, F->lstsqJobs.a, nj * sizeof(BfkLstsqJob));
-  if (!rc) rc = upload(&dOrder, order, nj * sizeof(int32_t));
-  if (!rc && nj)
-    rc = bfk_lstsq(dL, dOrder, (int)gb.n, gs.a, gb.a, F->dSrcXY, F->helm.k, F->opts.pinvRcond, F->dArena);
-  /* the launches are still running: release their tables in stream order */
-  bfk_free_async(dL);
-  bfk_free_async(dOrder);
+  /* the apply plan is host work: build it on a helper thread while this
+   * thread uploads the jobs and the kernels run */
+  ApplyPlanTask task = {F, 0, 0};
+  pthread_t th;
+  const int threaded = pthread_create(&th, NULL, apply_plan_task, &task) == 0;
+  if (!threaded) apply_plan_task(&task);
+  /* the jobs in class order, uploaded in chunks on the copy stream; each
+   * uploaded range is launched at once, so the upload (several GB for
+   * large or batched problems) overlaps the kernels */
+  BfkLstsqJob *sorted = malloc((nj ? nj : 1) * sizeof(BfkLstsqJob));
+  gather_jobs(F->lstsqJobs.a, order, sorted, nj);
   free(order);
+  BfkLstsqJob *dL = bfk_malloc((nj ? nj : 1) * sizeof(BfkLstsqJob));
+  if (!dL || !sorted) rc = -2;
+  /* jobs above the LDS capacity share one scratch buffer */
+  size_t bigBytes = 0;
+  int64_t numBig = 0;
+  for (size_t g = 0; g + 1 < gs.n; ++g)
+    if (gb.a[g] > bfk_lds_cap()) {
+      if (gb.a[g] > bigBytes) bigBytes = gb.a[g];
+      numBig += gs.a[g + 1] - gs.a[g];
+    }
+  if (!rc) rc = bfk_lstsq_begin(bigBytes, numBig);
+  const size_t chunk = (size_t)1 << 22; /* jobs per upload */
+  size_t up = 0, launched = 0, g = 0;
+  while (!rc && up < nj) {
+    const size_t n = nj - up < chunk ? nj - up : chunk;
+    rc = bfk_h2d_side(dL + up, sorted + up, n * sizeof(BfkLstsqJob));
+    up += n;
+    /* what is uploaded: the rest of the current class, the classes that
+     * are complete, and the uploaded start of the next */
+    while (!rc && g + 1 < gs.n && launched < up) {
+      const size_t end = (size_t)gs.a[g + 1] < up ? (size_t)gs.a[g + 1] : up;
+      rc = bfk_lstsq_launch(dL + launched, NULL, (int64_t)(end - launched), gb.a[g], F->dSrcXY, F->dArena);
+      launched = end;
+      if (launched == (size_t)gs.a[g + 1]) ++g;
+    }
+  }
+  if (!rc) rc = bfk_lstsq_end();
+  free(sorted);
+  if (threaded) pthread_join(th, NULL);
+  *tApplyPlan = task.seconds;
+  if (!rc && task.rc) rc = task.rc;
+  /* the launches are still running: release their tables after the next sync */
+  bfk_free_async(dL);
   BF_VEC_FREE(gs);
   BF_VEC_FREE(gb);
   return rc;
 }
 
-BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
-                                           BfFacHelm2Options const *opts) {
-  if (!helm || !srcTree || !tgtTree || !opts || !(helm->k > 0) || !srcTree->nodes || !tgtTree->nodes) {
-    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
-    return NULL;
-  }
-  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE) {
-    bfSetError(BF_ERROR_NOT_IMPLEMENTED);
-    return NULL;
-  }
-  if (bfk_init()) {
-    bfSetError(BF_ERROR_RUNTIME_ERROR);
-    return NULL;
-  }
-  double t0 = bfWallTime();
-  BfMatHipButterfly *F = calloc(1, sizeof(BfMatHipButterfly));
-  if (!F) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
-  F->helm = *helm;
-  F->opts = *opts;
-  F->numSrc = srcTree->points->size;
-  F->numTgt = tgtTree->points->size;
-  bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
-  if (bfPlanBuild(F, srcTree, tgtTree)) {
-    bfMatHipButterflyDelete((BfMat **)&F);
-    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
-    return NULL;
-  }
-  double t1 = bfWallTime();
-  int rc = 0;
-  double *sxy = tree_coords(srcTree), *txy = tree_coords(tgtTree);
-  rc = upload((void **)&F->dSrcXY, sxy, 2 * F->numSrc * sizeof(double));
+/* Everything after planning: coordinates up, sampling and compression
+ * launched, the apply plan built on the host meanwhile, then the apply
+ * tables up and the statistics.  Takes ownership of sxy / txy (tree-ordered
+ * coordinates); on failure deletes F and returns NULL. */
+static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, double t0, double t1) {
+  int rc = upload((void **)&F->dSrcXY, sxy, 2 * F->numSrc * sizeof(double));
   if (!rc) rc = upload((void **)&F->dTgtXY, txy, 2 * F->numTgt * sizeof(double));
   free(sxy);
   free(txy);
-  if (!rc) rc = run_factor_kernels(F);
-  /* the apply plan is host work: build it while the compression runs */
-  double ta = bfWallTime();
-  if (!rc) rc = bfApplyPlanBuild(F) ? -1 : 0;
-  double tApplyPlan = bfWallTime() - ta;
+  double tApplyPlan = 0;
+  if (!rc) rc = run_factor_kernels(F, &tApplyPlan);
   if (!rc) rc = bfk_sync();
   double t2 = bfWallTime();
   if (!rc) rc = upload((void **)&F->dSegs, F->segs.a, F->segs.n * sizeof(BfkSeg));
@@ -476,6 +547,98 @@ BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const
   F->stats[14] = ktimes[1];
   F->stats[15] = tApplyPlan + (t3 - t2);
   return &F->super;
+}
+
+static int problem_ok(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
+                      BfFacHelm2Options const *opts) {
+  if (!helm || !srcTree || !tgtTree || !opts || !(helm->k > 0) || !srcTree->nodes || !tgtTree->nodes) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return 0;
+  }
+  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE) {
+    bfSetError(BF_ERROR_NOT_IMPLEMENTED);
+    return 0;
+  }
+  return 1;
+}
+
+BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
+                                           BfFacHelm2Options const *opts) {
+  if (!problem_ok(helm, srcTree, tgtTree, opts)) return NULL;
+  if (bfk_init()) {
+    bfSetError(BF_ERROR_RUNTIME_ERROR);
+    return NULL;
+  }
+  double t0 = bfWallTime();
+  BfMatHipButterfly *F = calloc(1, sizeof(BfMatHipButterfly));
+  if (!F) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  F->helm = *helm;
+  F->opts = *opts;
+  F->numSrc = srcTree->points->size;
+  F->numTgt = tgtTree->points->size;
+  bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
+  if (bfPlanBuild(F, srcTree, tgtTree)) {
+    bfMatHipButterflyDelete((BfMat **)&F);
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  double t1 = bfWallTime();
+  return factor_from_plan(F, tree_coords(srcTree), tree_coords(tgtTree), t0, t1);
+}
+
+BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree const *const *srcTrees,
+                                BfQuadtree const *const *tgtTrees, BfFacHelm2Options const *opts) {
+  if (count == 0 || !helms || !srcTrees || !tgtTrees || !opts) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  for (BfSize i = 0; i < count; ++i)
+    if (!problem_ok(&helms[i], srcTrees[i], tgtTrees[i], &opts[i])) return NULL;
+  if (bfk_init()) {
+    bfSetError(BF_ERROR_RUNTIME_ERROR);
+    return NULL;
+  }
+  double t0 = bfWallTime();
+  BfMatHipButterfly *F = calloc(1, sizeof(BfMatHipButterfly));
+  if (!F) { bfSetError(BF_ERROR_MEMORY_ERROR); return NULL; }
+  F->helm = helms[0];
+  F->opts = opts[0];
+  for (BfSize i = 0; i < count; ++i) {
+    F->numSrc += srcTrees[i]->points->size;
+    F->numTgt += tgtTrees[i]->points->size;
+  }
+  bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
+  double *sxy = malloc(2 * F->numSrc * sizeof(double)), *txy = malloc(2 * F->numTgt * sizeof(double));
+  int64_t sBase = 0, tBase = 0;
+  int bad = !sxy || !txy;
+  for (BfSize i = 0; i < count && !bad; ++i) {
+    /* the planner reads the problem's wavenumber and options from F */
+    F->helm = helms[i];
+    F->opts = opts[i];
+    if (bfPlanBuildAt(F, srcTrees[i], tgtTrees[i], sBase, tBase)) {
+      bad = 1;
+    } else {
+      const BfSize ns = srcTrees[i]->points->size, nt = tgtTrees[i]->points->size;
+      double *a = tree_coords(srcTrees[i]), *b = tree_coords(tgtTrees[i]);
+      memcpy(sxy + 2 * sBase, a, 2 * ns * sizeof(double));
+      memcpy(txy + 2 * tBase, b, 2 * nt * sizeof(double));
+      free(a);
+      free(b);
+      sBase += (int64_t)ns;
+      tBase += (int64_t)nt;
+    }
+  }
+  F->helm = helms[0];
+  F->opts = opts[0];
+  if (bad) {
+    free(sxy);
+    free(txy);
+    bfMatHipButterflyDelete((BfMat **)&F);
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  double t1 = bfWallTime();
+  return factor_from_plan(F, sxy, txy, t0, t1);
 }
 
 BfMat *bfhipFacHelm2Plan(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
@@ -768,7 +931,7 @@ int bfhipFacOrthogonalize(BfMat *fac) {
       uint32_t c = (uint32_t)((bfk_ortho_units(jobs[g].a[i].rows, jobs[g].a[i].n) * 16 + 4095) >> 12);
       cls[i] = c > capCls ? capCls : c;
     }
-    group_by_class(cls, nj, order, &gs, &gb);
+    group_by_class(cls, nj, order, &gs, &gb, 0);
     rc = bfk_h2d(dJobs, jobs[g].a, nj * sizeof(BfkOrthoJob));
     if (!rc) rc = bfk_h2d(dOrder, order, nj * sizeof(int32_t));
     if (!rc) rc = bfk_ortho(dJobs, dOrder, (int)gb.n, gs.a, gb.a, dItems, F->dArena, dR[(g + 1) % 2], dR[g % 2]);

@@ -208,6 +208,7 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
   BfSize np = (m + panel - 1) / panel;
   BfkKernelJob *jobs = calloc(np ? np : 1, sizeof(BfkKernelJob));
   for (BfSize p = 0; p < np; ++p) {
+    jobs[p].k = helm->k;
     jobs[p].srcOff = 0;
     jobs[p].tgtOff = (int64_t)(p * panel);
     jobs[p].m = (int32_t)((p + 1) * panel <= m ? panel : m - p * panel);

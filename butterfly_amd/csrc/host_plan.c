@@ -100,6 +100,7 @@ static void push_kernel_job(Planner *P, BfHipBlock *b, const TreeView *sv, int32
   J.scx = sv->cx[s];
   J.scy = sv->cy[s];
   J.sr = sv->cr[s];
+  J.k = P->k;
   J.srcOff = srcIsPoints ? (int64_t)node(sv, s)->offset[0] : -1;
   J.tgtOff = (int64_t)node(&P->tgt, t)->offset[0];
   J.m = b->m;
@@ -120,6 +121,8 @@ static void push_lstsq_job(Planner *P, BfHipBlock *b, int32_t e, int32_t o, int 
   J.ecx = P->src.cx[e]; J.ecy = P->src.cy[e]; J.er = P->src.cr[e];
   J.tcx = P->tgt.cx[tt]; J.tcy = P->tgt.cy[tt]; J.tr = P->tgt.cr[tt];
   J.ocx = P->src.cx[o]; J.ocy = P->src.cy[o]; J.orr = P->src.cr[o];
+  J.k = P->k;
+  J.rcond = F->opts.pinvRcond;
   J.origOff = origIsPoints ? (int64_t)node(&P->src, o)->offset[0] : -1;
   J.p = b->m;
   J.n = b->n;
@@ -510,6 +513,7 @@ typedef struct {
   PlanChunk *chunks;
   size_t numChunks;
   BfMatHipButterfly *F;
+  int64_t srcPts, tgtPts; /* point offsets of this problem in F's stacked points */
   atomic_size_t next;
   int phase; /* 0 build, 1 merge */
 } PlanPool;
@@ -525,12 +529,17 @@ static void build_chunk(PlanPool *pool, PlanChunk *c) {
 
 #define COPY_TABLE(dst, src, base) memcpy((dst).a + (base), (src).a, (src).n * sizeof(*(src).a))
 
-static void merge_chunk(PlanPool *pool, PlanChunk *c) {
-  BfMatHipButterfly *F = pool->F, *T = &c->T;
+/* copies T's tables into F at the given bases (F's tables already sized),
+ * shifting indices and arena offsets, and point offsets by the source /
+ * target point bases; then frees T's tables */
+static void copy_tables(BfMatHipButterfly *F, BfMatHipButterfly *T, const PlanChunk *c, int64_t srcPts,
+                        int64_t tgtPts) {
   COPY_TABLE(F->leaves, T->leaves, c->leafBase);
   for (size_t i = 0; i < T->leaves.n; ++i) {
     BfHipLeaf *l = &F->leaves.a[c->leafBase + (int64_t)i];
     l->firstFactor += c->factorBase;
+    l->row0 += tgtPts;
+    l->col0 += srcPts;
     if (l->denseOff >= 0) l->denseOff += c->arenaBase;
   }
   COPY_TABLE(F->factors, T->factors, c->factorBase);
@@ -548,9 +557,18 @@ static void merge_chunk(PlanPool *pool, PlanChunk *c) {
   }
   COPY_TABLE(F->offs, T->offs, c->offsBase);
   COPY_TABLE(F->lstsqJobs, T->lstsqJobs, c->lBase);
-  for (size_t i = 0; i < T->lstsqJobs.n; ++i) F->lstsqJobs.a[c->lBase + (int64_t)i].out += c->arenaBase;
+  for (size_t i = 0; i < T->lstsqJobs.n; ++i) {
+    BfkLstsqJob *J = &F->lstsqJobs.a[c->lBase + (int64_t)i];
+    J->out += c->arenaBase;
+    if (J->origOff >= 0) J->origOff += srcPts;
+  }
   COPY_TABLE(F->kernelJobs, T->kernelJobs, c->kBase);
-  for (size_t i = 0; i < T->kernelJobs.n; ++i) F->kernelJobs.a[c->kBase + (int64_t)i].out += c->arenaBase;
+  for (size_t i = 0; i < T->kernelJobs.n; ++i) {
+    BfkKernelJob *J = &F->kernelJobs.a[c->kBase + (int64_t)i];
+    J->out += c->arenaBase;
+    if (J->srcOff >= 0) J->srcOff += srcPts;
+    J->tgtOff += tgtPts;
+  }
   BF_VEC_FREE(T->leaves);
   BF_VEC_FREE(T->factors);
   BF_VEC_FREE(T->blocks);
@@ -558,6 +576,8 @@ static void merge_chunk(PlanPool *pool, PlanChunk *c) {
   BF_VEC_FREE(T->lstsqJobs);
   BF_VEC_FREE(T->kernelJobs);
 }
+
+static void merge_chunk(PlanPool *pool, PlanChunk *c) { copy_tables(pool->F, &c->T, c, pool->srcPts, pool->tgtPts); }
 
 static void *plan_worker(void *arg) {
   PlanPool *pool = arg;
@@ -583,13 +603,16 @@ static void run_pool(PlanPool *pool, int phase, int nthreads) {
   for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
 }
 
+/* size a table to `total` entries, growing its capacity geometrically so
+ * that repeated appends copy each entry a bounded number of times */
 #define RESERVE(v, total)                                              \
   do {                                                                 \
     if ((v).cap < (size_t)(total)) {                                   \
-      void *na_ = realloc((v).a, ((size_t)(total) ? (size_t)(total) : 1) * sizeof(*(v).a)); \
+      size_t nc_ = 2 * (v).cap > (size_t)(total) ? 2 * (v).cap : (size_t)(total); \
+      void *na_ = realloc((v).a, (nc_ ? nc_ : 1) * sizeof(*(v).a));    \
       if (!na_) abort();                                               \
       (v).a = na_;                                                     \
-      (v).cap = (size_t)(total);                                       \
+      (v).cap = nc_;                                                   \
     }                                                                  \
     (v).n = (size_t)(total);                                           \
   } while (0)
@@ -602,6 +625,14 @@ static int plan_threads(void) {
 }
 
 int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt) {
+  return bfPlanBuildAt(F, src, tgt, 0, 0);
+}
+
+/* Plans one problem (F->helm, F->opts) and appends it to F's tables as the
+ * next diagonal block: leaf rows / columns and point offsets shifted by
+ * tgtPts / srcPts, indices and arena offsets after what F already holds. */
+int bfPlanBuildAt(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt, int64_t srcPts,
+                  int64_t tgtPts) {
   Planner P;
   memset(&P, 0, sizeof(P));
   P.F = F;
@@ -654,6 +685,8 @@ int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *t
   pool.chunks = chunks;
   pool.numChunks = nc;
   pool.F = F;
+  pool.srcPts = srcPts;
+  pool.tgtPts = tgtPts;
   run_pool(&pool, 0, nthreads);
   int64_t nl = (int64_t)F->leaves.n, nf = (int64_t)F->factors.n, nb = (int64_t)F->blocks.n;
   int64_t no = (int64_t)F->offs.n, njl = (int64_t)F->lstsqJobs.n, njk = (int64_t)F->kernelJobs.n;

@@ -274,6 +274,17 @@ BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const
 BfMat *bfhipFacHelm2Plan(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
                          BfFacHelm2Options const *opts);
 
+/* new (batched independent problems): factor `count` problems -- problem i
+ * is helms[i] on srcTrees[i] -> tgtTrees[i] with opts[i], each planned as by
+ * bfFacHelm2MakeMultilevelWithOptions -- as one block-diagonal operator
+ * diag(A_0, ..., A_{count-1}).  The compression jobs of all problems share
+ * one launch per LDS class and every apply stage is one launch over all
+ * problems.  X stacks the problems' inputs in order, each in its own
+ * source-tree order; Y likewise in target-tree order.  The result is a
+ * BF_TYPE_MAT_HIP_BUTTERFLY (bfMatMul, bfhipMatMulDevice, FP32 forms). */
+BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree const *const *srcTrees,
+                                BfQuadtree const *const *tgtTrees, BfFacHelm2Options const *opts);
+
 /* mat.c dispatchers (mat.h:45-110) */
 BfType bfMatGetType(BfMat const *mat);
 BfSize bfMatGetNumRows(BfMat const *mat);
