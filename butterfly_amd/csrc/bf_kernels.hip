@@ -1249,16 +1249,6 @@ template <> struct BufsOf<float2> { typedef BfkBufsF T; };
 
 /* compensated (Kahan) accumulation: s + (-c) carries the running sum to
  * about one rounding of the final value however long the chain */
-__device__ __forceinline__ void kahan_add(float &s, float &c, float p) {
-  const float y = p - c;
-  const float t = s + y;
-  c = (t - s) - y;
-  s = t;
-}
-__device__ __forceinline__ void kahan_addc(float2 &s, float2 &c, float2 p) {
-  kahan_add(s.x, c.x, p.x);
-  kahan_add(s.y, c.y, p.y);
-}
 /* add an FP32 accumulator tile into an FP64 running sum and clear it */
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
 template <class A> __device__ __forceinline__ void fold64(mf_d4 &s, A &p) {
@@ -1289,8 +1279,8 @@ __constant__ int g_foldMaskDev = 0;
  * R lanes of a group read R consecutive entries), then the G partial sums
  * meet in a shuffle reduction and the row is written once.  With LO the
  * factors are an FP32 pair hi + lo.  With COMP (FP32 only) every lane
- * sums its products with Kahan compensation, so the rounding of a stage no
- * longer grows with the length of its rows. */
+ * sums its FP32 products in FP64 and the lanes of a row meet in FP64, so
+ * the rounding of a stage no longer grows with the length of its rows. */
 template <typename V, bool LO, bool UNROLL, bool COMP = false>
 __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
                                                 const BfkTerm *__restrict__ terms, const V *__restrict__ hi,
@@ -1306,6 +1296,63 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
   const int r = lane & (R - 1);
   const int g = lane / R;
   V *y = bufs.out[S.ybuf] + S.yOff;
+  if constexpr (COMP) {
+    /* FP32 products summed in FP64 (two chains), reduced across the
+     * column groups in FP64, rounded to FP32 once */
+    for (int rb = 0; rb < m; rb += R) {
+      const int i = rb + r;
+      double sx0 = 0, sy0 = 0, sx1 = 0, sy1 = 0;
+      if (i < m) {
+        for (int t = S.t0; t < S.t1; ++t) {
+          const BfkTerm T = terms[t];
+          const V *x = bufs.in[T.xbuf] + T.xOff;
+          if (T.aOff < 0) {
+            if (g == 0) {
+              sx0 += (double)x[i].x;
+              sy0 += (double)x[i].y;
+            }
+            continue;
+          }
+          const int64_t lda = T.lda;
+          const V *A = hi + T.aOff + T.rowOff + i;
+          const V *B = lo + T.aOff + T.rowOff + i;
+          const int n = T.n;
+          int j = g;
+          for (; j + G < n; j += 2 * G) {
+            const V x0 = x[j], x1 = x[j + G];
+            V p0 = cmulf(A[(int64_t)j * lda], x0), p1 = cmulf(A[(int64_t)(j + G) * lda], x1);
+            if (LO) {
+              p0 = cfmaf(B[(int64_t)j * lda], x0, p0);
+              p1 = cfmaf(B[(int64_t)(j + G) * lda], x1, p1);
+            }
+            sx0 += (double)p0.x;
+            sy0 += (double)p0.y;
+            sx1 += (double)p1.x;
+            sy1 += (double)p1.y;
+          }
+          if (j < n) {
+            const V x0 = x[j];
+            V p0 = cmulf(A[(int64_t)j * lda], x0);
+            if (LO) p0 = cfmaf(B[(int64_t)j * lda], x0, p0);
+            sx0 += (double)p0.x;
+            sy0 += (double)p0.y;
+          }
+        }
+      }
+      double ax = sx0 + sx1, ay = sy0 + sy1;
+      for (int o = R; o < 64; o <<= 1) {
+        ax += __shfl_xor(ax, o, 64);
+        ay += __shfl_xor(ay, o, 64);
+      }
+      if (g == 0 && i < m) {
+        V out;
+        out.x = (decltype(out.x))ax;
+        out.y = (decltype(out.y))ay;
+        y[i] = out;
+      }
+    }
+    return;
+  }
   for (int rb = 0; rb < m; rb += R) {
     const int i = rb + r;
     V a0 = czero<V>(), a1 = czero<V>(), a2 = czero<V>(), a3 = czero<V>();
@@ -1313,21 +1360,6 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
       for (int t = S.t0; t < S.t1; ++t) {
         const BfkTerm T = terms[t];
         const V *x = bufs.in[T.xbuf] + T.xOff;
-        if constexpr (COMP) {
-          /* a0 = running sum, a1 = compensation */
-          if (T.aOff < 0) {
-            if (g == 0) kahan_addc(a0, a1, x[i]);
-            continue;
-          }
-          const int64_t lda = T.lda;
-          const V *A = hi + T.aOff + T.rowOff + i;
-          for (int j = g; j < T.n; j += G) {
-            V p = cmulf(A[(int64_t)j * lda], x[j]);
-            if (LO) p = cfmaf(lo[T.aOff + T.rowOff + i + (int64_t)j * lda], x[j], p);
-            kahan_addc(a0, a1, p);
-          }
-          continue;
-        }
         if (T.aOff < 0) {
           if (g == 0) a0 = cadd(a0, x[i]);
           continue;
@@ -1359,13 +1391,7 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
         }
       }
     }
-    V acc;
-    if constexpr (COMP) {
-      acc.x = a0.x - a1.x;
-      acc.y = a0.y - a1.y;
-    } else {
-      acc = cadd(cadd(a0, a1), cadd(a2, a3));
-    }
+    V acc = cadd(cadd(a0, a1), cadd(a2, a3));
     for (int o = R; o < 64; o <<= 1) {
       acc.x += __shfl_xor(acc.x, o, 64);
       acc.y += __shfl_xor(acc.y, o, 64);
