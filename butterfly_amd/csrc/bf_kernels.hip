@@ -2023,17 +2023,29 @@ int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, co
               const size_t *groupBytes, const BfkOrthoItem *dItems, double *dArena, const double *dRin,
               double *dRout) {
   if (bfk_init()) return -1;
-  for (int gi = 0; gi < numGroups; ++gi) {
+  for (int gi = 0; gi < numGroups; ++gi)
+    if (groupStart[gi + 1] > groupStart[gi] && groupBytes[gi] > g_ldsCap) return -3;
+  /* the classes of one stage are independent: largest first, spread over
+   * g_stream and the side streams, joined back into g_stream */
+  BFK_CHECK(hipEventRecord(g_fork, g_stream));
+  for (int i = 0; i < BFK_SIDE_STREAMS; ++i) BFK_CHECK(hipStreamWaitEvent(g_side[i], g_fork, 0));
+  int next = 0;
+  for (int gi = numGroups - 1; gi >= 0; --gi) {
     int64_t s = groupStart[gi], e = groupStart[gi + 1];
     if (e <= s) continue;
-    if (groupBytes[gi] > g_ldsCap) return -3;
+    hipStream_t st = next == 0 ? g_stream : g_side[next - 1];
+    next = (next + 1) % (BFK_SIDE_STREAMS + 1);
     const int64_t maxGrid = 1 << 30;
     for (int64_t j0 = s; j0 < e; j0 += maxGrid) {
       int64_t nj = e - j0 < maxGrid ? e - j0 : maxGrid;
-      hipLaunchKernelGGL(k_ortho, dim3((unsigned)nj), dim3(64), groupBytes[gi], g_stream, dJobs, dOrder + j0, dItems,
+      hipLaunchKernelGGL(k_ortho, dim3((unsigned)nj), dim3(64), groupBytes[gi], st, dJobs, dOrder + j0, dItems,
                          (double2 *)dArena, (const double2 *)dRin, (double2 *)dRout);
       BFK_CHECK(hipGetLastError());
     }
+  }
+  for (int i = 0; i < BFK_SIDE_STREAMS; ++i) {
+    BFK_CHECK(hipEventRecord(g_join[i], g_side[i]));
+    BFK_CHECK(hipStreamWaitEvent(g_stream, g_join[i], 0));
   }
   return 0;
 }
