@@ -384,9 +384,10 @@ void bfhipResetKernelTimes(void);
  * tiles of 16 per wave on the matrix cores in {2, 4, 8}, or 0 for the
  * vector-unit kernel */
 int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
-/* new: compensated (Kahan) accumulation in the FP32 applies, on (1, the
- * default) or off (0): the FP32 rounding of a stage then stays near one
- * unit in the last place however long its rows are */
+/* new: compensated accumulation in the FP32 applies, on (1, the default)
+ * or off (0): FP32 products summed in FP64 (1 RHS) or FP32 matrix-core
+ * tiles folded into FP64 sums (many RHS), so the FP32 rounding of a stage
+ * stays near one unit in the last place however long its rows are */
 int bfhipSetApplyCompensation(int on);
 
 /* new: plan export for parity checks (integers, in the oracle's canonical
