@@ -467,6 +467,7 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
       numBig += gs.a[g + 1] - gs.a[g];
     }
   if (!rc) rc = bfk_lstsq_begin(bigBytes, numBig);
+  const int begun = rc == 0;
   const size_t chunk = (size_t)1 << 22; /* jobs per upload */
   size_t up = 0, launched = 0, g = 0;
   while (!rc && up < nj) {
@@ -482,7 +483,12 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
       if (launched == (size_t)gs.a[g + 1]) ++g;
     }
   }
-  if (!rc) rc = bfk_lstsq_end();
+  /* close the launch sequence even after a failed launch, so the side
+   * streams are joined and the scratch is released */
+  if (begun) {
+    const int rc2 = bfk_lstsq_end();
+    if (!rc) rc = rc2;
+  }
   free(sorted);
   if (threaded) pthread_join(th, NULL);
   *tApplyPlan = task.seconds;
