@@ -17,6 +17,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbutterfly_amd.so")
 
 BF_LAYER_POTENTIAL_SINGLE = 1
+BF_LAYER_POTENTIAL_PV_DOUBLE = 2
+BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_SINGLE = 3
+BF_LAYER_POTENTIAL_COMBINED_FIELD = 5
 PRECISION = {"fp64": 0, "fp32": 1, "fp32x2": 2}
 
 ERROR_NAMES = {
@@ -36,6 +39,11 @@ class Helm2Struct(C.Structure):
 
 
 class Points2Struct(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_size_t),
+                ("capacity", C.c_size_t), ("isView", C.c_bool)]
+
+
+class Vectors2Struct(C.Structure):
     _fields_ = [("data", C.c_void_p), ("size", C.c_size_t),
                 ("capacity", C.c_size_t), ("isView", C.c_bool)]
 
@@ -207,23 +215,33 @@ def sample_box(n, x0, x1, y0, y1):
 
 
 class Helm2:
-    """Kernel parameters (helm2.h:10-15); single-layer potential."""
+    """Kernel parameters (helm2.h:10-15): the single layer, the double layer
+    (BF_LAYER_POTENTIAL_PV_DOUBLE) or the combined field alpha S + beta D
+    (BF_LAYER_POTENTIAL_COMBINED_FIELD)."""
 
-    def __init__(self, k, layer_pot=BF_LAYER_POTENTIAL_SINGLE):
-        self.s = Helm2Struct(float(k), layer_pot, (0.0, 0.0), (0.0, 0.0))
+    def __init__(self, k, layer_pot=None, alpha=1.0, beta=1.0):
+        if layer_pot is None:
+            layer_pot = BF_LAYER_POTENTIAL_SINGLE
+        a, b = complex(alpha), complex(beta)
+        self.s = Helm2Struct(float(k), layer_pot, (a.real, a.imag), (b.real, b.imag))
 
     @property
     def k(self):
         return self.s.k
 
-    def get_kernel_matrix(self, src, tgt):
-        """bfHelm2GetKernelMatrix: dense (tgt x src) kernel, sampled on the GPU."""
+    def get_kernel_matrix(self, src, tgt, src_normals=None):
+        """bfHelm2GetKernelMatrix: dense (tgt x src) kernel, sampled on the GPU
+        (source unit normals for the double layer and the combined field)."""
         src = np.ascontiguousarray(src, dtype=np.float64)
         tgt = np.ascontiguousarray(tgt, dtype=np.float64)
         L = lib()
         ps = L.bfPoints2NewViewFromPtr(_dp(src), len(src))
         pt = L.bfPoints2NewViewFromPtr(_dp(tgt), len(tgt))
-        m = L.bfHelm2GetKernelMatrix(C.byref(self.s), ps, pt, None, None)
+        nv = None
+        if src_normals is not None:
+            nrm = np.ascontiguousarray(src_normals, dtype=np.float64)
+            nv = Vectors2Struct(nrm.ctypes.data, len(nrm), len(nrm), True)
+        m = L.bfHelm2GetKernelMatrix(C.byref(self.s), ps, pt, C.byref(nv) if nv is not None else None, None)
         L.bfPoints2DeinitAndDealloc(C.byref(C.c_void_p(ps)))
         L.bfPoints2DeinitAndDealloc(C.byref(C.c_void_p(pt)))
         check()
@@ -233,12 +251,21 @@ class Helm2:
 class Quadtree:
     """bfQuadtreeInit over an (N, 2) array; keeps the array alive."""
 
-    def __init__(self, points, leaf_size=1):
+    def __init__(self, points, leaf_size=1, normals=None):
+        """`normals`: unit normals of the points (original order), needed by
+        the double layer and the combined field."""
         self.points = np.ascontiguousarray(points, dtype=np.float64)
         L = lib()
         self._pts = L.bfPoints2NewViewFromPtr(_dp(self.points), len(self.points))
+        self._nrm = None
+        if normals is not None:
+            self.normals = np.ascontiguousarray(normals, dtype=np.float64)
+            if self.normals.shape != self.points.shape:
+                raise ValueError("one unit normal per point")
+            self._nrm = Vectors2Struct(self.normals.ctypes.data, len(self.normals), len(self.normals), True)
         self.ptr = L.bfQuadtreeNew()
-        L.bfQuadtreeInitWithLeafSize(self.ptr, self._pts, None, leaf_size)
+        L.bfQuadtreeInitWithLeafSize(self.ptr, self._pts, C.byref(self._nrm) if self._nrm is not None else None,
+                                     leaf_size)
         check()
 
     @property

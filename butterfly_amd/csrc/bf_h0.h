@@ -1,6 +1,7 @@
 /*
- * Hankel function H0^(1)(x) = J0(x) + i Y0(x) in FP64 for x > 0, and the
- * 2-D Helmholtz single-layer kernel G = (i/4) H0^(1)(k r).
+ * Hankel functions H0^(1)(x) = J0(x) + i Y0(x) and H1^(1)(x) = J1 + i Y1
+ * in FP64 for x > 0, the 2-D Helmholtz single-layer kernel
+ * G = (i/4) H0^(1)(k r) and the double-layer kernel (i k / 4) H1(k r) n.(t-s)/r.
  *
  * Written for the device (it is also compiled for the host so the CPU
  * tests can check it against mpmath and glibc).  The reference takes H0
@@ -100,6 +101,79 @@ BF_HD static inline void bf_h0(double x, double *jr, double *yr) {
   double amp = sqrt(BF_TWO_OVER_PI / x);
   *jr = amp * (P * c - Q * s);
   *yr = amp * (P * s + Q * c);
+}
+
+/* ---- H1^(1)(x) = J1 + i Y1 for the double layer, same three branches */
+BF_HD static inline void bf_h1_series(double x, double *jr, double *yr) {
+  double z = 0.25 * x * x;
+  double j = bf_h1_ser_j[BF_H0_NSER - 1];
+  double s = bf_h1_ser_y[BF_H0_NSER - 1];
+  for (int k = BF_H0_NSER - 2; k >= 0; --k) {
+    j = fma(j, z, bf_h1_ser_j[k]);
+    s = fma(s, z, bf_h1_ser_y[k]);
+  }
+  const double hx = 0.5 * x;
+  const double j1 = hx * j;
+  *jr = j1;
+  *yr = BF_TWO_OVER_PI * (log(hx) * j1 - 1.0 / x) + hx * s;
+}
+
+BF_HD static inline void bf_h1_pq(double x, double *P, double *Q) {
+  if (x < BF_H0_X_ASYM) {
+    double y = 1.0 / x;
+    double t = (2.0 * y - (BF_H0_CHEB_YA + BF_H0_CHEB_YB)) / (BF_H0_CHEB_YB - BF_H0_CHEB_YA);
+    double t2 = 2.0 * t;
+    double bp1 = 0, bp2 = 0, bq1 = 0, bq2 = 0;
+    for (int k = BF_H1_NCHEB - 1; k >= 1; --k) {
+      double bp = fma(t2, bp1, bf_h1_cheb_p[k] - bp2);
+      double bq = fma(t2, bq1, bf_h1_cheb_q[k] - bq2);
+      bp2 = bp1; bp1 = bp;
+      bq2 = bq1; bq1 = bq;
+    }
+    *P = fma(t, bp1, bf_h1_cheb_p[0] - bp2);
+    *Q = fma(t, bq1, bf_h1_cheb_q[0] - bq2);
+  } else {
+    double w = 1.0 / x;
+    double w2 = w * w;
+    double p = bf_h1_asym_p[BF_H0_NASYM - 1];
+    double q = bf_h1_asym_q[BF_H0_NASYM - 1];
+    for (int k = BF_H0_NASYM - 2; k >= 0; --k) {
+      p = fma(p, w2, bf_h1_asym_p[k]);
+      q = fma(q, w2, bf_h1_asym_q[k]);
+    }
+    *P = p;
+    *Q = q * w;
+  }
+}
+
+/* H1^(1)(x) = J1 + i Y1, x > 0; above X_SMALL with the phase
+ * x - 3 pi/4 = (x - pi/4) - pi/2: cos = sin(x - pi/4), sin = -cos(x - pi/4) */
+BF_HD static inline void bf_h1(double x, double *jr, double *yr) {
+  if (x <= BF_H0_X_SMALL) {
+    bf_h1_series(x, jr, yr);
+    return;
+  }
+  double P, Q, c, s;
+  bf_h1_pq(x, &P, &Q);
+  bf_phase(x, &c, &s);
+  double amp = sqrt(BF_TWO_OVER_PI / x);
+  *jr = amp * (P * s + Q * c);
+  *yr = amp * (Q * s - P * c);
+}
+
+/* double layer with the source normal n: (i/4) k H1(k r) (n . (t - s)) / r,
+ * 0 for r < 1e-15 (helm2.c:51-58); `dot` = n . (t - s) */
+BF_HD static inline void bf_helm2_double(double k, double r, double dot, double *gr, double *gi) {
+  if (r < 1e-15) {
+    *gr = 0;
+    *gi = 0;
+    return;
+  }
+  double j, y;
+  bf_h1(k * r, &j, &y);
+  const double sc = 0.25 * k * dot / r;
+  *gr = -sc * y;
+  *gi = sc * j;
 }
 
 /* G = (i/4) H0(k r) with G = 0 at r == 0 (helm2.c:114) */

@@ -26,11 +26,11 @@ void bfk_get_time_sums(double sums[8], double counts[8]);
 void bfk_reset_times(void);
 int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles);
 int bfk_set_compensation(int on);
-int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
-               double *dArena);
+int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY,
+               const double *dSrcN, BfkLayer layer, double *dArena);
 int bfk_lstsq_begin(size_t maxOversizeBytes, int64_t numOversize);
 int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t numJobs, size_t ldsBytes,
-                     const double *dSrcXY, double *dArena);
+                     const double *dSrcXY, const double *dSrcN, BfkLayer layer, double *dArena);
 int bfk_lstsq_end(void);
 int bfk_pair_green(int64_t n, const double *dA, const double *dB, double k, double *dOut);
 int bfk_scale_rows(int64_t n, int nrhs, const double *dW, const void *dX, void *dU, int fp32);
@@ -135,6 +135,7 @@ typedef struct BfMatHipButterfly {
   double *dArena;
   float *dArena32, *dArena32lo;
   double *dSrcXY, *dTgtXY;
+  double *dSrcN; /* source unit normals in tree order (double layer, combined field) */
   BfkSeg *dSegs;
   BfkTerm *dTerms;
   void *dPing, *dPong;

@@ -85,9 +85,29 @@ __device__ __forceinline__ double2 green(double k, double2 t, double2 s) {
   return make_double2(gr, gi);
 }
 
-/* the same, as a call: keeps the H0 coefficients from being hoisted into
- * registers for the whole of a kernel that samples only in its prologue */
-__device__ __attribute__((noinline)) double2 green_call(double k, double2 t, double2 s) { return green(k, t, s); }
+/* the layer potential of a factorization: 0 single, 1 double (with the
+ * source normal ns), 2 combined field alpha S + beta D (helm2.c:61-90) */
+__device__ __forceinline__ double2 green_layer(const BfkLayer &L, double k, double2 t, double2 s, double2 ns) {
+  if (L.layer == 0) return green(k, t, s);
+  const double dx = t.x - s.x, dy = t.y - s.y;
+  double dr, di;
+  bf_helm2_double(k, hypot(dx, dy), ns.x * dx + ns.y * dy, &dr, &di);
+  if (L.layer == 1) return make_double2(dr, di);
+  const double2 g = green(k, t, s);
+  return make_double2(L.ar * g.x - L.ai * g.y + L.br * dr - L.bi * di, L.ar * g.y + L.ai * g.x + L.br * di + L.bi * dr);
+}
+
+/* unit normal of a point on a proxy circle (circle.c: the radial direction) */
+__device__ __forceinline__ double2 circle_normal(double cx, double cy, double r, double2 pt) {
+  return make_double2((pt.x - cx) / r, (pt.y - cy) / r);
+}
+
+/* the same, as a call: keeps the Hankel coefficients from being hoisted
+ * into registers for the whole of a kernel that samples only in its
+ * prologue */
+__device__ __attribute__((noinline)) double2 green_call(BfkLayer L, double k, double2 t, double2 s, double2 ns) {
+  return green_layer(L, k, t, s, ns);
+}
 
 /* ---------------------------------------------------------- sampling */
 
@@ -96,23 +116,30 @@ __device__ __attribute__((noinline)) double2 green_call(double k, double2 t, dou
 
 __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *__restrict__ jobs,
                                                            const double2 *__restrict__ srcXY,
-                                                           const double2 *__restrict__ tgtXY, double k,
+                                                           const double2 *__restrict__ tgtXY,
+                                                           const double2 *__restrict__ srcN, BfkLayer L,
                                                            double2 *__restrict__ arena) {
   __shared__ double2 spts[SAMPLE_CHUNK];
+  __shared__ double2 snrm[SAMPLE_CHUNK];
   const BfkKernelJob J = jobs[blockIdx.x];
   const int m = J.m;
   for (int c0 = 0; c0 < J.n; c0 += SAMPLE_CHUNK) {
     int cn = min(SAMPLE_CHUNK, J.n - c0);
     __syncthreads();
-    for (int j = threadIdx.x; j < cn; j += SAMPLE_THREADS)
-      spts[j] = J.srcOff >= 0 ? srcXY[J.srcOff + c0 + j] : circle_pt(J.scx, J.scy, J.sr, J.n, c0 + j);
+    for (int j = threadIdx.x; j < cn; j += SAMPLE_THREADS) {
+      const double2 pt = J.srcOff >= 0 ? srcXY[J.srcOff + c0 + j] : circle_pt(J.scx, J.scy, J.sr, J.n, c0 + j);
+      spts[j] = pt;
+      if (L.layer != 0)
+        snrm[j] = J.srcOff >= 0 ? srcN[J.srcOff + c0 + j] : circle_normal(J.scx, J.scy, J.sr, pt);
+    }
     __syncthreads();
     const int64_t cnt = (int64_t)m * cn;
     double2 *out = arena + J.out + (int64_t)c0 * m;
     for (int64_t e = threadIdx.x; e < cnt; e += SAMPLE_THREADS) {
       int j = (int)(e / m);
       int i = (int)(e - (int64_t)j * m);
-      out[e] = green(J.k, tgtXY[J.tgtOff + i], spts[j]);
+      out[e] = L.layer == 0 ? green(J.k, tgtXY[J.tgtOff + i], spts[j])
+                            : green_layer(L, J.k, tgtXY[J.tgtOff + i], spts[j], snrm[j]);
     }
   }
 }
@@ -801,9 +828,10 @@ __device__ __forceinline__ LstsqView lstsq_view(double2 *base, int p, int n) {
  * LDS of the common 8-12 KiB jobs allows about that many per CU anyway */
 template <int T, bool GLOBAL>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_lstsq(const BfkLstsqJob *__restrict__ jobs, const int32_t *__restrict__ order,
-                                             const double2 *__restrict__ srcXY, double k,
+                                             const double2 *__restrict__ srcXY,
+                                             const double2 *__restrict__ srcN, BfkLayer Lay,
                                              double2 *__restrict__ arena, double2 *__restrict__ gwork,
-                                             int64_t gstride, double rcond) {
+                                             int64_t gstride) {
   extern __shared__ double2 lds[];
   const BfkLstsqJob J = jobs[order ? order[blockIdx.x] : blockIdx.x];
   const int p = J.p, n = J.n, ld = p + n + 1;
@@ -822,7 +850,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_l
     int i = e / W;
     int c = e - i * W;
     double2 s = c < p ? ep[c] : op[c - p];
-    L.M[i * ld + c] = green_call(J.k, tp[i], s);
+    double2 ns = make_double2(0, 0);
+    if (Lay.layer != 0)
+      ns = c < p ? circle_normal(J.ecx, J.ecy, J.er, s)
+                 : (J.origOff >= 0 ? srcN[J.origOff + c - p] : circle_normal(J.ocx, J.ocy, J.orr, s));
+    L.M[i * ld + c] = green_call(Lay, J.k, tp[i], s, ns);
   }
   __syncthreads();
   if (T == 64 && !GLOBAL)
@@ -1797,8 +1829,8 @@ static int time_end(int slot) {
 }
 
 /* kernel blocks; jobs already in device memory */
-int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY, double k,
-               double *dArena) {
+int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY,
+               const double *dSrcN, BfkLayer layer, double *dArena) {
   if (bfk_init()) return -1;
   if (numJobs == 0) {
     g_times[0] = 0;
@@ -1809,7 +1841,8 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
   for (int64_t j0 = 0; j0 < numJobs; j0 += maxGrid) {
     int64_t nj = numJobs - j0 < maxGrid ? numJobs - j0 : maxGrid;
     hipLaunchKernelGGL(k_sample, dim3((unsigned)nj), dim3(SAMPLE_THREADS), 0, g_stream, dJobs + j0,
-                       (const double2 *)dSrcXY, (const double2 *)dTgtXY, k, (double2 *)dArena);
+                       (const double2 *)dSrcXY, (const double2 *)dTgtXY, (const double2 *)dSrcN, layer,
+                       (double2 *)dArena);
     BFK_CHECK(hipGetLastError());
   }
   return time_end(0);
@@ -1852,7 +1885,7 @@ int bfk_lstsq_begin(size_t maxOversizeBytes, int64_t numOversize) {
  * above the LDS capacity run 256 threads each on the scratch of
  * bfk_lstsq_begin, chunk after chunk on g_stream. */
 int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t numJobs, size_t ldsBytes,
-                     const double *dSrcXY, double *dArena) {
+                     const double *dSrcXY, const double *dSrcN, BfkLayer layer, double *dArena) {
   if (numJobs <= 0) return 0;
   if (ldsBytes > g_ldsCap) {
     if (!g_lstsqWork || ldsBytes > g_lstsqWorkStride) return -3; /* bfk_lstsq_begin sized the scratch */
@@ -1861,7 +1894,8 @@ int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t nu
       int64_t nj = numJobs - j0 < BFK_SCRATCH_CHUNK ? numJobs - j0 : BFK_SCRATCH_CHUNK;
       hipLaunchKernelGGL((k_lstsq<256, true>), dim3((unsigned)nj), dim3(256), 0, g_stream,
                          dOrder ? dJobs : dJobs + j0, dOrder ? dOrder + j0 : (const int32_t *)nullptr,
-                         (const double2 *)dSrcXY, 0.0, (double2 *)dArena, g_lstsqWork, stride, 0.0);
+                         (const double2 *)dSrcXY, (const double2 *)dSrcN, layer, (double2 *)dArena, g_lstsqWork,
+                         stride);
       BFK_CHECK(hipGetLastError());
     }
     return 0;
@@ -1872,8 +1906,8 @@ int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t nu
   for (int64_t j0 = 0; j0 < numJobs; j0 += maxGrid) {
     int64_t nj = numJobs - j0 < maxGrid ? numJobs - j0 : maxGrid;
     hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), ldsBytes, st, dOrder ? dJobs : dJobs + j0,
-                       dOrder ? dOrder + j0 : (const int32_t *)nullptr, (const double2 *)dSrcXY, 0.0,
-                       (double2 *)dArena, (double2 *)nullptr, (int64_t)0, 0.0);
+                       dOrder ? dOrder + j0 : (const int32_t *)nullptr, (const double2 *)dSrcXY,
+                       (const double2 *)dSrcN, layer, (double2 *)dArena, (double2 *)nullptr, (int64_t)0);
     BFK_CHECK(hipGetLastError());
   }
   return 0;

@@ -42,6 +42,14 @@ typedef struct BfkKernelJob {
   int32_t m, n;
 } BfkKernelJob;
 
+/* The layer potential of a factorization (uniform over its jobs): 0 the
+ * single layer S, 1 the double layer D (source normals), 2 the combined
+ * field alpha S + beta D; alpha = ar + i ai, beta = br + i bi. */
+typedef struct BfkLayer {
+  int32_t layer, pad;
+  double ar, ai, br, bi;
+} BfkLayer;
+
 /* Vector buffers an apply stage reads or writes. */
 enum { BFK_BUF_X = 0, BFK_BUF_PING = 1, BFK_BUF_PONG = 2, BFK_BUF_Y = 3 };
 

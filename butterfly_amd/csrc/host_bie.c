@@ -19,6 +19,7 @@
  * host.  The mixed solver refines in FP64: r = b - A x with the FP64
  * apply, an inner GMRES on A d = r with the FP32 apply, x += d.
  */
+#include <complex.h>
 #include <math.h>
 
 #include "bf_host.h"
@@ -69,6 +70,15 @@ BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return NULL;
   }
+  /* the corrections treat the logarithmic singularity of the single layer:
+   * none for the (smooth) double layer, alpha times them for the combined
+   * field alpha S + beta D */
+  const BfLayerPotential lp = ((BfMatHipButterfly const *)fac)->helm.layerPot;
+  if (lp == BF_LAYER_POTENTIAL_PV_DOUBLE && krOrder != 0) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  const BfComplex cscale = lp == BF_LAYER_POTENTIAL_COMBINED_FIELD ? ((BfMatHipButterfly const *)fac)->helm.alpha : 1;
   const int64_t n = (int64_t)points->size;
   const int ord = (int)krOrder, nc = 2 * ord;
   const double *g = ord == 2 ? KR2 : KR6;
@@ -118,8 +128,9 @@ BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points
     double *c = malloc((size_t)np * 16);
     rc = bfk_d2h(c, A->dC, (size_t)np * 16);
     for (int64_t q = 0; q < np && !rc; ++q) {
-      c[2 * q] *= gam[q];
-      c[2 * q + 1] *= gam[q];
+      const BfComplex v = (c[2 * q] + I * c[2 * q + 1]) * gam[q] * cscale;
+      c[2 * q] = creal(v);
+      c[2 * q + 1] = cimag(v);
     }
     if (!rc) rc = bfk_h2d(A->dC, c, (size_t)np * 16);
     free(c);

@@ -7,6 +7,7 @@
 #define _DEFAULT_SOURCE 1
 #include <math.h>
 
+#include <complex.h>
 #include <pthread.h>
 
 #include "bf_host.h"
@@ -51,13 +52,14 @@ static void free_device(BfMatHipButterfly *F) {
   bfk_free(F->dArena32lo);
   bfk_free(F->dSrcXY);
   bfk_free(F->dTgtXY);
+  bfk_free(F->dSrcN);
   bfk_free(F->dSegs);
   bfk_free(F->dTerms);
   bfk_free(F->dPing);
   bfk_free(F->dPong);
   F->dArena = NULL;
   F->dArena32 = F->dArena32lo = NULL;
-  F->dSrcXY = F->dTgtXY = NULL;
+  F->dSrcXY = F->dTgtXY = F->dSrcN = NULL;
   F->dSegs = NULL;
   F->dTerms = NULL;
   F->dPing = F->dPong = NULL;
@@ -337,6 +339,32 @@ static double *tree_coords(BfQuadtree const *t) {
   return xy;
 }
 
+/* unit normals of the tree's points in tree order (NULL without normals) */
+static double *tree_normals(BfQuadtree const *t) {
+  if (!t->unitNormals) return NULL;
+  const BfSize n = t->points->size;
+  double *nv = malloc(2 * (n ? n : 1) * sizeof(double));
+  for (BfSize i = 0; i < n; ++i) {
+    const BfSize j = t->super.perm->index[i];
+    nv[2 * i] = t->unitNormals->data[j][0];
+    nv[2 * i + 1] = t->unitNormals->data[j][1];
+  }
+  return nv;
+}
+
+/* the kernels' form of the layer potential (the proxies use the same one,
+ * layer_pot.h:63-69, for S, D and the combined field) */
+static BfkLayer layer_of(BfHelm2 const *h) {
+  BfkLayer L;
+  memset(&L, 0, sizeof(L));
+  L.layer = h->layerPot == BF_LAYER_POTENTIAL_PV_DOUBLE ? 1 : h->layerPot == BF_LAYER_POTENTIAL_COMBINED_FIELD ? 2 : 0;
+  L.ar = creal(h->alpha);
+  L.ai = cimag(h->alpha);
+  L.br = creal(h->beta);
+  L.bi = cimag(h->beta);
+  return L;
+}
+
 typedef BF_VEC(size_t) BfSizeVec;
 
 /* Stable counting sort of jobs by LDS class (class c launches with c * 4 KiB):
@@ -430,7 +458,8 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   if (!F->dArena) return -2;
   void *dK = NULL;
   if ((rc = upload(&dK, F->kernelJobs.a, F->kernelJobs.n * sizeof(BfkKernelJob)))) return rc;
-  rc = bfk_sample(dK, (int64_t)F->kernelJobs.n, F->dSrcXY, F->dTgtXY, F->helm.k, F->dArena);
+  const BfkLayer layer = layer_of(&F->helm);
+  rc = bfk_sample(dK, (int64_t)F->kernelJobs.n, F->dSrcXY, F->dTgtXY, F->dSrcN, layer, F->dArena);
   bfk_free(dK);
   if (rc) return rc;
   const size_t nj = F->lstsqJobs.n;
@@ -478,7 +507,8 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
      * are complete, and the uploaded start of the next */
     while (!rc && g + 1 < gs.n && launched < up) {
       const size_t end = (size_t)gs.a[g + 1] < up ? (size_t)gs.a[g + 1] : up;
-      rc = bfk_lstsq_launch(dL + launched, NULL, (int64_t)(end - launched), gb.a[g], F->dSrcXY, F->dArena);
+      rc = bfk_lstsq_launch(dL + launched, NULL, (int64_t)(end - launched), gb.a[g], F->dSrcXY, F->dSrcN, layer,
+                            F->dArena);
       launched = end;
       if (launched == (size_t)gs.a[g + 1]) ++g;
     }
@@ -503,12 +533,18 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
 /* Everything after planning: coordinates up, sampling and compression
  * launched, the apply plan built on the host meanwhile, then the apply
  * tables up and the statistics.  Takes ownership of sxy / txy (tree-ordered
- * coordinates); on failure deletes F and returns NULL. */
-static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, double t0, double t1) {
-  int rc = upload((void **)&F->dSrcXY, sxy, 2 * F->numSrc * sizeof(double));
+ * coordinates) and snrm (tree-ordered source normals or NULL); on failure
+ * deletes F and returns NULL. */
+static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, double *snrm, double t0, double t1) {
+  /* the normal-dependent layers read the normals in every sampling and
+   * compression job: no kernel runs without them */
+  int rc = (F->helm.layerPot != BF_LAYER_POTENTIAL_SINGLE && !snrm) ? -2 : 0;
+  if (!rc) rc = upload((void **)&F->dSrcXY, sxy, 2 * F->numSrc * sizeof(double));
   if (!rc) rc = upload((void **)&F->dTgtXY, txy, 2 * F->numTgt * sizeof(double));
+  if (!rc && snrm) rc = upload((void **)&F->dSrcN, snrm, 2 * F->numSrc * sizeof(double));
   free(sxy);
   free(txy);
+  free(snrm);
   double tApplyPlan = 0;
   if (!rc) rc = run_factor_kernels(F, &tApplyPlan);
   if (!rc) rc = bfk_sync();
@@ -561,8 +597,16 @@ static int problem_ok(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return 0;
   }
-  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE) {
+  /* the single layer, the double layer and the combined field; the
+   * target-normal layers have no re-expansion (helm2.c:341-342) */
+  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE && helm->layerPot != BF_LAYER_POTENTIAL_PV_DOUBLE &&
+      helm->layerPot != BF_LAYER_POTENTIAL_COMBINED_FIELD) {
     bfSetError(BF_ERROR_NOT_IMPLEMENTED);
+    return 0;
+  }
+  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE &&
+      (!srcTree->unitNormals || srcTree->unitNormals->size != srcTree->points->size)) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return 0;
   }
   return 1;
@@ -589,7 +633,8 @@ BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const
     return NULL;
   }
   double t1 = bfWallTime();
-  return factor_from_plan(F, tree_coords(srcTree), tree_coords(tgtTree), t0, t1);
+  return factor_from_plan(F, tree_coords(srcTree), tree_coords(tgtTree),
+                          helm->layerPot != BF_LAYER_POTENTIAL_SINGLE ? tree_normals(srcTree) : NULL, t0, t1);
 }
 
 BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree const *const *srcTrees,
@@ -598,8 +643,16 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return NULL;
   }
-  for (BfSize i = 0; i < count; ++i)
+  for (BfSize i = 0; i < count; ++i) {
     if (!problem_ok(&helms[i], srcTrees[i], tgtTrees[i], &opts[i])) return NULL;
+    /* one layer potential per launch */
+    if (helms[i].layerPot != helms[0].layerPot || helms[i].alpha != helms[0].alpha ||
+        helms[i].beta != helms[0].beta) {
+      bfSetError(BF_ERROR_NOT_IMPLEMENTED);
+      return NULL;
+    }
+  }
+  const int withNormals = helms[0].layerPot != BF_LAYER_POTENTIAL_SINGLE;
   if (bfk_init()) {
     bfSetError(BF_ERROR_RUNTIME_ERROR);
     return NULL;
@@ -615,6 +668,7 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
   }
   bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
   double *sxy = malloc(2 * F->numSrc * sizeof(double)), *txy = malloc(2 * F->numTgt * sizeof(double));
+  double *snrm = withNormals ? malloc(2 * F->numSrc * sizeof(double)) : NULL;
   int64_t sBase = 0, tBase = 0;
   int bad = !sxy || !txy;
   for (BfSize i = 0; i < count && !bad; ++i) {
@@ -630,6 +684,11 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
       memcpy(txy + 2 * tBase, b, 2 * nt * sizeof(double));
       free(a);
       free(b);
+      if (snrm) {
+        double *c = tree_normals(srcTrees[i]);
+        memcpy(snrm + 2 * sBase, c, 2 * ns * sizeof(double));
+        free(c);
+      }
       sBase += (int64_t)ns;
       tBase += (int64_t)nt;
     }
@@ -639,12 +698,13 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
   if (bad) {
     free(sxy);
     free(txy);
+    free(snrm);
     bfMatHipButterflyDelete((BfMat **)&F);
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return NULL;
   }
   double t1 = bfWallTime();
-  return factor_from_plan(F, sxy, txy, t0, t1);
+  return factor_from_plan(F, sxy, txy, snrm, t0, t1);
 }
 
 BfMat *bfhipFacHelm2Plan(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,

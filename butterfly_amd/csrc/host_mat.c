@@ -4,6 +4,7 @@
  * (helm2.c:282-319), and the thin device wrappers of bf.h.
  */
 #define _DEFAULT_SOURCE 1
+#include <complex.h>
 #include <math.h>
 
 #include "bf_host.h"
@@ -195,14 +196,31 @@ BfMatDenseComplex *bfMatToMatDenseComplex(BfMat *mat) {
 
 /* --------------------------------------------- dense kernel matrix (GPU) */
 
-/* helm2.c:282-319 for the single-layer potential: rows are targets,
- * columns sources.  Sampled on the GPU in row panels of 64 targets. */
+/* helm2.c:282-319 for the single layer, the double layer and the combined
+ * field (the last two with the source normals Nsrc): rows are targets,
+ * columns sources.  Sampled on the GPU in row panels of 64 targets.  The
+ * target-normal layers are not provided (the reference's S' matrix,
+ * helm2.c:128-131, has no body). */
 BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoints2 const *Xtgt,
                               BfVectors2 const *Nsrc, BfVectors2 const *Ntgt) {
   if (!helm || !Xsrc || !Xtgt) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
   if (Nsrc && Nsrc->size != Xsrc->size) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
   if (Ntgt && Ntgt->size != Xtgt->size) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
-  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE) { bfSetError(BF_ERROR_NOT_IMPLEMENTED); return NULL; }
+  BfkLayer layer;
+  memset(&layer, 0, sizeof(layer));
+  if (helm->layerPot == BF_LAYER_POTENTIAL_PV_DOUBLE) {
+    layer.layer = 1;
+  } else if (helm->layerPot == BF_LAYER_POTENTIAL_COMBINED_FIELD) {
+    layer.layer = 2;
+    layer.ar = creal(helm->alpha);
+    layer.ai = cimag(helm->alpha);
+    layer.br = creal(helm->beta);
+    layer.bi = cimag(helm->beta);
+  } else if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE) {
+    bfSetError(BF_ERROR_NOT_IMPLEMENTED);
+    return NULL;
+  }
+  if (layer.layer != 0 && !Nsrc) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
   BfSize m = Xtgt->size, n = Xsrc->size;
   const BfSize panel = 64;
   BfSize np = (m + panel - 1) / panel;
@@ -217,13 +235,15 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
   }
   void *dJobs = bfk_malloc(np * sizeof(BfkKernelJob) + 16);
   void *dS = bfk_malloc(n * 16 + 16), *dT = bfk_malloc(m * 16 + 16), *dOut = bfk_malloc(m * n * 16 + 16);
+  void *dN = layer.layer != 0 ? bfk_malloc(n * 16 + 16) : NULL;
   BfMatDenseComplex *K = NULL;
   BfComplex *tmp = malloc((m * n > 0 ? m * n : 1) * sizeof(BfComplex));
-  int rc = (!dJobs || !dS || !dT || !dOut || !tmp) ? -2 : 0;
+  int rc = (!dJobs || !dS || !dT || !dOut || !tmp || (layer.layer != 0 && !dN)) ? -2 : 0;
+  if (!rc && dN) rc = bfk_h2d(dN, Nsrc->data, n * 16);
   if (!rc) rc = bfk_h2d(dJobs, jobs, np * sizeof(BfkKernelJob));
   if (!rc) rc = bfk_h2d(dS, Xsrc->data, n * 16);
   if (!rc) rc = bfk_h2d(dT, Xtgt->data, m * 16);
-  if (!rc) rc = bfk_sample(dJobs, (int64_t)np, dS, dT, helm->k, dOut);
+  if (!rc) rc = bfk_sample(dJobs, (int64_t)np, dS, dT, (const double *)dN, layer, dOut);
   if (!rc) rc = bfk_d2h(tmp, dOut, m * n * 16);
   if (!rc) {
     K = bfMatDenseComplexNew();
@@ -239,6 +259,7 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
   bfk_free(dS);
   bfk_free(dT);
   bfk_free(dOut);
+  bfk_free(dN);
   free(tmp);
   free(jobs);
   if (rc) {

@@ -236,8 +236,12 @@ BfSize bfQuadtreeNodeGetNumPoints(BfQuadtreeNode const *node);
 /* helm2.c:14-36 */
 BfSize bfHelm2RankEstForTwoCircles(BfHelm2 const *helm, BfCircle const *circ1,
                                    BfCircle const *circ2, BfReal C, BfReal eps);
-/* helm2.c:282-319: dense kernel matrix, rows = targets, cols = sources.
- * Sampled on the GPU by the sampling kernel; returns a host matrix. */
+/* helm2.c:282-319: dense kernel matrix, rows = targets, cols = sources, for
+ * the single layer, the double layer (PV_DOUBLE, source normals Nsrc) and
+ * the combined field alpha S + beta D (helm2.c:38-90).  Sampled on the GPU
+ * by the sampling kernel; returns a host matrix.  The target-normal layers
+ * return BF_ERROR_NOT_IMPLEMENTED (the reference's S' matrix,
+ * helm2.c:128-131, has no body). */
 BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoints2 const *Xtgt,
                               BfVectors2 const *Nsrc, BfVectors2 const *Ntgt);
 
@@ -246,7 +250,11 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
  * rules (dense below 128*128 entries, rank estimates at 1e-15,
  * bfFacHelm2Prepare's level choice).  Factors live in GPU memory; the
  * returned matrix has type BF_TYPE_MAT_HIP_BUTTERFLY and is applied with
- * bfMatMul. */
+ * bfMatMul.  Layer potentials: single, double (PV_DOUBLE) and combined
+ * field, the last two with the unit normals of `srcTree` (its proxies use
+ * the same layer with the circles' radial normals, layer_pot.h:63-69,
+ * fac_helm2.c:83-118); the target-normal layers have no re-expansion
+ * (helm2.c:341-342) and return BF_ERROR_NOT_IMPLEMENTED. */
 BfMat *bfFacHelm2MakeMultilevel(BfHelm2 const *helm, BfQuadtree const *srcTree,
                                 BfQuadtree const *tgtTree);
 

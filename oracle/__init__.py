@@ -43,6 +43,9 @@ def lib():
         "orc_svd_values": (None, [sz, sz, dp, dp]),
         "orc_fac_make": (vp, [d, dp, sz, dp, sz, sz, d, sz, C.c_int]),
         "orc_set_plan_only": (None, [C.c_int]),
+        "orc_h1": (None, [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "orc_dense_rows_layer": (None, [C.c_double, C.c_int, dp, dp, C.c_size_t, dp, C.c_size_t, dp, dp, dp, dp]),
+        "orc_kernel_matrix_layer": (None, [C.c_double, C.c_int, dp, dp, C.c_size_t, dp, C.c_size_t, dp, dp, dp]),
         "orc_set_rank_const": (None, [d]),
         "orc_set_pinv_rcond": (None, [d]),
         "orc_set_impedance": (None, [C.c_int]),
@@ -138,6 +141,48 @@ def tree(points, leaf_size):
     lib().orc_tree_build(_dp(points), n, leaf_size, perm.ctypes.data_as(C.POINTER(C.c_size_t)),
                          nodes.ctypes.data_as(C.POINTER(C.c_int64)), _dp(bbox), cnt)
     return perm.astype(np.int64), nodes, bbox
+
+
+def _layer_args(layer, alpha, beta):
+    lay = {"single": 0, "double": 1, "combined": 2}[layer]
+    a = np.array([complex(alpha).real, complex(alpha).imag])
+    b = np.array([complex(beta).real, complex(beta).imag])
+    return lay, a, b
+
+
+def dense_rows_layer(k, tgt, src, src_normals, x, layer="double", alpha=1.0, beta=1.0):
+    """Rows of K x for the double layer or the combined field alpha S + beta D
+    (helm2.c:38-90), Neumaier-summed."""
+    tgt = np.ascontiguousarray(tgt, np.float64)
+    src = np.ascontiguousarray(src, np.float64)
+    nrm = np.ascontiguousarray(src_normals, np.float64)
+    x = np.ascontiguousarray(x, np.complex128)
+    lay, a, b = _layer_args(layer, alpha, beta)
+    out = np.zeros(2 * len(tgt))
+    lib().orc_dense_rows_layer(k, lay, _dp(a), _dp(b), len(tgt), _dp(tgt), len(src), _dp(src), _dp(nrm),
+                               _dp(x.view(np.float64)), _dp(out))
+    return out.view(np.complex128)
+
+
+def kernel_matrix_layer(k, tgt, src, src_normals, layer="double", alpha=1.0, beta=1.0):
+    tgt = np.ascontiguousarray(tgt, np.float64)
+    src = np.ascontiguousarray(src, np.float64)
+    nrm = np.ascontiguousarray(src_normals, np.float64)
+    lay, a, b = _layer_args(layer, alpha, beta)
+    out = np.zeros(2 * len(tgt) * len(src))
+    lib().orc_kernel_matrix_layer(k, lay, _dp(a), _dp(b), len(tgt), _dp(tgt), len(src), _dp(src), _dp(nrm),
+                                  _dp(out))
+    return out.view(np.complex128).reshape(len(tgt), len(src))
+
+
+def h1(x):
+    x = np.atleast_1d(np.asarray(x, np.float64))
+    out = np.empty(len(x), complex)
+    re, im = C.c_double(), C.c_double()
+    for i, v in enumerate(x):
+        lib().orc_h1(float(v), C.byref(re), C.byref(im))
+        out[i] = complex(re.value, im.value)
+    return out
 
 
 def dense_rows(k, tgt, src, x):

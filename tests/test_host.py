@@ -173,3 +173,35 @@ def test_device_h0_source_on_host(tmp_path):
     got = out.view(np.complex128)
     err = np.abs(got - want) / np.abs(want)
     assert err.max() < 2e-15, (x[np.argmax(err)], err.max())
+
+
+def test_device_h1_source_on_host(tmp_path):
+    """The device H1 (double layer) compiled for the host against the mpmath
+    table, and the double-layer kernel value against its closed form."""
+    shim = tmp_path / "h1shim.cpp"
+    shim.write_text('#include "bf_h0.h"\nextern "C" void h1v(int n, const double *x, double *o)'
+                    '{for(int i=0;i<n;i++) bf_h1(x[i], &o[2*i], &o[2*i+1]);}\n'
+                    'extern "C" void dlv(double k, double r, double dot, double *o)'
+                    '{bf_helm2_double(k, r, dot, &o[0], &o[1]);}\n')
+    so = tmp_path / "h1shim.so"
+    subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-I",
+                    os.path.join(ROOT, "butterfly_amd", "csrc"), str(shim), "-o", str(so)], check=True)
+    L = C.CDLL(str(so))
+    tab = json.load(open(os.path.join(ROOT, "tests", "golden", "h1_table.json")))["rows"]
+    x = np.array([float(r[0]) for r in tab])
+    want = np.array([complex(float(r[1]), float(r[2])) for r in tab])
+    out = np.zeros(2 * len(x))
+    L.h1v(len(x), x.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
+    got = out.view(np.complex128)
+    err = np.abs(got - want) / np.abs(want)
+    assert err.max() < 3e-15, (x[np.argmax(err)], err.max())
+    # (i k / 4) H1(k r) dot / r, and 0 below r = 1e-15
+    from scipy.special import hankel1
+    o = np.zeros(2)
+    L.dlv.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    for k, r, dot in [(30.0, 0.2, 0.13), (7.0, 2.5, -1.7), (1000.0, 0.01, 0.004)]:
+        L.dlv(k, r, dot, o.ctypes.data_as(C.POINTER(C.c_double)))
+        ref = 0.25j * k * hankel1(1, k * r) * dot / r
+        assert abs(complex(o[0], o[1]) - ref) < 1e-13 * abs(ref)
+    L.dlv(30.0, 1e-16, 1.0, o.ctypes.data_as(C.POINTER(C.c_double)))
+    assert o[0] == 0 and o[1] == 0
