@@ -57,6 +57,41 @@ def test_h0_matches_mpmath_table():
     assert np.all(err <= 1e-14 + 4e-16 * x), (x[np.argmax(err)], err.max())
 
 
+def test_h1_matches_mpmath_table():
+    tab = json.load(open(os.path.join(HERE, "golden", "h1_table.json")))["rows"]
+    x = np.array([float(r[0]) for r in tab])
+    want = np.array([complex(float(r[1]), float(r[2])) for r in tab])
+    err = np.abs(orc.h1(x) - want) / np.abs(want)
+    assert np.all(err <= 1e-14 + 4e-16 * x), (x[np.argmax(err)], err.max())
+
+
+def test_double_layer_is_the_source_normal_derivative():
+    """D(t, s) = n_s . grad_s G(t, s): a central difference of the oracle's
+    single-layer entries along the source normal (a sign or direction slip
+    in n_s.(t - s) flips it); the combined field is alpha S + beta D; r = 0
+    gives 0."""
+    rng = np.random.default_rng(5)
+    k = 11.0
+    src = rng.uniform(-1, 1, (40, 2))
+    ang = rng.uniform(0, 2 * np.pi, 40)
+    nrm = np.stack([np.cos(ang), np.sin(ang)], 1)
+    tgt = rng.uniform(2, 3, (30, 2))
+    h = 1e-5
+    fd = (orc.kernel_matrix(k, tgt, src + h * nrm) - orc.kernel_matrix(k, tgt, src - h * nrm)) / (2 * h)
+    D = orc.kernel_matrix_layer(k, tgt, src, nrm, "double")
+    assert np.abs(D - fd).max() < 1e-7 * np.abs(D).max()
+    S = orc.kernel_matrix(k, tgt, src)
+    a, b = 0.3 - 2j, -1.5 + 0.25j
+    CF = orc.kernel_matrix_layer(k, tgt, src, nrm, "combined", a, b)
+    assert np.abs(CF - (a * S + b * D)).max() < 1e-14 * np.abs(CF).max()
+    Z = orc.kernel_matrix_layer(k, src[:5], src[:5], nrm[:5], "double")
+    assert np.all(np.diag(Z) == 0)
+    # dense rows (Neumaier-summed) agree with the matrix product
+    x = rng.standard_normal(40) + 1j * rng.standard_normal(40)
+    y = orc.dense_rows_layer(k, tgt, src, nrm, x, "combined", a, b)
+    assert np.abs(y - CF @ x).max() < 1e-13 * np.abs(y).max()
+
+
 def test_kernel_matrix_closed_form_and_zero_distance():
     k = 7.5
     tgt = np.array([[0.0, 0.0], [1.0, 2.0], [0.5, 0.5]])
