@@ -91,6 +91,7 @@ def lib():
         raise BfError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C butterfly_amd/csrc)")
     L = C.CDLL(LIB_PATH)
     vp, sz, dp, i64p = C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    fp = C.POINTER(C.c_float)
     sig = {
         "bfGetError": (C.c_int, []),
         "bfSetError": (None, [C.c_int]),
@@ -171,6 +172,20 @@ def lib():
         "bfhipBieMulDevice": (C.c_int, [vp, vp, vp, sz, C.c_int]),
         "bfSolveGMRES": (vp, [vp, vp, vp, C.c_double, sz, C.POINTER(sz), vp]),
         "bfhipSolveGmresMixed": (C.c_int, [vp, dp, dp, C.c_double, sz, sz, C.POINTER(sz), C.POINTER(sz), dp]),
+        "bfhipSvdBatchedSketch": (C.c_int, [sz, C.POINTER(sz), C.POINTER(sz), dp, C.c_double, C.POINTER(sz),
+                                            C.c_uint64, C.POINTER(sz), dp, dp, C.POINTER(C.c_int)]),
+        "bfhipRankFeatures": (None, [dp, dp, C.c_double, sz, C.c_double, fp]),
+        "bfhipRankModelNew": (vp, [C.c_uint64]),
+        "bfhipRankModelDelete": (None, [C.POINTER(vp)]),
+        "bfhipRankModelTrain": (C.c_int, [vp, sz, fp, fp, sz, sz, C.c_float, fp]),
+        "bfhipRankModelPredict": (C.c_int, [vp, sz, fp, fp]),
+        "bfhipRankModelGradient": (C.c_int, [vp, sz, fp, fp, fp, fp]),
+        "bfhipRankModelNumParams": (sz, []),
+        "bfhipRankModelGetParams": (C.c_int, [vp, fp, fp]),
+        "bfhipRankModelSetParams": (C.c_int, [vp, fp, fp]),
+        "bfhipFacStreamerSetRankModel": (C.c_int, [vp, vp, C.c_double, sz]),
+        "bfhipFacStreamerGetRankSamples": (sz, [vp, fp, fp, sz]),
+        "bfhipFacStreamerGetSketchStats": (None, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -505,6 +520,119 @@ def svd_batched(mats, tol):
     return out
 
 
+def svd_batched_sketch(mats, tol, widths, seed=1):
+    """bfhipSvdBatchedSketch: randomized truncated SVDs of sketch widths
+    `widths` (full SVD where a sketch is rejected or not narrower);
+    returns [(U, W, k, fell_back)]."""
+    ms = [np.asarray(a, np.complex128) for a in mats]
+    nb = len(ms)
+    m = (C.c_size_t * nb)(*[a.shape[0] for a in ms])
+    n = (C.c_size_t * nb)(*[a.shape[1] for a in ms])
+    w = (C.c_size_t * nb)(*[int(x) for x in widths])
+    A = np.concatenate([a.ravel(order="F") for a in ms]) if ms else np.zeros(1, complex)
+    full = [min(a.shape) for a in ms]
+    U = np.zeros(max(1, sum(a.shape[0] * f for a, f in zip(ms, full))), np.complex128)
+    W = np.zeros(max(1, sum(f * a.shape[1] for a, f in zip(ms, full))), np.complex128)
+    k = (C.c_size_t * nb)()
+    fell = (C.c_int * nb)()
+    if lib().bfhipSvdBatchedSketch(nb, m, n, _dp(A.view(np.float64)), tol, w, seed, k, _dp(U.view(np.float64)),
+                                   _dp(W.view(np.float64)), fell):
+        check()
+        raise BfError("svd_batched_sketch failed")
+    out, uo, wo = [], 0, 0
+    for a, f, kk, fb in zip(ms, full, k, fell):
+        out.append((U[uo:uo + a.shape[0] * kk].reshape(kk, a.shape[0]).T,
+                    W[wo:wo + kk * a.shape[1]].reshape(a.shape[1], kk).T, kk, bool(fb)))
+        uo += a.shape[0] * f
+        wo += f * a.shape[1]
+    return out
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def rank_features(col_box, row_box, k, level, tol):
+    """bfhipRankFeatures: the rank model's six inputs for a (column node,
+    row node) pair; boxes are (xmin, ymin, xmax, ymax)."""
+    cb = np.ascontiguousarray(col_box, np.float64)
+    rb = np.ascontiguousarray(row_box, np.float64)
+    out = np.zeros(6, np.float32)
+    lib().bfhipRankFeatures(_dp(cb), _dp(rb), float(k), int(level), float(tol), _fp(out))
+    return out
+
+
+class RankModel:
+    """bfhipRankModel*: the FP32 rank MLP (6 -> 64 -> 64 -> 64 -> 1, GELU),
+    trained and evaluated on the device."""
+
+    def __init__(self, seed=0):
+        self.ptr = lib().bfhipRankModelNew(seed)
+        if not self.ptr:
+            check()
+            raise BfError("rank model allocation failed")
+
+    def train(self, features, ranks, epochs=10, batch=4096, lr=1e-3):
+        """Mini-batch Adam on the squared error; returns the mean squared
+        error (ranks^2) of the last epoch's mini-batches."""
+        X = np.ascontiguousarray(features, np.float32).reshape(-1, 6)
+        y = np.ascontiguousarray(ranks, np.float32).ravel()
+        loss = np.zeros(1, np.float32)
+        if lib().bfhipRankModelTrain(self.ptr, len(y), _fp(X), _fp(y), epochs, batch, lr, _fp(loss)):
+            check()
+            raise BfError("rank model training failed")
+        return float(loss[0])
+
+    def predict(self, features):
+        X = np.ascontiguousarray(features, np.float32).reshape(-1, 6)
+        out = np.zeros(len(X), np.float32)
+        if lib().bfhipRankModelPredict(self.ptr, len(X), _fp(X), _fp(out)):
+            check()
+            raise BfError("rank model prediction failed")
+        return out
+
+    def gradient(self, features, ranks):
+        """(gradient, mean squared error) over all samples in standardized
+        units, parameters unchanged."""
+        X = np.ascontiguousarray(features, np.float32).reshape(-1, 6)
+        y = np.ascontiguousarray(ranks, np.float32).ravel()
+        g = np.zeros(lib().bfhipRankModelNumParams(), np.float32)
+        loss = np.zeros(1, np.float32)
+        if lib().bfhipRankModelGradient(self.ptr, len(y), _fp(X), _fp(y), _fp(g), _fp(loss)):
+            check()
+            raise BfError("rank model gradient failed")
+        return g, float(loss[0])
+
+    @property
+    def params(self):
+        """(parameters, normalization): W1 64x6, b1, W2 64x64, b2, W3 64x64,
+        b3, W4 1x64, b4 (row-major W[out][in]); feature mean[6], deviation[6],
+        target mean, deviation."""
+        p = np.zeros(lib().bfhipRankModelNumParams(), np.float32)
+        nm = np.zeros(14, np.float32)
+        if lib().bfhipRankModelGetParams(self.ptr, _fp(p), _fp(nm)):
+            check()
+            raise BfError("rank model read failed")
+        return p, nm
+
+    def set_params(self, p, nm):
+        p = np.ascontiguousarray(p, np.float32)
+        nm = np.ascontiguousarray(nm, np.float32)
+        if len(p) != lib().bfhipRankModelNumParams() or len(nm) != 14:
+            raise BfError("wrong parameter count")
+        if lib().bfhipRankModelSetParams(self.ptr, _fp(p), _fp(nm)):
+            check()
+            raise BfError("rank model write failed")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().bfhipRankModelDelete(C.byref(C.c_void_p(self.ptr)))
+                self.ptr = None
+        except Exception:
+            pass
+
+
 class FacSpecStruct(C.Structure):
     _fields_ = [("rowTree", C.c_void_p), ("colTree", C.c_void_p), ("rowTreeInitDepth", C.c_size_t),
                 ("tol", C.c_double), ("minNumRows", C.c_size_t), ("minNumCols", C.c_size_t),
@@ -585,6 +713,30 @@ class FacStreamer:
             if err:
                 raise BfError(f"feed failed (error {err})")
         return bool(ok)
+
+    def set_rank_model(self, model, k=0.0, oversample=8):
+        """bfhipFacStreamerSetRankModel: randomized SVDs of width predicted
+        rank + oversample (model None: plain SVDs); samples recorded either way."""
+        self.rank_model = model
+        if lib().bfhipFacStreamerSetRankModel(self.ptr, model.ptr if model is not None else None, float(k),
+                                              int(oversample)):
+            check()
+            raise BfError("set_rank_model failed")
+
+    def rank_samples(self):
+        """(features n x 6, numerical ranks n) of the SVDs so far."""
+        L = lib()
+        n = L.bfhipFacStreamerGetRankSamples(self.ptr, None, None, 0)
+        X = np.zeros((n, 6), np.float32)
+        y = np.zeros(n, np.float32)
+        if n:
+            L.bfhipFacStreamerGetRankSamples(self.ptr, _fp(X), _fp(y), n)
+        return X, y
+
+    def sketch_stats(self):
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        lib().bfhipFacStreamerGetSketchStats(self.ptr, C.byref(a), C.byref(b), C.byref(c))
+        return {"svds": a.value, "sketched": b.value, "fell_back": c.value}
 
     def fac(self):
         p = lib().bfFacStreamerGetFac(self.ptr)

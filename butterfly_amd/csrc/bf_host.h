@@ -12,6 +12,19 @@
 int bfk_init(void);
 int bfk_device_count(void);
 size_t bfk_lds_cap(void);
+void *bfk_stream_handle(void);
+/* rank model and randomized range finder (bf_rank.hip) */
+int bfk_mlp_nparam(void);
+int bfk_mlp_groups(int B);
+int bfk_mlp_step(float *dP, float *dM, float *dV, const float *dX, const float *dY, const int32_t *dIdx, int64_t b0,
+                 int B, float lr, float c1, float c2, float *dPart, float *dLossPart, double *dLossAcc);
+int bfk_mlp_grad(const float *dP, const float *dX, const float *dY, const int32_t *dIdx, int B, float *dPart,
+                 float *dLossPart);
+int bfk_mlp_predict(const float *dP, const float *dX, int64_t n, float *dOut);
+int bfk_sketch_jobs_bytes(void);
+int bfk_sketch_range(int64_t batch, const void *dJobs, uint64_t seed, const double *dA, double *dOm, double *dY);
+int bfk_sketch_project(int64_t batch, const void *dJobs, const int64_t *dSOff, const double *dSig, const double *dA,
+                       double *dY, double *dBH, double *dRes);
 size_t bfk_lstsq_lds_bytes(int p, int n);
 void *bfk_malloc(size_t bytes);
 void bfk_free(void *p);

@@ -1698,6 +1698,9 @@ int bfk_device_count(void) {
 
 size_t bfk_lds_cap(void) { return g_ldsCap; }
 
+/* the library's main stream, for the launchers of bf_rank.hip */
+void *bfk_stream_handle(void) { return (void *)g_stream; }
+
 size_t bfk_lstsq_lds_bytes(int p, int n) { return lstsq_units(p, n) * 16; }
 
 void *bfk_malloc(size_t bytes) {

@@ -326,6 +326,248 @@ int bfhipSvdBatched(BfSize batch, BfSize const *m, BfSize const *n, BfComplex co
   return rc;
 }
 
+/* ------------------------------------------- randomized truncated SVD */
+
+/* one matrix of a sketch batch (the kernels' BfkSketchJob) */
+typedef struct {
+  int32_t m, n, w, pad;
+  int64_t aOff, oOff, yOff, bOff;
+} SketchJob;
+
+/* Truncated SVDs through a randomized range finder (Halko, Martinsson &
+ * Tropp 2011, Alg. 4.1 then 5.1): Y = A Omega (n x w Gaussian Omega),
+ * Q = orth(Y) from the one-sided Jacobi of Y, B^H = A^H Q, then the Jacobi
+ * SVD of B^H gives A ~ Q B = sum_j (Q y_j)(sigma_j x_j^H).  A sketch is
+ * accepted when ||A - Q B||_F <= tol sigma_0 (so every discarded singular
+ * value of A is below the cut) and fewer than w terms pass the cut (the
+ * width did not saturate); a rejected sketch (fell[i] = 1) and every job
+ * with width 0, width >= min(m, n) or width > 256 take the full SVD.  The
+ * rule for k is svd_batch's (linalg.c:26-35). */
+static int svd_batch_sketch(SvdJob *jobs, size_t nj, double tol, const int64_t *width, uint64_t seed, int *fell) {
+  if (nj == 0) return 0;
+  if (bfk_sketch_jobs_bytes() != (int)sizeof(SketchJob)) return -1;
+  size_t *sk = malloc(nz((int64_t)nj) * sizeof(size_t)), *full = malloc(nz((int64_t)nj) * sizeof(size_t));
+  size_t ns = 0, nf = 0;
+  for (size_t i = 0; i < nj; ++i) {
+    fell[i] = 0;
+    const int64_t mn = jobs[i].m < jobs[i].n ? jobs[i].m : jobs[i].n;
+    if (width[i] > 0 && width[i] < mn && width[i] <= 256) sk[ns++] = i;
+    else full[nf++] = i;
+  }
+  int rc = 0;
+  if (ns) {
+    SketchJob *sj = calloc(ns, sizeof(SketchJob));
+    int64_t *yOff = malloc(ns * 8), *bOff = malloc(ns * 8), *v1 = malloc(ns * 8), *v2 = malloc(ns * 8),
+            *s1 = malloc(ns * 8), *s2 = malloc(ns * 8);
+    int32_t *M = malloc(ns * 4), *N = malloc(ns * 4), *Wd = malloc(ns * 4);
+    int64_t tA = 0, tO = 0, tY = 0, tB = 0, tV = 0, tS = 0;
+    for (size_t q = 0; q < ns; ++q) {
+      const SvdJob *J = &jobs[sk[q]];
+      const int64_t w = width[sk[q]];
+      SketchJob x = {(int32_t)J->m, (int32_t)J->n, (int32_t)w, 0, tA, tO, tY, tB};
+      sj[q] = x;
+      M[q] = (int32_t)J->m;
+      N[q] = (int32_t)J->n;
+      Wd[q] = (int32_t)w;
+      yOff[q] = tY;
+      bOff[q] = tB;
+      v1[q] = tV;
+      v2[q] = tV + w * w;
+      s1[q] = tS;
+      s2[q] = tS + w;
+      tA += J->m * J->n;
+      tO += J->n * w;
+      tY += J->m * w;
+      tB += J->n * w;
+      tV += 2 * w * w;
+      tS += 2 * w;
+    }
+    BfComplex *hA = malloc(nz(tA) * sizeof(BfComplex)), *hY = malloc(nz(tY) * sizeof(BfComplex));
+    BfComplex *hB = malloc(nz(tB) * sizeof(BfComplex)), *hV = malloc(nz(tV) * sizeof(BfComplex));
+    double *hS = malloc(nz(tS) * sizeof(double)), *hR = malloc(ns * sizeof(double));
+    for (size_t q = 0; q < ns; ++q) {
+      const SvdJob *J = &jobs[sk[q]];
+      memcpy(hA + sj[q].aOff, J->a, (size_t)(J->m * J->n) * sizeof(BfComplex));
+    }
+    void *dA = bfk_malloc(nz(tA) * 16), *dO = bfk_malloc(nz(tO) * 16), *dY = bfk_malloc(nz(tY) * 16),
+         *dB = bfk_malloc(nz(tB) * 16), *dV = bfk_malloc(nz(tV) * 16), *dS = bfk_malloc(nz(tS) * 8),
+         *dR = bfk_malloc(ns * 8), *dJ = bfk_malloc(ns * sizeof(SketchJob)), *dYo = bfk_malloc(ns * 8),
+         *dBo = bfk_malloc(ns * 8), *dV1 = bfk_malloc(ns * 8), *dV2 = bfk_malloc(ns * 8), *dS1 = bfk_malloc(ns * 8),
+         *dS2 = bfk_malloc(ns * 8), *dM = bfk_malloc(ns * 4), *dN = bfk_malloc(ns * 4), *dW = bfk_malloc(ns * 4);
+    if (!dA || !dO || !dY || !dB || !dV || !dS || !dR || !dJ || !dYo || !dBo || !dV1 || !dV2 || !dS1 || !dS2 ||
+        !dM || !dN || !dW)
+      rc = -2;
+    if (!rc) rc = bfk_h2d(dA, hA, (size_t)tA * 16);
+    if (!rc) rc = bfk_h2d(dJ, sj, ns * sizeof(SketchJob));
+    if (!rc) rc = bfk_h2d(dYo, yOff, ns * 8);
+    if (!rc) rc = bfk_h2d(dBo, bOff, ns * 8);
+    if (!rc) rc = bfk_h2d(dV1, v1, ns * 8);
+    if (!rc) rc = bfk_h2d(dV2, v2, ns * 8);
+    if (!rc) rc = bfk_h2d(dS1, s1, ns * 8);
+    if (!rc) rc = bfk_h2d(dS2, s2, ns * 8);
+    if (!rc) rc = bfk_h2d(dM, M, ns * 4);
+    if (!rc) rc = bfk_h2d(dN, N, ns * 4);
+    if (!rc) rc = bfk_h2d(dW, Wd, ns * 4);
+    if (!rc) rc = bfk_sketch_range((int64_t)ns, dJ, seed, dA, dO, dY);
+    if (!rc) rc = bfk_svd_jacobi((int64_t)ns, dYo, dM, dW, dY, dV1, dV, dS1, dS, 60);
+    if (!rc) rc = bfk_sketch_project((int64_t)ns, dJ, dS1, dS, dA, dY, dB, dR);
+    if (!rc) rc = bfk_svd_jacobi((int64_t)ns, dBo, dN, dW, dB, dV2, dV, dS2, dS, 60);
+    if (!rc) rc = bfk_d2h(hY, dY, (size_t)tY * 16);
+    if (!rc) rc = bfk_d2h(hB, dB, (size_t)tB * 16);
+    if (!rc) rc = bfk_d2h(hV, dV, (size_t)tV * 16);
+    if (!rc) rc = bfk_d2h(hS, dS, (size_t)tS * 8);
+    if (!rc) rc = bfk_d2h(hR, dR, ns * 8);
+    bfk_free(dA); bfk_free(dO); bfk_free(dY); bfk_free(dB); bfk_free(dV); bfk_free(dS); bfk_free(dR);
+    bfk_free(dJ); bfk_free(dYo); bfk_free(dBo); bfk_free(dV1); bfk_free(dV2); bfk_free(dS1); bfk_free(dS2);
+    bfk_free(dM); bfk_free(dN); bfk_free(dW);
+    for (size_t q = 0; q < ns && !rc; ++q) {
+      SvdJob *J = &jobs[sk[q]];
+      const int64_t m = J->m, n = J->n, w = Wd[q];
+      const double *sg = hS + s2[q];
+      SigIdx *si = malloc((size_t)w * sizeof(SigIdx));
+      for (int64_t j = 0; j < w; ++j) { si[j].s = sg[j]; si[j].j = (int32_t)j; }
+      qsort(si, (size_t)w, sizeof(SigIdx), cmp_sig_desc);
+      const double s0 = si[0].s;
+      int64_t k = 0;
+      while (k < w && si[k].s >= tol * s0) ++k;
+      if (!(s0 > 0) || k >= w || !(hR[q] <= tol * s0 * tol * s0)) {
+        fell[sk[q]] = 1;
+        full[nf++] = sk[q];
+        free(si);
+        continue;
+      }
+      const BfComplex *Q = hY + yOff[q], *BH = hB + bOff[q], *V = hV + v2[q];
+      J->k = k;
+      J->truncated = 1;
+      J->U = malloc((size_t)(m * k) * sizeof(BfComplex));
+      J->W = malloc((size_t)(k * n) * sizeof(BfComplex));
+      for (int64_t t = 0; t < k; ++t) {
+        const int32_t j = si[t].j;
+        const BfComplex *vj = V + (int64_t)j * w;
+        for (int64_t r = 0; r < m; ++r) {
+          BfComplex u = 0;
+          for (int64_t c = 0; c < w; ++c) u += Q[c * m + r] * vj[c];
+          J->U[t * m + r] = u;
+        }
+        for (int64_t c = 0; c < n; ++c) J->W[c * k + t] = conj(BH[(int64_t)j * n + c]);
+      }
+      free(si);
+    }
+    free(hA); free(hY); free(hB); free(hV); free(hS); free(hR);
+    free(sj); free(yOff); free(bOff); free(v1); free(v2); free(s1); free(s2); free(M); free(N); free(Wd);
+  }
+  if (!rc && nf) { /* the full SVDs, rejected sketches included */
+    SvdJob *fj = malloc(nf * sizeof(SvdJob));
+    for (size_t x = 0; x < nf; ++x) fj[x] = jobs[full[x]];
+    rc = svd_batch(fj, nf, tol);
+    for (size_t x = 0; x < nf; ++x) {
+      SvdJob *J = &jobs[full[x]];
+      J->k = fj[x].k;
+      J->truncated = fj[x].truncated;
+      J->U = fj[x].U;
+      J->W = fj[x].W;
+    }
+    free(fj);
+  }
+  free(sk);
+  free(full);
+  return rc;
+}
+
+int bfhipSvdBatchedSketch(BfSize batch, BfSize const *m, BfSize const *n, BfComplex const *A, BfReal tol,
+                          BfSize const *width, uint64_t seed, BfSize *k, BfComplex *U, BfComplex *W, int *fellBack) {
+  if (!m || !n || !A || !width || !k || !U || !W || !fellBack) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  SvdJob *jobs = calloc(batch ? batch : 1, sizeof(SvdJob));
+  int64_t *wd = malloc((batch ? batch : 1) * sizeof(int64_t));
+  int64_t off = 0;
+  for (BfSize i = 0; i < batch; ++i) {
+    jobs[i].m = (int64_t)m[i];
+    jobs[i].n = (int64_t)n[i];
+    jobs[i].a = cdup(A + off, jobs[i].m * jobs[i].n);
+    wd[i] = (int64_t)width[i];
+    off += jobs[i].m * jobs[i].n;
+  }
+  int rc = svd_batch_sketch(jobs, batch, tol, wd, seed, fellBack);
+  int64_t uo = 0, wo = 0;
+  for (BfSize i = 0; i < batch; ++i) {
+    if (!rc) {
+      k[i] = (BfSize)jobs[i].k;
+      memcpy(U + uo, jobs[i].U, (size_t)(jobs[i].m * jobs[i].k) * sizeof(BfComplex));
+      memcpy(W + wo, jobs[i].W, (size_t)(jobs[i].k * jobs[i].n) * sizeof(BfComplex));
+    }
+    int64_t fullr = jobs[i].m < jobs[i].n ? jobs[i].m : jobs[i].n;
+    uo += jobs[i].m * fullr;
+    wo += fullr * jobs[i].n;
+    svd_job_free(&jobs[i]);
+  }
+  free(jobs);
+  free(wd);
+  if (rc) bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : rc == -3 ? BF_ERROR_NOT_IMPLEMENTED : BF_ERROR_RUNTIME_ERROR);
+  return rc;
+}
+
+/* ------------------------------------------------ rank model in a stream */
+
+/* the stream's rank model, its samples and counters */
+typedef struct {
+  const BfhipRankModel *model;
+  double k;
+  int64_t oversample;
+  int on;              /* record samples (set by bfhipFacStreamerSetRankModel) */
+  TreeView ct;
+  BF_VEC(float) feat;  /* 6 per sample */
+  BF_VEC(float) rank;
+  int64_t numSvd, numSketched, numFell;
+  uint64_t seed;
+} RankCtx;
+
+/* the SVDs of one round: with a model, widths from its predictions;
+ * samples recorded afterwards */
+static int svd_round(RankCtx *rk, const BfFacSpec *spec, TreeView rt, int32_t colNode, const int32_t *rowNodes,
+                     SvdJob *jobs, size_t nj) {
+  if (!rk || !rk->on || nj == 0) return svd_batch(jobs, nj, spec->tol);
+  float *f = malloc(nj * 6 * sizeof(float));
+  const BfQuadtreeNode *cn = &rk->ct.nodes[colNode];
+  const BfReal cb[4] = {cn->bboxMin[0], cn->bboxMin[1], cn->bboxMax[0], cn->bboxMax[1]};
+  for (size_t i = 0; i < nj; ++i) {
+    const BfQuadtreeNode *rn = &rt.nodes[rowNodes[i]];
+    const BfReal rb[4] = {rn->bboxMin[0], rn->bboxMin[1], rn->bboxMax[0], rn->bboxMax[1]};
+    bfhipRankFeatures(cb, rb, rk->k, (BfSize)rn->depth, spec->tol, f + 6 * i);
+  }
+  int rc = 0;
+  if (rk->model) {
+    float *pr = malloc(nj * sizeof(float));
+    int64_t *wd = malloc(nj * sizeof(int64_t));
+    int *fell = malloc(nj * sizeof(int));
+    rc = bfhipRankModelPredict(rk->model, nj, f, pr);
+    for (size_t i = 0; i < nj && !rc; ++i) {
+      const double p = pr[i] > 0 ? ceil((double)pr[i]) : 0;
+      wd[i] = (int64_t)p + rk->oversample;
+    }
+    if (!rc) rc = svd_batch_sketch(jobs, nj, spec->tol, wd, rk->seed++, fell);
+    for (size_t i = 0; i < nj && !rc; ++i) {
+      const int64_t mn = jobs[i].m < jobs[i].n ? jobs[i].m : jobs[i].n;
+      if (wd[i] < mn && wd[i] <= 256) rk->numSketched += 1;
+      rk->numFell += fell[i];
+    }
+    free(pr);
+    free(wd);
+    free(fell);
+  } else {
+    rc = svd_batch(jobs, nj, spec->tol);
+  }
+  for (size_t i = 0; i < nj && !rc; ++i) {
+    for (int q = 0; q < 6; ++q) BF_VEC_PUSH(rk->feat, f[6 * i + q]);
+    BF_VEC_PUSH(rk->rank, (float)jobs[i].k);
+  }
+  rk->numSvd += (int64_t)nj;
+  free(f);
+  return rc;
+}
+
 /* --------------------------------------------- epsilon-rank row cuts */
 
 /* One row block being cut: node `nd` of the row tree, the dense rows of
@@ -363,7 +605,7 @@ static BfComplex *rows_copy(const BfComplex *A, int64_t lda, int64_t cols, int64
  * Returns 0, or 1 when a leaf would not compress and `bail` is set. */
 static int eps_cut(const BfFacSpec *spec, TreeView rt, size_t numOwners, int32_t *const *frontiers,
                    const size_t *frontierLen, const int32_t *roots, BfComplex *const *mats, const int64_t *cols,
-                   int mergeRule, int bail, CutOutVec *out, int *rcOut) {
+                   int mergeRule, int bail, CutOutVec *out, int *rcOut, RankCtx *rk, int32_t colNode) {
   CutItemVec cur, nxt;
   memset(&cur, 0, sizeof(cur));
   memset(&nxt, 0, sizeof(nxt));
@@ -381,6 +623,8 @@ static int eps_cut(const BfFacSpec *spec, TreeView rt, size_t numOwners, int32_t
     /* classify, build the SVD batch */
     BF_VEC(SvdJob) jobs;
     memset(&jobs, 0, sizeof(jobs));
+    BF_VEC(int32_t) jobNodes;
+    memset(&jobNodes, 0, sizeof(jobNodes));
     for (size_t i = 0; i < cur.n; ++i) {
       CutItem *it = &cur.a[i];
       const int o = it->owner;
@@ -397,9 +641,11 @@ static int eps_cut(const BfFacSpec *spec, TreeView rt, size_t numOwners, int32_t
         J.n = cols[o];
         J.a = rows_copy(mats[o], lda, cols[o], r0, r1);
         BF_VEC_PUSH(jobs, J);
+        BF_VEC_PUSH(jobNodes, it->nd);
       }
     }
-    int rc = svd_batch(jobs.a, jobs.n, spec->tol);
+    int rc = svd_round(rk, spec, rt, colNode, jobNodes.a, jobs.a, jobs.n);
+    BF_VEC_FREE(jobNodes);
     if (rc) {
       *rcOut = rc;
       for (size_t j = 0; j < jobs.n; ++j) svd_job_free(&jobs.a[j]);
@@ -496,6 +742,7 @@ struct BfFacStreamer {
   size_t numFacs, capFacs;
   int32_t *initCut;
   size_t initCutLen;
+  RankCtx rank;
 };
 
 BfFacStreamer *bfFacStreamerNew(void) {
@@ -559,7 +806,39 @@ void bfFacStreamerDeinit(BfFacStreamer *s) {
   free(s->facs);
   free(s->post);
   free(s->initCut);
+  BF_VEC_FREE(s->rank.feat);
+  BF_VEC_FREE(s->rank.rank);
   memset(s, 0, sizeof(*s));
+}
+
+/* after bfFacStreamerInit (which clears it) */
+int bfhipFacStreamerSetRankModel(BfFacStreamer *s, BfhipRankModel const *model, BfReal k, BfSize oversample) {
+  if (!s || !s->post || !(k >= 0) || oversample > 256) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  s->rank.model = model;
+  s->rank.k = k;
+  s->rank.oversample = (int64_t)oversample;
+  s->rank.on = 1;
+  s->rank.ct = s->ct;
+  s->rank.seed = 0x2545F4914F6CDD1Dull;
+  return 0;
+}
+
+BfSize bfhipFacStreamerGetRankSamples(BfFacStreamer const *s, float *features, float *ranks, BfSize cap) {
+  if (!s) return 0;
+  const BfSize n = (BfSize)s->rank.rank.n;
+  const BfSize c = n < cap ? n : cap;
+  if (features && c) memcpy(features, s->rank.feat.a, c * 6 * sizeof(float));
+  if (ranks && c) memcpy(ranks, s->rank.rank.a, c * sizeof(float));
+  return n;
+}
+
+void bfhipFacStreamerGetSketchStats(BfFacStreamer const *s, BfSize *svds, BfSize *sketched, BfSize *fellBack) {
+  if (svds) *svds = s ? (BfSize)s->rank.numSvd : 0;
+  if (sketched) *sketched = s ? (BfSize)s->rank.numSketched : 0;
+  if (fellBack) *fellBack = s ? (BfSize)s->rank.numFell : 0;
 }
 
 void bfFacStreamerDealloc(BfFacStreamer **s) {
@@ -676,7 +955,7 @@ static BfFac *merge_children(BfFacStreamer *s, int32_t c, int *rc) {
   CutOutVec outs;
   memset(&outs, 0, sizeof(outs));
   int svdRc = 0;
-  eps_cut(&s->spec, rt, nr, fr, frl, cut.a, star, starCols, 1, 0, &outs, &svdRc);
+  eps_cut(&s->spec, rt, nr, fr, frl, cut.a, star, starCols, 1, 0, &outs, &svdRc, &s->rank, c);
   free(fr);
   free(frl);
   BfFac *m = NULL;
@@ -814,7 +1093,7 @@ bool bfFacStreamerFeed(BfFacStreamer *s, BfMat *PhiMat) {
   memset(&outs, 0, sizeof(outs));
   int svdRc = 0;
   int bailed = eps_cut(&s->spec, s->rt, 1, &fr, &frl, &root, &A, &ncols, 0, s->spec.bailIfBottomedOut, &outs,
-                       &svdRc);
+                       &svdRc, &s->rank, c);
   free(A);
   if (svdRc || bailed) {
     for (size_t q = 0; q < outs.n; ++q) { free(outs.a[q].Psi); free(outs.a[q].W); }

@@ -531,6 +531,48 @@ BfMat *bfhipFacLoadFile(char const *path);
 int bfhipSvdBatched(BfSize batch, BfSize const *m, BfSize const *n, BfComplex const *A, BfReal tol, BfSize *k,
                     BfComplex *U, BfComplex *W);
 
+/* new (no counterpart): the same truncated SVDs through a randomized range
+ * finder of width width_b (Halko, Martinsson & Tropp 2011, Alg. 4.1 and
+ * 5.1: Y = A Omega with a Gaussian Omega, Q = orth(Y), SVD of Q^H A).  A
+ * sketch is accepted when ||A - Q Q^H A||_F <= tol sigma_0 and fewer than
+ * width_b terms pass the cut; otherwise, or when width_b is 0 or not
+ * below min(m_b, n_b), the matrix takes the full SVD (fellBack_b = 1 for a
+ * rejected sketch).  The sketch lifts the 256-column limit of the full SVD
+ * for widths up to 256.  `seed` fixes Omega. */
+int bfhipSvdBatchedSketch(BfSize batch, BfSize const *m, BfSize const *n, BfComplex const *A, BfReal tol,
+                          BfSize const *width, uint64_t seed, BfSize *k, BfComplex *U, BfComplex *W, int *fellBack);
+
+/* new (no counterpart): the rank model, an FP32 MLP (6 -> 64 -> 64 -> 64
+ * -> 1, GELU) that predicts the numerical rank of a node-pair block from
+ * bfhipRankFeatures; trained (mini-batch Adam on the squared error) and
+ * evaluated by HIP kernels.  Features are n x 6 row-major floats. */
+typedef struct BfhipRankModel BfhipRankModel;
+void bfhipRankFeatures(BfReal const colBox[4], BfReal const rowBox[4], BfReal k, BfSize level, BfReal tol,
+                       float out[6]);
+BfhipRankModel *bfhipRankModelNew(uint64_t seed);
+void bfhipRankModelDelete(BfhipRankModel **model);
+int bfhipRankModelTrain(BfhipRankModel *model, BfSize n, float const *features, float const *ranks, BfSize epochs,
+                        BfSize batch, float lr, float *lastLoss);
+int bfhipRankModelPredict(BfhipRankModel const *model, BfSize n, float const *features, float *ranks);
+/* the gradient (standardized units, parameter layout of GetParams) and the
+ * mean squared error over all n samples, without a step */
+int bfhipRankModelGradient(BfhipRankModel const *model, BfSize n, float const *features, float const *ranks,
+                           float *grad, float *loss);
+BfSize bfhipRankModelNumParams(void);
+int bfhipRankModelGetParams(BfhipRankModel const *model, float *params, float norm[14]);
+int bfhipRankModelSetParams(BfhipRankModel *model, float const *params, float const norm[14]);
+
+/* new (no counterpart): with a model, every truncated SVD of the stream
+ * takes the randomized path with width = predicted rank + oversample (the
+ * full SVD stays the fallback and the judge); with or without one, the
+ * (features, numerical rank) pair of every SVD is recorded for training.
+ * `k` is the wavenumber of the kernel behind the stream (0 if none). */
+int bfhipFacStreamerSetRankModel(BfFacStreamer *facStreamer, BfhipRankModel const *model, BfReal k,
+                                 BfSize oversample);
+BfSize bfhipFacStreamerGetRankSamples(BfFacStreamer const *facStreamer, float *features, float *ranks, BfSize cap);
+void bfhipFacStreamerGetSketchStats(BfFacStreamer const *facStreamer, BfSize *svds, BfSize *sketched,
+                                    BfSize *fellBack);
+
 #ifdef __cplusplus
 }
 #endif

@@ -205,3 +205,13 @@ def test_device_h1_source_on_host(tmp_path):
         assert abs(complex(o[0], o[1]) - ref) < 1e-13 * abs(ref)
     L.dlv(30.0, 1e-16, 1.0, o.ctypes.data_as(C.POINTER(C.c_double)))
     assert o[0] == 0 and o[1] == 0
+
+
+def test_rank_features_closed_form():
+    """bfhipRankFeatures (host code): log diameters, k times the larger
+    diameter, centre distance over it, depth, log10 tol."""
+    import butterfly_amd as bf
+    f = bf.rank_features([0, 0, 1, 1], [2, 0, 4, 2], 10.0, 3, 1e-8)
+    dc, dr = np.sqrt(2), np.sqrt(8)
+    want = [np.log(dc), np.log(dr), 10 * dr, np.hypot(2.5, 0.5) / dr, 3, -8]
+    assert np.allclose(f, want, rtol=1e-6)

@@ -346,3 +346,28 @@ def test_kapur_rokhlin_single_layer_on_a_circle():
     assert 2.0 < rate[2] < 3.5, rate
     assert rate[6] > 5.5, rate
     assert errs[6, 512] < 1e-6, errs
+
+
+def test_rank_mlp_restatement_gradient_by_central_differences():
+    """The float64 rank-model restatement: its analytic gradient against
+    central differences of its own loss on coordinates of every layer, and
+    the closed form of the all-zero network (output = b4, GELU(0) = 0)."""
+    from oracle import rank_mlp as R
+    rng = np.random.default_rng(7)
+    p = 0.3 * rng.standard_normal(R.NPARAM)
+    X = rng.standard_normal((40, 6))
+    y = rng.standard_normal(40)
+    _, g = R.loss_grad(p, X, y)
+    offs = np.cumsum([0] + [int(np.prod(s)) for s in R.SHAPES])
+    h = 1e-6
+    for lo, hi in zip(offs[:-1], offs[1:]):
+        for i in rng.choice(np.arange(lo, hi), size=min(6, hi - lo), replace=False):
+            e = np.zeros(R.NPARAM)
+            e[i] = h
+            fd = (R.loss(p + e, X, y) - R.loss(p - e, X, y)) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
+    z = np.zeros(R.NPARAM)
+    z[-1] = 0.7
+    l, g0 = R.loss_grad(z, X, y)
+    assert np.isclose(l, np.mean((0.7 - y) ** 2)) and np.isclose(g0[-1], 2 * np.mean(0.7 - y))
+    assert np.all(g0[:-1] == 0)
