@@ -984,7 +984,8 @@ __global__ void k_f32_to_f64(int64_t n, const float2 *__restrict__ a, double2 *_
 /* ------------------------------------------------ batched one-sided SVD */
 
 /* Singular value decomposition of a batch of m x n complex matrices
- * (column-major, m >= 1, 1 <= n <= 256), one 256-thread block each, in
+ * (column-major, m >= 1, n >= 1; the host keeps n <= 1024, as the work grows
+ * with n^2), one 256-thread block each, in
  * place: one-sided (Hestenes) Jacobi rotates pairs of columns until they
  * are orthogonal, A V = W with orthogonal columns w_j = sigma_j u_j.  On
  * return A holds W, V (n x n, column-major) the accumulated rotations and
@@ -1078,6 +1079,105 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const int64_t *__restrict__ 
     for (int o = 1; o < g; o <<= 1) q += __shfl_xor(q, o, 64);
     if (sub == 0) s[j] = sqrt(q);
   }
+}
+
+/* The same Jacobi for large matrices, spread over the chip: one launch per
+ * round-robin step, workgroup (p, b) rotates pair p of big matrix b with
+ * 256 threads over the rows (block reductions), so a step of a matrix with
+ * 300 pairs uses 300 workgroups instead of one.  Pairs, order, rotation
+ * and stopping rule are those of k_svd_jacobi; a matrix whose `active`
+ * flag is 0 (no rotation in its previous sweep) is skipped, and every
+ * rotation sets rot[b]. */
+__global__ __launch_bounds__(256) void k_svd_step(const int32_t *__restrict__ big, const int64_t *__restrict__ aOff,
+                                                  const int32_t *__restrict__ mm, const int32_t *__restrict__ nn,
+                                                  double2 *__restrict__ A, const int64_t *__restrict__ vOff,
+                                                  double2 *__restrict__ V, int step,
+                                                  const int32_t *__restrict__ active, int32_t *__restrict__ rot) {
+  __shared__ double red[4][4];
+  const int bi = blockIdx.y;
+  if (!active[bi]) return;
+  const int b = big[bi];
+  const int m = mm[b], n = nn[b];
+  const int P = n + (n & 1), npairs = P / 2, pr = blockIdx.x;
+  if (n < 2 || step >= P - 1 || pr >= npairs) return;
+  const int p0 = rr_row(pr, step, P), p1 = rr_row(P - 1 - pr, step, P);
+  const int i = min(p0, p1), j = max(p0, p1);
+  if (j >= n) return;
+  double2 *ci = A + aOff[b] + (int64_t)i * m, *cj = A + aOff[b] + (int64_t)j * m;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double al = 0, be = 0, gx = 0, gy = 0;
+  for (int r = tid; r < m; r += 256) {
+    const double2 x = ci[r], y = cj[r];
+    al = fma(x.x, x.x, fma(x.y, x.y, al));
+    be = fma(y.x, y.x, fma(y.y, y.y, be));
+    gx = fma(x.x, y.x, fma(x.y, y.y, gx));
+    gy = fma(x.x, y.y, fma(-x.y, y.x, gy));
+  }
+  al = group_sum(al, 64);
+  be = group_sum(be, 64);
+  gx = group_sum(gx, 64);
+  gy = group_sum(gy, 64);
+  if (lane == 0) {
+    red[w][0] = al;
+    red[w][1] = be;
+    red[w][2] = gx;
+    red[w][3] = gy;
+  }
+  __syncthreads();
+  al = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+  be = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+  gx = red[0][2] + red[1][2] + red[2][2] + red[3][2];
+  gy = red[0][3] + red[1][3] + red[2][3] + red[3][3];
+  const double eps = 2.220446049250313e-16;
+  const double ag = sqrt(gx * gx + gy * gy);
+  if (!(ag > 0 && ag > eps * sqrt(al * be))) return;
+  if (tid == 0) rot[bi] = 1;
+  const double ex = gx / ag, ey = -gy / ag;
+  const double zeta = (be - al) / (2 * ag);
+  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+  const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+  for (int r = tid; r < m; r += 256) {
+    const double2 x = ci[r], y = cj[r];
+    const double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x;
+    ci[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
+    cj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
+  }
+  double2 *vi = V + vOff[b] + (int64_t)i * n, *vj = V + vOff[b] + (int64_t)j * n;
+  for (int r = tid; r < n; r += 256) {
+    const double2 x = vi[r], y = vj[r];
+    const double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x;
+    vi[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
+    vj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
+  }
+}
+
+/* V = I and, after the sweeps, sig[j] = |column j of A|, for big matrix
+ * blockIdx.y (one workgroup row per matrix) */
+__global__ __launch_bounds__(256) void k_svd_big_init(const int32_t *__restrict__ big, const int32_t *__restrict__ nn,
+                                                      const int64_t *__restrict__ vOff, double2 *__restrict__ V) {
+  const int b = big[blockIdx.y];
+  const int n = nn[b];
+  const int64_t cnt = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * 256)
+    V[vOff[b] + e] = make_double2((e % (n + 1)) == 0 ? 1.0 : 0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_svd_big_norms(const int32_t *__restrict__ big,
+                                                       const int64_t *__restrict__ aOff,
+                                                       const int32_t *__restrict__ mm, const int32_t *__restrict__ nn,
+                                                       const double2 *__restrict__ A,
+                                                       const int64_t *__restrict__ sOff, double *__restrict__ sig) {
+  __shared__ double red[4];
+  const int b = big[blockIdx.y];
+  const int m = mm[b], n = nn[b], c = blockIdx.x;
+  if (c >= n) return;
+  const double2 *col = A + aOff[b] + (int64_t)c * m;
+  double q = 0;
+  for (int r = threadIdx.x; r < m; r += 256) q = fma(col[r].x, col[r].x, fma(col[r].y, col[r].y, q));
+  q = group_sum(q, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = q;
+  __syncthreads();
+  if (threadIdx.x == 0) sig[sOff[b] + c] = sqrt(red[0] + red[1] + red[2] + red[3]);
 }
 
 /* ---------------------------------------------------- orthogonalization */
@@ -2050,6 +2150,60 @@ int bfk_svd_jacobi(int64_t batch, const int64_t *dAOff, const int32_t *dM, const
   }
   BFK_CHECK(hipStreamSynchronize(g_stream));
   return 0;
+}
+
+/* the big matrices big[0 .. nb) of a batch (indices into the tables):
+ * sweeps of per-step launches until a sweep rotates nothing in any of
+ * them; maxN is their largest column count */
+int bfk_svd_jacobi_big(int64_t nb, const int32_t *dBig, const int64_t *dAOff, const int32_t *dM, const int32_t *dN,
+                       double *dA, const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig,
+                       int maxSweeps, int maxN) {
+  if (bfk_init()) return -1;
+  if (nb <= 0) return 0;
+  if (nb > 65535) return -3;
+  int32_t *dAct = nullptr, *dRot = nullptr;
+  BFK_CHECK(hipMalloc((void **)&dAct, (size_t)nb * 4));
+  BFK_CHECK(hipMalloc((void **)&dRot, (size_t)nb * 4));
+  int32_t *act = (int32_t *)malloc((size_t)nb * 4), *rot = (int32_t *)malloc((size_t)nb * 4);
+  int rc = (act && rot) ? 0 : -2;
+  for (int64_t i = 0; i < nb && !rc; ++i) act[i] = 1;
+  const int P = maxN + (maxN & 1);
+  const unsigned vBlocks = (unsigned)((((int64_t)maxN * maxN) + 255) / 256 < 1024 ? (((int64_t)maxN * maxN) + 255) / 256 : 1024);
+  if (!rc) {
+    hipLaunchKernelGGL(k_svd_big_init, dim3(vBlocks > 0 ? vBlocks : 1, (unsigned)nb), dim3(256), 0, g_stream, dBig,
+                       dN, dVOff, (double2 *)dV);
+    if (hipGetLastError() != hipSuccess) rc = -1;
+  }
+  for (int sweep = 0; sweep < maxSweeps && !rc && P > 2; ++sweep) {
+    int any = 0;
+    for (int64_t i = 0; i < nb; ++i) any |= act[i];
+    if (!any) break;
+    if (hipMemcpyAsync(dAct, act, (size_t)nb * 4, hipMemcpyHostToDevice, g_stream) != hipSuccess ||
+        hipMemsetAsync(dRot, 0, (size_t)nb * 4, g_stream) != hipSuccess) {
+      rc = -1;
+      break;
+    }
+    for (int step = 0; step < P - 1; ++step)
+      hipLaunchKernelGGL(k_svd_step, dim3((unsigned)(P / 2), (unsigned)nb), dim3(256), 0, g_stream, dBig, dAOff, dM,
+                         dN, (double2 *)dA, dVOff, (double2 *)dV, step, dAct, dRot);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(rot, dRot, (size_t)nb * 4, hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+        hipStreamSynchronize(g_stream) != hipSuccess) {
+      rc = -1;
+      break;
+    }
+    for (int64_t i = 0; i < nb; ++i) act[i] = act[i] && rot[i];
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_svd_big_norms, dim3((unsigned)maxN, (unsigned)nb), dim3(256), 0, g_stream, dBig, dAOff, dM,
+                       dN, (const double2 *)dA, dSOff, dSig);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g_stream) != hipSuccess) rc = -1;
+  }
+  free(act);
+  free(rot);
+  (void)hipFree(dAct);
+  (void)hipFree(dRot);
+  return rc;
 }
 
 size_t bfk_ortho_units(int rows, int n) { return ortho_units(rows, n); }

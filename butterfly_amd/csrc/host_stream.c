@@ -189,6 +189,56 @@ static int cmp_sig_desc(const void *a, const void *b) {
   return x->j - y->j;
 }
 
+/* the largest smaller side of a matrix the batched SVD takes (its cost
+ * grows as the square of the column count) */
+#define SVD_MAX_COLS 1024
+/* matrices of at least this many entries get the whole chip per SVD */
+#define SVD_BIG_ENTRIES (1 << 15)
+
+/* One-sided Jacobi SVDs of a batch whose tables (host copies h*, device
+ * copies d*) are in place: large matrices take the per-step launches over
+ * the whole chip, the rest one workgroup each in one launch.  The device
+ * tables are overwritten (with the small matrices' entries). */
+static int jacobi_dispatch(int64_t nj, const int64_t *aOff, const int32_t *M, const int32_t *N, const int64_t *vOff,
+                           const int64_t *sOff, void *dAo, void *dM, void *dN, void *dVo, void *dSo, void *dA,
+                           void *dV, void *dS) {
+  int32_t *bigIdx = malloc(nz(nj) * sizeof(int32_t));
+  int64_t *sA = malloc(nz(nj) * 8), *sV = malloc(nz(nj) * 8), *sS = malloc(nz(nj) * 8);
+  int32_t *sM = malloc(nz(nj) * 4), *sN = malloc(nz(nj) * 4);
+  int64_t nbig = 0, nsmall = 0;
+  int maxN = 0, rc = 0;
+  for (int64_t i = 0; i < nj; ++i) {
+    if ((int64_t)M[i] * N[i] >= SVD_BIG_ENTRIES && N[i] >= 2) {
+      bigIdx[nbig++] = (int32_t)i;
+      if (N[i] > maxN) maxN = N[i];
+    } else {
+      sA[nsmall] = aOff[i];
+      sV[nsmall] = vOff[i];
+      sS[nsmall] = sOff[i];
+      sM[nsmall] = M[i];
+      sN[nsmall] = N[i];
+      ++nsmall;
+    }
+  }
+  if (nbig) {
+    void *dB = bfk_malloc((size_t)nbig * 4);
+    if (!dB) rc = -2;
+    if (!rc) rc = bfk_h2d(dB, bigIdx, (size_t)nbig * 4);
+    if (!rc) rc = bfk_svd_jacobi_big(nbig, dB, dAo, dM, dN, dA, dVo, dV, dSo, dS, 60, maxN);
+    bfk_free(dB);
+    if (!rc && nsmall) { /* the small ones' tables */
+      rc = bfk_h2d(dAo, sA, (size_t)nsmall * 8);
+      if (!rc) rc = bfk_h2d(dVo, sV, (size_t)nsmall * 8);
+      if (!rc) rc = bfk_h2d(dSo, sS, (size_t)nsmall * 8);
+      if (!rc) rc = bfk_h2d(dM, sM, (size_t)nsmall * 4);
+      if (!rc) rc = bfk_h2d(dN, sN, (size_t)nsmall * 4);
+    }
+  }
+  if (!rc && nsmall) rc = bfk_svd_jacobi(nsmall, dAo, dM, dN, dA, dVo, dV, dSo, dS, 60);
+  free(bigIdx); free(sA); free(sV); free(sS); free(sM); free(sN);
+  return rc;
+}
+
 /* All jobs in one launch.  Wide blocks (m < n) are decomposed through
  * their conjugate transpose so the kernel always sees n <= m columns. */
 static int svd_batch(SvdJob *jobs, size_t nj, double tol) {
@@ -200,7 +250,7 @@ static int svd_batch(SvdJob *jobs, size_t nj, double tol) {
   for (size_t i = 0; i < nj; ++i) {
     int wide = jobs[i].m < jobs[i].n;
     int64_t m = wide ? jobs[i].n : jobs[i].m, n = wide ? jobs[i].m : jobs[i].n;
-    if (n > 256) { /* the kernel handles up to 256 columns */
+    if (n > SVD_MAX_COLS) {
       free(aOff); free(vOff); free(sOff); free(M); free(N);
       return -3;
     }
@@ -243,7 +293,7 @@ static int svd_batch(SvdJob *jobs, size_t nj, double tol) {
   if (!rc) rc = bfk_h2d(dSo, sOff, nj * 8);
   if (!rc) rc = bfk_h2d(dM, M, nj * 4);
   if (!rc) rc = bfk_h2d(dN, N, nj * 4);
-  if (!rc) rc = bfk_svd_jacobi((int64_t)nj, dAo, dM, dN, dA, dVo, dV, dSo, dS, 60);
+  if (!rc) rc = jacobi_dispatch((int64_t)nj, aOff, M, N, vOff, sOff, dAo, dM, dN, dVo, dSo, dA, dV, dS);
   if (!rc) rc = bfk_d2h(hA, dA, (size_t)totA * 16);
   if (!rc) rc = bfk_d2h(hV, dV, (size_t)totV * 16);
   if (!rc) rc = bfk_d2h(hS, dS, (size_t)totS * 8);
@@ -341,7 +391,7 @@ typedef struct {
  * accepted when ||A - Q B||_F <= tol sigma_0 (so every discarded singular
  * value of A is below the cut) and fewer than w terms pass the cut (the
  * width did not saturate); a rejected sketch (fell[i] = 1) and every job
- * with width 0, width >= min(m, n) or width > 256 take the full SVD.  The
+ * with width 0, width >= min(m, n) or width > SVD_MAX_COLS take the full SVD.  The
  * rule for k is svd_batch's (linalg.c:26-35). */
 static int svd_batch_sketch(SvdJob *jobs, size_t nj, double tol, const int64_t *width, uint64_t seed, int *fell) {
   if (nj == 0) return 0;
@@ -351,7 +401,7 @@ static int svd_batch_sketch(SvdJob *jobs, size_t nj, double tol, const int64_t *
   for (size_t i = 0; i < nj; ++i) {
     fell[i] = 0;
     const int64_t mn = jobs[i].m < jobs[i].n ? jobs[i].m : jobs[i].n;
-    if (width[i] > 0 && width[i] < mn && width[i] <= 256) sk[ns++] = i;
+    if (width[i] > 0 && width[i] < mn && width[i] <= SVD_MAX_COLS) sk[ns++] = i;
     else full[nf++] = i;
   }
   int rc = 0;
@@ -409,9 +459,13 @@ static int svd_batch_sketch(SvdJob *jobs, size_t nj, double tol, const int64_t *
     if (!rc) rc = bfk_h2d(dN, N, ns * 4);
     if (!rc) rc = bfk_h2d(dW, Wd, ns * 4);
     if (!rc) rc = bfk_sketch_range((int64_t)ns, dJ, seed, dA, dO, dY);
-    if (!rc) rc = bfk_svd_jacobi((int64_t)ns, dYo, dM, dW, dY, dV1, dV, dS1, dS, 60);
+    if (!rc) rc = jacobi_dispatch((int64_t)ns, yOff, M, Wd, v1, s1, dYo, dM, dW, dV1, dS1, dY, dV, dS);
+    /* the dispatcher left the small matrices' tables in place: the
+     * projection reads the sigma offsets of every matrix */
+    if (!rc) rc = bfk_h2d(dS1, s1, ns * 8);
+    if (!rc) rc = bfk_h2d(dW, Wd, ns * 4);
     if (!rc) rc = bfk_sketch_project((int64_t)ns, dJ, dS1, dS, dA, dY, dB, dR);
-    if (!rc) rc = bfk_svd_jacobi((int64_t)ns, dBo, dN, dW, dB, dV2, dV, dS2, dS, 60);
+    if (!rc) rc = jacobi_dispatch((int64_t)ns, bOff, N, Wd, v2, s2, dBo, dN, dW, dV2, dS2, dB, dV, dS);
     if (!rc) rc = bfk_d2h(hY, dY, (size_t)tY * 16);
     if (!rc) rc = bfk_d2h(hB, dB, (size_t)tB * 16);
     if (!rc) rc = bfk_d2h(hV, dV, (size_t)tV * 16);
@@ -544,13 +598,17 @@ static int svd_round(RankCtx *rk, const BfFacSpec *spec, TreeView rt, int32_t co
     int *fell = malloc(nj * sizeof(int));
     rc = bfhipRankModelPredict(rk->model, nj, f, pr);
     for (size_t i = 0; i < nj && !rc; ++i) {
+      /* a sketch costs two Jacobi SVDs of w columns (on m and n rows) and
+       * three m n w products: it pays only when w is well below min(m, n) */
       const double p = pr[i] > 0 ? ceil((double)pr[i]) : 0;
+      const int64_t mn = jobs[i].m < jobs[i].n ? jobs[i].m : jobs[i].n;
       wd[i] = (int64_t)p + rk->oversample;
+      if (2 * wd[i] > mn) wd[i] = 0;
     }
     if (!rc) rc = svd_batch_sketch(jobs, nj, spec->tol, wd, rk->seed++, fell);
     for (size_t i = 0; i < nj && !rc; ++i) {
       const int64_t mn = jobs[i].m < jobs[i].n ? jobs[i].m : jobs[i].n;
-      if (wd[i] < mn && wd[i] <= 256) rk->numSketched += 1;
+      if (wd[i] > 0 && wd[i] < mn && wd[i] <= SVD_MAX_COLS) rk->numSketched += 1;
       rk->numFell += fell[i];
     }
     free(pr);

@@ -537,8 +537,9 @@ int bfhipSvdBatched(BfSize batch, BfSize const *m, BfSize const *n, BfComplex co
  * sketch is accepted when ||A - Q Q^H A||_F <= tol sigma_0 and fewer than
  * width_b terms pass the cut; otherwise, or when width_b is 0 or not
  * below min(m_b, n_b), the matrix takes the full SVD (fellBack_b = 1 for a
- * rejected sketch).  The sketch lifts the 256-column limit of the full SVD
- * for widths up to 256.  `seed` fixes Omega. */
+ * rejected sketch).  The full SVD takes matrices whose smaller side is at
+ * most 1024; a sketch needs only its width within that.  `seed` fixes
+ * Omega. */
 int bfhipSvdBatchedSketch(BfSize batch, BfSize const *m, BfSize const *n, BfComplex const *A, BfReal tol,
                           BfSize const *width, uint64_t seed, BfSize *k, BfComplex *U, BfComplex *W, int *fellBack);
 

@@ -21,7 +21,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("m,n", [(40, 8), (300, 32), (17, 64), (128, 128), (5, 1), (1000, 20)])
+@pytest.mark.parametrize("m,n", [(40, 8), (300, 32), (17, 64), (128, 128), (5, 1), (1000, 20), (600, 300)])
 def test_svd_batched_matches_numpy(gpu_lib, m, n):
     bf = gpu_lib
     rng = np.random.default_rng(m * 1000 + n)
@@ -42,7 +42,8 @@ def test_svd_batched_matches_numpy(gpu_lib, m, n):
             sk = np.linalg.norm(W, axis=1)
             assert np.allclose(sk, s[:k], rtol=1e-11, atol=1e-14 * s[0])
             err = np.linalg.norm(U @ W - A, 2)
-            assert err <= (s[k] if k < len(s) else 0) * 1.0001 + 1e-13 * s[0]
+            # rounding of the rotations grows with the number of columns
+            assert err <= (s[k] if k < len(s) else 0) * 1.0001 + 1e-13 * max(1, n / 64) * s[0]
 
 
 def _nodes(tree):
