@@ -39,7 +39,14 @@ def curve_problem(n, ppw=10):
     return src, tgt, 2 * np.pi * n / (ppw * CURVE_L)
 
 
-def dense_rows_threaded(k, tgt_rows, src, x, threads):
+# ---- the CPU leg: the oracle (the C restatement of the reference, test
+# infrastructure) computes the exact rows every accuracy figure is checked
+# against, and is timed as the CPU baseline (cpu_baseline below).  Nothing
+# timed on the GPU runs through it.
+
+def cpu_baseline_exact_rows(k, tgt_rows, src, x, threads):
+    """Exact FP64 rows of the single-layer kernel matrix times x
+    (Neumaier-summed, oracle), on `threads` host threads."""
     import oracle as orc
     chunks = np.array_split(np.arange(len(tgt_rows)), threads)
     out = np.zeros(len(tgt_rows), complex)
@@ -147,7 +154,6 @@ def batch_section(bf, count, nrhs, steps, rows):
     Yseq = np.concatenate([dy.to_numpy(np.complex128, (len(t_), nrhs)) for dy, (_, t_, _, _) in zip(ys, probs)])
     err_rows = None
     if rows:
-        import oracle as orc
         errs = []
         for i in range(0, count, max(1, count // 4)):
             s_, t_, k, _ = probs[i]
@@ -156,7 +162,7 @@ def batch_section(bf, count, nrhs, steps, rows):
             x = np.empty(len(s_), complex)
             c0 = sum(len(p_[0]) for p_ in probs[:i])
             x[ts.perm] = X[c0:c0 + len(s_), 0]
-            yd = orc.dense_rows(k, t_[r], s_, x)
+            yd = cpu_baseline_exact_rows(k, t_[r], s_, x, 4)
             y = np.empty(len(t_), complex)
             r0 = sum(len(p_[1]) for p_ in probs[:i])
             y[tt.perm] = Yseq[r0:r0 + len(t_), 0]
@@ -219,6 +225,70 @@ def batch_section(bf, count, nrhs, steps, rows):
     return out
 
 
+def stream_section(bf, n, tol, oversample, epochs, rows, threads):
+    """Streaming factorization (bfFacStreamer*) of n sources U[0,1]^2 ->
+    n targets U[2,3]x[0,1] (seed 0), k for 8 points per wavelength, leaf 64,
+    tol `tol`, min rows / columns 20, the column blocks sampled on the device
+    beforehand: plain (every truncated SVD in full) against rank-predicted
+    (randomized SVDs of width predicted rank + `oversample` where that is
+    under half the block's smaller side, full SVD as fallback), the model
+    trained on the (geometry, numerical rank) samples the plain run recorded.
+    Times are the feed loops only; errors against exact rows."""
+    rng = np.random.default_rng(0)
+    src = rng.uniform(0, 1, (n, 2))
+    tgt = rng.uniform(0, 1, (n, 2))
+    tgt[:, 0] += 2
+    k = 2 * np.pi * np.sqrt(n) / 8
+    ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
+    qs = bf.QuadtreeStruct.from_address(ts.ptr)
+    nodes = [(int(a.offset[0]), int(a.offset[4]))
+             for a in (bf.QuadtreeNodeStruct * qs.numNodes).from_address(qs.nodes)]
+    sp, tp = src[ts.perm], tgt[tt.perm]
+    helm = bf.Helm2(k)
+    blocks = {}
+
+    def stream(model):
+        S = bf.FacStreamer(tt, ts, tol, init_depth=1, min_rows=20, min_cols=20)
+        S.set_rank_model(model, k, oversample)
+        t = 0.0
+        while not S.done:
+            c = S.current_column_node()
+            if c not in blocks:
+                a, e = nodes[c]
+                blocks[c] = helm.get_kernel_matrix(sp[a:e], tp)
+            t0 = time.perf_counter()
+            S.feed(blocks[c])
+            t += time.perf_counter() - t0
+        return S, t
+
+    stream(None)  # first pass: samples the blocks and warms the kernels
+    S0, t_plain = stream(None)
+    X, y = S0.rank_samples()
+    model = bf.RankModel(1)
+    t0 = time.perf_counter()
+    mse = model.train(X, y, epochs=epochs, batch=512, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    mae = float(np.abs(model.predict(X) - y).mean())
+    S1, t_model = stream(model)
+    st = S1.sketch_stats()
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    errs = [None, None]
+    if rows:
+        r = np.sort(rng.choice(n, size=min(rows, n), replace=False))
+        yd = cpu_baseline_exact_rows(k, tp[r], sp, x, threads)
+        errs = [float(np.linalg.norm((S.fac().mat() @ x)[r] - yd) / np.linalg.norm(yd)) for S in (S0, S1)]
+    print(f"[bench] stream N={n} k={k:.1f} tol={tol:g}: plain {t_plain:.2f}s ({len(y)} SVDs), rank model "
+          f"trained on its samples in {t_train:.2f}s (MAE {mae:.2f}), rank-predicted {t_model:.2f}s "
+          f"({st['sketched']} sketched, {st['fell_back']} fell back); errors {errs[0]} / {errs[1]}",
+          file=sys.stderr, flush=True)
+    return {"workload": f"bfFacStreamer: {n} sources U[0,1]^2 -> {n} targets U[2,3]x[0,1], k={k:.1f} (8 ppw), "
+                        f"leaf 64, tol {tol:g}, min rows/cols 20",
+            "N": n, "plain_s": t_plain, "rank_predicted_s": t_model, "speedup": t_plain / t_model,
+            "svds": st["svds"], "sketched": st["sketched"], "fell_back": st["fell_back"],
+            "train_samples": int(len(y)), "train_s": t_train, "train_mse": mse, "train_mae": mae,
+            "plain_rel_err_vs_rows": errs[0], "rank_predicted_rel_err_vs_rows": errs[1]}
+
+
 def cpu_baseline(n_sample, tol, rank_const, rcond, n_full_entries):
     """Oracle (single-threaded C restatement) factor + apply on the same
     geometry at n_sample points; the apply rate is scaled to the full size
@@ -274,6 +344,7 @@ def main():
     ap.add_argument("--gmres-tol", type=float, default=1e-10)
     ap.add_argument("--batch", type=int, default=64, help="independent-problems section size (0 = skip)")
     ap.add_argument("--batch-rhs", type=int, default=16)
+    ap.add_argument("--stream-n", type=int, default=8192, help="streaming / rank-model section size (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4096)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -338,7 +409,7 @@ def main():
         rng = np.random.default_rng(1)
         rows = np.sort(rng.choice(n, size=min(args.rows, n), replace=False))
         t0 = time.perf_counter()
-        yd = dense_rows_threaded(k, tgt[rows], src, x, args.threads)
+        yd = cpu_baseline_exact_rows(k, tgt[rows], src, x, args.threads)
         rel_err = float(np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd))
         print(f"[bench] rel err vs {len(rows)} exact rows: {rel_err:.3e} ({time.perf_counter() - t0:.1f}s)",
               file=sys.stderr, flush=True)
@@ -435,6 +506,10 @@ def main():
     if args.batch:
         bt = batch_section(bf, args.batch, args.batch_rhs, max(1, args.mixed_steps), 256 if args.rows else 0)
 
+    sm = None
+    if args.stream_n:
+        sm = stream_section(bf, args.stream_n, args.tol, 8, 200, 256 if args.rows else 0, args.threads)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.rank_const, args.pinv_rcond, st["entries"])
@@ -474,6 +549,7 @@ def main():
         "mixed": mixed,
         "gmres": gm,
         "batch": bt,
+        "stream": sm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

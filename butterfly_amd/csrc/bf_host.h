@@ -22,9 +22,11 @@ int bfk_mlp_grad(const float *dP, const float *dX, const float *dY, const int32_
                  float *dLossPart);
 int bfk_mlp_predict(const float *dP, const float *dX, int64_t n, float *dOut);
 int bfk_sketch_jobs_bytes(void);
-int bfk_sketch_range(int64_t batch, const void *dJobs, uint64_t seed, const double *dA, double *dOm, double *dY);
+int bfk_sketch_resid_chunks(int64_t maxEntries);
+int bfk_sketch_range(int64_t batch, const void *dJobs, uint64_t seed, const double *dA, double *dOm, double *dY,
+                     int64_t maxMW);
 int bfk_sketch_project(int64_t batch, const void *dJobs, const int64_t *dSOff, const double *dSig, const double *dA,
-                       double *dY, double *dBH, double *dRes);
+                       double *dY, double *dBH, double *dPart, int64_t maxMW, int64_t maxNW, int64_t maxMN);
 size_t bfk_lstsq_lds_bytes(int p, int n);
 void *bfk_malloc(size_t bytes);
 void bfk_free(void *p);

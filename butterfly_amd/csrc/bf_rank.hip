@@ -337,72 +337,81 @@ __global__ __launch_bounds__(256) void k_sketch_omega(const BfkSketchJob *__rest
   }
 }
 
-/* Y = A Omega */
+/* Y = A Omega: grid (chunks of 256 entries of Y, matrices) */
 __global__ __launch_bounds__(256) void k_sketch_mul(const BfkSketchJob *__restrict__ jobs,
                                                     const double2 *__restrict__ A, const double2 *__restrict__ Om,
                                                     double2 *__restrict__ Y) {
-  const BfkSketchJob J = jobs[blockIdx.x];
-  const int64_t cnt = (int64_t)J.m * J.w;
-  for (int64_t e = threadIdx.x; e < cnt; e += 256) {
-    const int c = (int)(e / J.m), r = (int)(e - (int64_t)c * J.m);
-    const double2 *a = A + J.aOff + r, *o = Om + J.oOff + (int64_t)c * J.n;
-    double sx = 0, sy = 0;
-    for (int j = 0; j < J.n; ++j) {
-      const double2 x = a[(int64_t)j * J.m], y = o[j];
-      sx = fma(x.x, y.x, fma(-x.y, y.y, sx));
-      sy = fma(x.x, y.y, fma(x.y, y.x, sy));
-    }
-    Y[J.yOff + e] = make_double2(sx, sy);
+  const BfkSketchJob J = jobs[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)J.m * J.w) return;
+  const int c = (int)(e / J.m), r = (int)(e - (int64_t)c * J.m);
+  const double2 *a = A + J.aOff + r, *o = Om + J.oOff + (int64_t)c * J.n;
+  double sx = 0, sy = 0;
+  for (int j = 0; j < J.n; ++j) {
+    const double2 x = a[(int64_t)j * J.m], y = o[j];
+    sx = fma(x.x, y.x, fma(-x.y, y.y, sx));
+    sy = fma(x.x, y.y, fma(x.y, y.x, sy));
   }
+  Y[J.yOff + e] = make_double2(sx, sy);
 }
 
 /* Q = the columns sigma_c u_c of Y (after one-sided Jacobi) divided by
  * sigma_c; directions below 1e-14 sigma_max carry no information about A
- * and are set to zero */
+ * and are set to zero.  Grid (chunks of 256 entries, matrices). */
 __global__ __launch_bounds__(256) void k_sketch_q(const BfkSketchJob *__restrict__ jobs,
                                                   const int64_t *__restrict__ sOff, const double *__restrict__ sig,
                                                   double2 *__restrict__ Y) {
-  const BfkSketchJob J = jobs[blockIdx.x];
-  const double *s = sig + sOff[blockIdx.x];
+  const BfkSketchJob J = jobs[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)J.m * J.w) return;
+  const double *s = sig + sOff[blockIdx.y];
   double smax = 0;
   for (int c = 0; c < J.w; ++c) smax = fmax(smax, s[c]);
-  const int64_t cnt = (int64_t)J.m * J.w;
-  for (int64_t e = threadIdx.x; e < cnt; e += 256) {
-    const int c = (int)(e / J.m);
-    const double sc = s[c] > 1e-14 * smax ? 1.0 / s[c] : 0.0;
-    const double2 y = Y[J.yOff + e];
-    Y[J.yOff + e] = make_double2(y.x * sc, y.y * sc);
-  }
+  const int c = (int)(e / J.m);
+  const double sc = s[c] > 1e-14 * smax ? 1.0 / s[c] : 0.0;
+  const double2 y = Y[J.yOff + e];
+  Y[J.yOff + e] = make_double2(y.x * sc, y.y * sc);
 }
 
-/* B^H = A^H Q (n x w) */
+/* B^H = A^H Q (n x w): one wave per entry, the lanes over the rows of A and
+ * Q (coalesced), a wave sum; grid (entries / 4, matrices) */
 __global__ __launch_bounds__(256) void k_sketch_mulh(const BfkSketchJob *__restrict__ jobs,
                                                      const double2 *__restrict__ A, const double2 *__restrict__ Q,
                                                      double2 *__restrict__ BH) {
-  const BfkSketchJob J = jobs[blockIdx.x];
-  const int64_t cnt = (int64_t)J.n * J.w;
-  for (int64_t e = threadIdx.x; e < cnt; e += 256) {
-    const int c = (int)(e / J.n), j = (int)(e - (int64_t)c * J.n);
-    const double2 *a = A + J.aOff + (int64_t)j * J.m, *q = Q + J.yOff + (int64_t)c * J.m;
-    double sx = 0, sy = 0;
-    for (int r = 0; r < J.m; ++r) {
-      const double2 x = a[r], y = q[r]; /* conj(x) y */
-      sx = fma(x.x, y.x, fma(x.y, y.y, sx));
-      sy = fma(x.x, y.y, fma(-x.y, y.x, sy));
-    }
-    BH[J.bOff + e] = make_double2(sx, sy);
+  const BfkSketchJob J = jobs[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= (int64_t)J.n * J.w) return;
+  const int lane = threadIdx.x & 63;
+  const int c = (int)(e / J.n), j = (int)(e - (int64_t)c * J.n);
+  const double2 *a = A + J.aOff + (int64_t)j * J.m, *q = Q + J.yOff + (int64_t)c * J.m;
+  double sx = 0, sy = 0;
+  for (int r = lane; r < J.m; r += 64) {
+    const double2 x = a[r], y = q[r]; /* conj(x) y */
+    sx = fma(x.x, y.x, fma(x.y, y.y, sx));
+    sy = fma(x.x, y.y, fma(-x.y, y.x, sy));
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    sx += __shfl_xor(sx, o, 64);
+    sy += __shfl_xor(sy, o, 64);
+  }
+  if (lane == 0) BH[J.bOff + e] = make_double2(sx, sy);
 }
 
-/* res[b] = ||A - Q (B^H)^H||_F^2 */
+#define RESID_CHUNK 4096 /* entries of A per workgroup of the residual */
+
+/* part[b * maxChunks + chunk] = the squared entries of A - Q (B^H)^H in
+ * chunk `chunk` of matrix b (the host sums them in order) */
 __global__ __launch_bounds__(256) void k_sketch_resid(const BfkSketchJob *__restrict__ jobs,
                                                       const double2 *__restrict__ A, const double2 *__restrict__ Q,
-                                                      const double2 *__restrict__ BH, double *__restrict__ res) {
-  __shared__ double red[256];
-  const BfkSketchJob J = jobs[blockIdx.x];
-  const int64_t cnt = (int64_t)J.m * J.n;
+                                                      const double2 *__restrict__ BH, int maxChunks,
+                                                      double *__restrict__ part) {
+  __shared__ double red[4];
+  const BfkSketchJob J = jobs[blockIdx.y];
+  const int64_t cnt = (int64_t)J.m * J.n, e0 = (int64_t)blockIdx.x * RESID_CHUNK;
+  if (e0 >= cnt) return;
+  const int64_t e1 = e0 + RESID_CHUNK < cnt ? e0 + RESID_CHUNK : cnt;
   double acc = 0;
-  for (int64_t e = threadIdx.x; e < cnt; e += 256) {
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
     const int j = (int)(e / J.m), r = (int)(e - (int64_t)j * J.m);
     double2 d = A[J.aOff + e];
     for (int c = 0; c < J.w; ++c) {
@@ -413,13 +422,10 @@ __global__ __launch_bounds__(256) void k_sketch_resid(const BfkSketchJob *__rest
     }
     acc = fma(d.x, d.x, fma(d.y, d.y, acc));
   }
-  red[threadIdx.x] = acc;
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) res[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * maxChunks + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 /* ============================================================ launchers */
@@ -487,34 +493,44 @@ int bfk_mlp_predict(const float *dP, const float *dX, int64_t n, float *dOut) {
 
 int bfk_sketch_jobs_bytes(void) { return (int)sizeof(BfkSketchJob); }
 
-/* the range finder up to B^H: Omega, Y = A Omega, Jacobi of Y (by the
- * caller's SVD launcher, between the two calls), Q, B^H and the residual */
-int bfk_sketch_range(int64_t batch, const void *dJobs, uint64_t seed, const double *dA, double *dOm, double *dY) {
+int bfk_sketch_resid_chunks(int64_t maxEntries) { return (int)((maxEntries + RESID_CHUNK - 1) / RESID_CHUNK); }
+
+/* the range finder up to Y: Omega and Y = A Omega (maxMW = the largest
+ * m w of the batch) */
+int bfk_sketch_range(int64_t batch, const void *dJobs, uint64_t seed, const double *dA, double *dOm, double *dY,
+                     int64_t maxMW) {
   if (bfk_init()) return -1;
   if (batch <= 0) return 0;
+  if (batch > 65535) return -3;
   hipStream_t s = (hipStream_t)bfk_stream_handle();
   const BfkSketchJob *J = (const BfkSketchJob *)dJobs;
   hipLaunchKernelGGL(k_sketch_omega, dim3((unsigned)batch), dim3(256), 0, s, J, seed, (double2 *)dOm);
   RK_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_sketch_mul, dim3((unsigned)batch), dim3(256), 0, s, J, (const double2 *)dA,
-                     (const double2 *)dOm, (double2 *)dY);
+  hipLaunchKernelGGL(k_sketch_mul, dim3((unsigned)((maxMW + 255) / 256), (unsigned)batch), dim3(256), 0, s, J,
+                     (const double2 *)dA, (const double2 *)dOm, (double2 *)dY);
   RK_CHECK(hipGetLastError());
   return 0;
 }
 
+/* after the Jacobi of Y: Q, B^H and the residual partials (maxMW, maxNW,
+ * maxMN the largest m w, n w, m n of the batch; part holds batch x
+ * bfk_sketch_resid_chunks(maxMN) doubles) */
 int bfk_sketch_project(int64_t batch, const void *dJobs, const int64_t *dSOff, const double *dSig, const double *dA,
-                       double *dY, double *dBH, double *dRes) {
+                       double *dY, double *dBH, double *dPart, int64_t maxMW, int64_t maxNW, int64_t maxMN) {
   if (bfk_init()) return -1;
   if (batch <= 0) return 0;
+  if (batch > 65535) return -3;
   hipStream_t s = (hipStream_t)bfk_stream_handle();
   const BfkSketchJob *J = (const BfkSketchJob *)dJobs;
-  hipLaunchKernelGGL(k_sketch_q, dim3((unsigned)batch), dim3(256), 0, s, J, dSOff, dSig, (double2 *)dY);
+  hipLaunchKernelGGL(k_sketch_q, dim3((unsigned)((maxMW + 255) / 256), (unsigned)batch), dim3(256), 0, s, J, dSOff,
+                     dSig, (double2 *)dY);
   RK_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_sketch_mulh, dim3((unsigned)batch), dim3(256), 0, s, J, (const double2 *)dA,
-                     (const double2 *)dY, (double2 *)dBH);
+  hipLaunchKernelGGL(k_sketch_mulh, dim3((unsigned)((maxNW + 3) / 4), (unsigned)batch), dim3(256), 0, s, J,
+                     (const double2 *)dA, (const double2 *)dY, (double2 *)dBH);
   RK_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_sketch_resid, dim3((unsigned)batch), dim3(256), 0, s, J, (const double2 *)dA,
-                     (const double2 *)dY, (const double2 *)dBH, dRes);
+  const int chunks = bfk_sketch_resid_chunks(maxMN);
+  hipLaunchKernelGGL(k_sketch_resid, dim3((unsigned)chunks, (unsigned)batch), dim3(256), 0, s, J,
+                     (const double2 *)dA, (const double2 *)dY, (const double2 *)dBH, chunks, dPart);
   RK_CHECK(hipGetLastError());
   return 0;
 }

@@ -410,10 +410,13 @@ static int svd_batch_sketch(SvdJob *jobs, size_t nj, double tol, const int64_t *
     int64_t *yOff = malloc(ns * 8), *bOff = malloc(ns * 8), *v1 = malloc(ns * 8), *v2 = malloc(ns * 8),
             *s1 = malloc(ns * 8), *s2 = malloc(ns * 8);
     int32_t *M = malloc(ns * 4), *N = malloc(ns * 4), *Wd = malloc(ns * 4);
-    int64_t tA = 0, tO = 0, tY = 0, tB = 0, tV = 0, tS = 0;
+    int64_t tA = 0, tO = 0, tY = 0, tB = 0, tV = 0, tS = 0, maxMW = 0, maxNW = 0, maxMN = 0;
     for (size_t q = 0; q < ns; ++q) {
       const SvdJob *J = &jobs[sk[q]];
       const int64_t w = width[sk[q]];
+      if (J->m * w > maxMW) maxMW = J->m * w;
+      if (J->n * w > maxNW) maxNW = J->n * w;
+      if (J->m * J->n > maxMN) maxMN = J->m * J->n;
       SketchJob x = {(int32_t)J->m, (int32_t)J->n, (int32_t)w, 0, tA, tO, tY, tB};
       sj[q] = x;
       M[q] = (int32_t)J->m;
@@ -434,14 +437,15 @@ static int svd_batch_sketch(SvdJob *jobs, size_t nj, double tol, const int64_t *
     }
     BfComplex *hA = malloc(nz(tA) * sizeof(BfComplex)), *hY = malloc(nz(tY) * sizeof(BfComplex));
     BfComplex *hB = malloc(nz(tB) * sizeof(BfComplex)), *hV = malloc(nz(tV) * sizeof(BfComplex));
-    double *hS = malloc(nz(tS) * sizeof(double)), *hR = malloc(ns * sizeof(double));
+    const int chunks = bfk_sketch_resid_chunks(maxMN);
+    double *hS = malloc(nz(tS) * sizeof(double)), *hR = malloc(ns * (size_t)chunks * sizeof(double));
     for (size_t q = 0; q < ns; ++q) {
       const SvdJob *J = &jobs[sk[q]];
       memcpy(hA + sj[q].aOff, J->a, (size_t)(J->m * J->n) * sizeof(BfComplex));
     }
     void *dA = bfk_malloc(nz(tA) * 16), *dO = bfk_malloc(nz(tO) * 16), *dY = bfk_malloc(nz(tY) * 16),
          *dB = bfk_malloc(nz(tB) * 16), *dV = bfk_malloc(nz(tV) * 16), *dS = bfk_malloc(nz(tS) * 8),
-         *dR = bfk_malloc(ns * 8), *dJ = bfk_malloc(ns * sizeof(SketchJob)), *dYo = bfk_malloc(ns * 8),
+         *dR = bfk_malloc(ns * (size_t)chunks * 8), *dJ = bfk_malloc(ns * sizeof(SketchJob)), *dYo = bfk_malloc(ns * 8),
          *dBo = bfk_malloc(ns * 8), *dV1 = bfk_malloc(ns * 8), *dV2 = bfk_malloc(ns * 8), *dS1 = bfk_malloc(ns * 8),
          *dS2 = bfk_malloc(ns * 8), *dM = bfk_malloc(ns * 4), *dN = bfk_malloc(ns * 4), *dW = bfk_malloc(ns * 4);
     if (!dA || !dO || !dY || !dB || !dV || !dS || !dR || !dJ || !dYo || !dBo || !dV1 || !dV2 || !dS1 || !dS2 ||
@@ -458,19 +462,25 @@ static int svd_batch_sketch(SvdJob *jobs, size_t nj, double tol, const int64_t *
     if (!rc) rc = bfk_h2d(dM, M, ns * 4);
     if (!rc) rc = bfk_h2d(dN, N, ns * 4);
     if (!rc) rc = bfk_h2d(dW, Wd, ns * 4);
-    if (!rc) rc = bfk_sketch_range((int64_t)ns, dJ, seed, dA, dO, dY);
+    if (!rc) rc = bfk_sketch_range((int64_t)ns, dJ, seed, dA, dO, dY, maxMW);
     if (!rc) rc = jacobi_dispatch((int64_t)ns, yOff, M, Wd, v1, s1, dYo, dM, dW, dV1, dS1, dY, dV, dS);
     /* the dispatcher left the small matrices' tables in place: the
      * projection reads the sigma offsets of every matrix */
     if (!rc) rc = bfk_h2d(dS1, s1, ns * 8);
     if (!rc) rc = bfk_h2d(dW, Wd, ns * 4);
-    if (!rc) rc = bfk_sketch_project((int64_t)ns, dJ, dS1, dS, dA, dY, dB, dR);
+    if (!rc) rc = bfk_sketch_project((int64_t)ns, dJ, dS1, dS, dA, dY, dB, dR, maxMW, maxNW, maxMN);
     if (!rc) rc = jacobi_dispatch((int64_t)ns, bOff, N, Wd, v2, s2, dBo, dN, dW, dV2, dS2, dB, dV, dS);
     if (!rc) rc = bfk_d2h(hY, dY, (size_t)tY * 16);
     if (!rc) rc = bfk_d2h(hB, dB, (size_t)tB * 16);
     if (!rc) rc = bfk_d2h(hV, dV, (size_t)tV * 16);
     if (!rc) rc = bfk_d2h(hS, dS, (size_t)tS * 8);
-    if (!rc) rc = bfk_d2h(hR, dR, ns * 8);
+    if (!rc) rc = bfk_d2h(hR, dR, ns * (size_t)chunks * 8);
+    for (size_t q = 0; q < ns && !rc; ++q) { /* the residual partials, summed in order */
+      const int used = bfk_sketch_resid_chunks(sj[q].m * (int64_t)sj[q].n);
+      double t = 0;
+      for (int c = 0; c < used; ++c) t += hR[q * (size_t)chunks + c];
+      hR[q] = t;
+    }
     bfk_free(dA); bfk_free(dO); bfk_free(dY); bfk_free(dB); bfk_free(dV); bfk_free(dS); bfk_free(dR);
     bfk_free(dJ); bfk_free(dYo); bfk_free(dBo); bfk_free(dV1); bfk_free(dV2); bfk_free(dS1); bfk_free(dS2);
     bfk_free(dM); bfk_free(dN); bfk_free(dW);
@@ -599,11 +609,13 @@ static int svd_round(RankCtx *rk, const BfFacSpec *spec, TreeView rt, int32_t co
     rc = bfhipRankModelPredict(rk->model, nj, f, pr);
     for (size_t i = 0; i < nj && !rc; ++i) {
       /* a sketch costs two Jacobi SVDs of w columns (on m and n rows) and
-       * three m n w products: it pays only when w is well below min(m, n) */
+       * three m n w products: it pays only for the large matrices (one
+       * launch per Jacobi step: w - 1 steps twice against min(m, n) - 1)
+       * and only when w is well below min(m, n) */
       const double p = pr[i] > 0 ? ceil((double)pr[i]) : 0;
       const int64_t mn = jobs[i].m < jobs[i].n ? jobs[i].m : jobs[i].n;
       wd[i] = (int64_t)p + rk->oversample;
-      if (2 * wd[i] > mn) wd[i] = 0;
+      if (2 * wd[i] > mn || jobs[i].m * jobs[i].n < SVD_BIG_ENTRIES) wd[i] = 0;
     }
     if (!rc) rc = svd_batch_sketch(jobs, nj, spec->tol, wd, rk->seed++, fell);
     for (size_t i = 0; i < nj && !rc; ++i) {

@@ -118,27 +118,31 @@ def test_sketch_svd_matches_full_svd(gpu_lib):
 def test_streamer_with_rank_model(gpu_lib):
     """Stream a Helmholtz kernel matrix with plain SVDs (recording the
     samples), train the model on them, stream again with randomized SVDs of
-    width predicted rank + 8: same matrix to the streamer's accuracy, and
-    most SVDs sketched."""
+    width predicted rank + 8: the same matrix to the streamer's accuracy
+    against exact rows (oracle), with the large SVDs sketched."""
     bf = gpu_lib
     import oracle as orc
     rng = np.random.default_rng(8)
-    n = 2048
+    n = 4096
     src = rng.uniform(0, 1, (n, 2))
-    tgt = rng.uniform(2, 3, (n, 2))
-    tgt[:, 1] -= 2
-    k = 40.0
+    tgt = rng.uniform(0, 1, (n, 2))
+    tgt[:, 0] += 2
+    k = 2 * np.pi * np.sqrt(n) / 8
     tol = 1e-8
-    ts, tt = bf.Quadtree(src, 32), bf.Quadtree(tgt, 32)
-    A = orc.kernel_matrix(k, tgt[tt.perm], src[ts.perm])
+    ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
+    sp, tp = src[ts.perm], tgt[tt.perm]
     nodes = QuadNodes(bf, ts)
+    helm = bf.Helm2(k)
+    blocks = {}
 
     def stream(model):
         S = bf.FacStreamer(tt, ts, tol, init_depth=1, min_rows=20, min_cols=20)
         S.set_rank_model(model, k, 8)
         while not S.done:
             c = S.current_column_node()
-            S.feed(A[:, nodes[c][0]:nodes[c][1]])
+            if c not in blocks:
+                blocks[c] = helm.get_kernel_matrix(sp[nodes[c][0]:nodes[c][1]], tp)
+            S.feed(blocks[c])
         return S
 
     S0 = stream(None)
@@ -151,13 +155,12 @@ def test_streamer_with_rank_model(gpu_lib):
     S1 = stream(m)
     st1 = S1.sketch_stats()
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    y0 = S0.fac().mat() @ x
-    y1 = S1.fac().mat() @ x
-    want = A @ x
-    e0, e1 = rel(y0, want), rel(y1, want)
+    rows = np.sort(rng.choice(n, 128, replace=False))
+    want = orc.dense_rows(k, tp[rows], sp, x)
+    e0 = rel((S0.fac().mat() @ x)[rows], want)
+    e1 = rel((S1.fac().mat() @ x)[rows], want)
     print("streamer: samples", len(y), "train MAE", mae, "stats", st1, "errors", e0, e1)
-    assert st1["svds"] > 0 and st1["sketched"] >= st1["svds"] // 4
-    assert e1 < max(4 * e0, 50 * tol)
+    assert st1["sketched"] > 10 and e0 < 20 * tol and e1 < 20 * tol
 
 
 def QuadNodes(bf, tree):
