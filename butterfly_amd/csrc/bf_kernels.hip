@@ -485,6 +485,7 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
   const double thr2 = thr * thr;
   for (int sweep = 0; sweep < 40; ++sweep) {
     int rotated = 0;
+    double worst = 0; /* largest |g|^2 / (a b) rotated in this sweep */
     for (int step = 0; step < P - 1; ++step) {
       for (int pb = 0; pb < npairs; pb += pairsPerPass) {
         const int pr = pb + slot;
@@ -528,6 +529,7 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
         /* rows below 1e-145 in norm lie far under any cut; they are not rotated */
         if (valid && ag2 > 1e-290 && ag2 > thr2 * al * be) {
           rotated = 1;
+          worst = fmax(worst, ag2 / (al * be));
           const double rg = rsqrt_nr(ag2);
           const double ex = gx * rg, ey = gy * rg;
           const double zeta = (be - al) * (0.5 * rg);
@@ -568,6 +570,12 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
       }
     }
     if (!__any(rotated)) break;
+    /* cyclic Jacobi converges quadratically: once every rotation of a sweep
+     * was below 1e-8 in |g| / sqrt(a b), the next sweep's would be at the
+     * rounding level (below thr), so that sweep, which would only confirm
+     * convergence, is skipped */
+    for (int o = 32; o > 0; o >>= 1) worst = fmax(worst, __shfl_xor(worst, o, 64));
+    if (worst < 1e-16) break;
   }
 }
 
