@@ -506,11 +506,21 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
     /* what is uploaded: the rest of the current class, the classes that
      * are complete, and the uploaded start of the next */
     while (!rc && g + 1 < gs.n && launched < up) {
-      const size_t end = (size_t)gs.a[g + 1] < up ? (size_t)gs.a[g + 1] : up;
-      rc = bfk_lstsq_launch(dL + launched, NULL, (int64_t)(end - launched), gb.a[g], F->dSrcXY, F->dSrcN, layer,
+      /* classes [g, gEnd): one class, or all the classes above the LDS
+       * capacity (they come first and share the scratch, whose stride fits
+       * the largest, so they go out in one launch: per class they were ~40
+       * launches of a few dozen workgroups each) */
+      size_t gEnd = g + 1, lds = gb.a[g];
+      if (gb.a[g] > bfk_lds_cap()) {
+        while (gEnd + 1 < gs.n && gb.a[gEnd] > bfk_lds_cap()) ++gEnd;
+        lds = bigBytes;
+      }
+      const size_t classEnd = (size_t)gs.a[gEnd];
+      const size_t end = classEnd < up ? classEnd : up;
+      rc = bfk_lstsq_launch(dL + launched, NULL, (int64_t)(end - launched), lds, F->dSrcXY, F->dSrcN, layer,
                             F->dArena);
       launched = end;
-      if (launched == (size_t)gs.a[g + 1]) ++g;
+      if (launched == classEnd) g = gEnd;
     }
   }
   /* close the launch sequence even after a failed launch, so the side
