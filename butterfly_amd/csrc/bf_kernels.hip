@@ -1383,20 +1383,6 @@ template <typename V> __device__ __forceinline__ V cadd(V a, V b) {
   return a;
 }
 
-/* factor entries are read once per apply: non-temporal loads keep them
- * from displacing the reused input vectors in the caches (the read rate of
- * a streaming kernel rises from ~6.2 to ~7.1 TB/s on MI355X with them) */
-typedef double nt_d2 __attribute__((ext_vector_type(2)));
-typedef float nt_f2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ double2 ldnt(const double2 *p) {
-  const nt_d2 v = __builtin_nontemporal_load((const nt_d2 *)p);
-  return make_double2(v.x, v.y);
-}
-__device__ __forceinline__ float2 ldnt(const float2 *p) {
-  const nt_f2 v = __builtin_nontemporal_load((const nt_f2 *)p);
-  return make_float2(v.x, v.y);
-}
-
 template <typename V> struct BufsOf;
 template <> struct BufsOf<double2> { typedef BfkBufs T; };
 template <> struct BufsOf<float2> { typedef BfkBufsF T; };
@@ -1474,10 +1460,10 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
           int j = g;
           for (; j + G < n; j += 2 * G) {
             const V x0 = x[j], x1 = x[j + G];
-            V p0 = cmulf(ldnt(A + (int64_t)j * lda), x0), p1 = cmulf(ldnt(A + (int64_t)(j + G) * lda), x1);
+            V p0 = cmulf(A[(int64_t)j * lda], x0), p1 = cmulf(A[(int64_t)(j + G) * lda], x1);
             if (LO) {
-              p0 = cfmaf(ldnt(B + (int64_t)j * lda), x0, p0);
-              p1 = cfmaf(ldnt(B + (int64_t)(j + G) * lda), x1, p1);
+              p0 = cfmaf(B[(int64_t)j * lda], x0, p0);
+              p1 = cfmaf(B[(int64_t)(j + G) * lda], x1, p1);
             }
             sx0 += (double)p0.x;
             sy0 += (double)p0.y;
@@ -1486,8 +1472,8 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
           }
           if (j < n) {
             const V x0 = x[j];
-            V p0 = cmulf(ldnt(A + (int64_t)j * lda), x0);
-            if (LO) p0 = cfmaf(ldnt(B + (int64_t)j * lda), x0, p0);
+            V p0 = cmulf(A[(int64_t)j * lda], x0);
+            if (LO) p0 = cfmaf(B[(int64_t)j * lda], x0, p0);
             sx0 += (double)p0.x;
             sy0 += (double)p0.y;
           }
@@ -1525,20 +1511,20 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
         /* two independent chains: two factor loads in flight per lane at
          * the register count of one (occupancy stays at 8 waves/SIMD) */
         if (UNROLL) for (; j + G < n; j += 2 * G) {
-          const V m0 = ldnt(A + (int64_t)j * lda), m1 = ldnt(A + (int64_t)(j + G) * lda);
+          const V m0 = A[(int64_t)j * lda], m1 = A[(int64_t)(j + G) * lda];
           const V x0 = x[j], x1 = x[j + G];
           a0 = cfmaT(m0, x0, a0);
           a1 = cfmaT(m1, x1, a1);
           if (LO) {
             const V *B = lo + T.aOff + T.rowOff + i;
-            a0 = cfmaT(ldnt(B + (int64_t)j * lda), x0, a0);
-            a1 = cfmaT(ldnt(B + (int64_t)(j + G) * lda), x1, a1);
+            a0 = cfmaT(B[(int64_t)j * lda], x0, a0);
+            a1 = cfmaT(B[(int64_t)(j + G) * lda], x1, a1);
           }
         }
         for (; j < n; j += G) {
           V xj = x[j];
-          a0 = cfmaT(ldnt(A + (int64_t)j * lda), xj, a0);
-          if (LO) a0 = cfmaT(ldnt(lo + T.aOff + T.rowOff + i + (int64_t)j * lda), xj, a0);
+          a0 = cfmaT(A[(int64_t)j * lda], xj, a0);
+          if (LO) a0 = cfmaT(lo[T.aOff + T.rowOff + i + (int64_t)j * lda], xj, a0);
         }
       }
     }
