@@ -120,7 +120,7 @@ def lib():
         "bfMatMul": (vp, [vp, vp]),
         "bfMatDenseComplexNew": (vp, []),
         "bfMatDenseComplexInitViewFromPtr": (None, [vp, sz, sz, vp]),
-        "bfMatProductNumFactors": (sz, [vp, sz]),
+        "bfhipFacLeafNumFactors": (sz, [vp, sz]),
         "bfhipMalloc": (vp, [sz]),
         "bfhipFree": (None, [vp]),
         "bfhipMemcpyHtoD": (C.c_int, [vp, vp, sz]),

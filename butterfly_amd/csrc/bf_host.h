@@ -184,6 +184,14 @@ struct BfMatVtable {
   void (*PermuteRows)(BfMat *, BfPerm const *);
 };
 void bfMatInitBase(BfMat *mat, BfMatVtable *vtbl, BfSize numRows, BfSize numCols);
+/* one block of a factor handed to bfk_product_from_blocks */
+typedef struct {
+  int64_t i0, j0, m, n;
+  BfComplex *data; /* column-major m x n; NULL = identity (m == n) */
+} BfkBlockRef;
+BfMat *bfk_product_from_blocks(size_t nf, const int64_t *rows, const int64_t *cols, const size_t *nb,
+                               const BfkBlockRef *const *blocks);
+void bfk_mat_refresh_shape(BfMat const *m);
 BfMat *bfMatHipButterflyMul(BfMat const *A, BfMat const *X);
 void bfMatHipButterflyDelete(BfMat **mat);
 BfSize bfMatHipButterflyNumBytes(BfMat const *mat);

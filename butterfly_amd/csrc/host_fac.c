@@ -1073,7 +1073,7 @@ BfMat *bfMatHipButterflyMul(BfMat const *A, BfMat const *Xm) {
   return &Y->super;
 }
 
-BfSize bfMatProductNumFactors(BfMat const *fac, BfSize leaf) {
+BfSize bfhipFacLeafNumFactors(BfMat const *fac, BfSize leaf) {
   if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
     bfSetError(BF_ERROR_TYPE_ERROR);
     return BF_SIZE_BAD_VALUE;

@@ -33,11 +33,13 @@ BfType bfMatGetType(BfMat const *mat) {
 
 BfSize bfMatGetNumRows(BfMat const *mat) {
   if (!mat) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return BF_SIZE_BAD_VALUE; }
+  bfk_mat_refresh_shape(mat);
   return mat->numRows;
 }
 
 BfSize bfMatGetNumCols(BfMat const *mat) {
   if (!mat) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return BF_SIZE_BAD_VALUE; }
+  bfk_mat_refresh_shape(mat);
   return mat->numCols;
 }
 
@@ -60,6 +62,8 @@ void bfMatDelete(BfMat **mat) {
 BfMat *bfMatMul(BfMat const *op1, BfMat const *op2) {
   NEED(op1, Mul, NULL);
   if (!op2) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
+  bfk_mat_refresh_shape(op1);
+  bfk_mat_refresh_shape(op2);
   if (op1->numCols != op2->numRows) { bfSetError(BF_ERROR_INCOMPATIBLE_SHAPES); return NULL; }
   return op1->vtbl->Mul(op1, op2);
 }

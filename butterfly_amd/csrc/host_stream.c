@@ -1447,6 +1447,35 @@ static BfMat *prod_mul(BfMat const *A, BfMat const *Xm) {
   return &Y->super;
 }
 
+/* a device product from caller-described factors: factor f is rows[f] x
+ * cols[f] with nb[f] column-major blocks (a block without data is an
+ * identity block); the data are copied into the product's arena */
+BfMat *bfk_product_from_blocks(size_t nf, const int64_t *rows, const int64_t *cols, const size_t *nb,
+                               const BfkBlockRef *const *blocks) {
+  if (nf == 0) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  SMat *S = calloc(nf, sizeof(SMat));
+  const SMat **F = malloc(nf * sizeof(SMat *));
+  for (size_t f = 0; f < nf; ++f) {
+    smat_init(&S[f], rows[f], cols[f]);
+    S[f].b = malloc((nb[f] ? nb[f] : 1) * sizeof(SBlock));
+    S[f].nb = S[f].cap = nb[f];
+    for (size_t i = 0; i < nb[f]; ++i) {
+      const BfkBlockRef *r = &blocks[f][i];
+      SBlock blk = {r->i0, r->j0, r->m, r->n, r->data};
+      S[f].b[i] = blk;
+    }
+    F[f] = &S[f];
+  }
+  BfMat *m = product_new(F, nf);
+  for (size_t f = 0; f < nf; ++f) free(S[f].b); /* the data belong to the caller */
+  free(S);
+  free(F);
+  return m;
+}
+
 /* Psi W0 ... W_L as one device product (fac.c:40-75) */
 BfMat *bfFacGetMat(BfFac const *fac, BfPolicy policy) {
   (void)policy;

@@ -312,15 +312,16 @@ void bfMatDenseComplexInit(BfMatDenseComplex *mat, BfSize numRows, BfSize numCol
 void bfMatDenseComplexInitViewFromPtr(BfMatDenseComplex *mat, BfSize numRows, BfSize numCols,
                                       BfComplex *data);
 BfMatDenseComplex *bfMatDenseComplexNewRandn(BfSize numRows, BfSize numCols);
+BfMatDenseComplex *bfMatDenseComplexNewViewFromPtr(BfSize numRows, BfSize numCols, BfComplex *data);
 void bfMatDenseComplexDeinit(BfMatDenseComplex *mat);
 void bfMatDenseComplexDeinitAndDealloc(BfMatDenseComplex **mat);
 BfComplex *bfMatDenseComplexGetDataPtr(BfMatDenseComplex *mat);
 BfMat *bfMatDenseComplexToMat(BfMatDenseComplex *mat);
 BfMatDenseComplex *bfMatToMatDenseComplex(BfMat *mat);
 
-/* mat_product.h:38-41 on the device factorization: number of top-level
- * blocks (dense or butterfly) and factors of butterfly leaf `i` */
-BfSize bfMatProductNumFactors(BfMat const *fac, BfSize leaf);
+/* new (no counterpart): on a device factorization, the number of factors
+ * of butterfly leaf `leaf` (the reference's BfMatProduct per block) */
+BfSize bfhipFacLeafNumFactors(BfMat const *fac, BfSize leaf);
 
 /* ------------------------------------------------ device entry points */
 
@@ -429,6 +430,134 @@ typedef enum BfPolicies {
   BF_POLICY_COPY,
   BF_POLICY_STEAL,
 } BfPolicy;
+
+/* ------------------------------------------- the block matrix algebra */
+
+/* ptr_array.h:7-40 */
+typedef struct BfPtrArray {
+  void **data;
+  BfSize capacity;
+  BfSize num_elts;
+  bool isView;
+} BfPtrArray;
+void bfInitPtrArray(BfPtrArray *arr, BfSize capacity);
+void bfInitPtrArrayWithDefaultCapacity(BfPtrArray *arr);
+void bfPtrArrayDeinit(BfPtrArray *arr);
+BfSize bfPtrArraySize(BfPtrArray const *arr);
+bool bfPtrArrayIsEmpty(BfPtrArray const *arr);
+void bfPtrArrayAppend(BfPtrArray *arr, void *ptr);
+void *bfPtrArrayGet(BfPtrArray const *arr, BfSize pos);
+void *bfPtrArrayPopLast(BfPtrArray *arr);
+
+/* mat.h:199: the matrix itself (VIEW: the caller keeps ownership; STEAL:
+ * ownership passes) or a copy (COPY) */
+BfMat *bfMatGet(BfMat *mat, BfPolicy policy);
+
+/* mat_block.h:39-66.  Block i covers rows rowOffset[r] .. rowOffset[r+1]
+ * and columns colOffset[c] .. colOffset[c+1] of its block row r and block
+ * column c (block-diagonal: r = c = i; block-dense: block r * numBlockCols
+ * + c; block-COO: r = rowInd[i], c = colInd[i]).  Callers may fill the
+ * arrays directly, as the reference's drivers do. */
+typedef struct BfMatBlock {
+  BfMat super;
+  void *vtbl;        /* unused (the reference's block vtable) */
+  BfMat **block;
+  BfSize *rowOffset; /* numBlockRows + 1 entries */
+  BfSize *colOffset; /* numBlockCols + 1 entries */
+} BfMatBlock;
+
+/* mat_block_diag.h:33-37 */
+typedef struct BfMatBlockDiag {
+  BfMatBlock super;
+  BfSize numBlockRows, numBlockCols; /* the block grid (numBlocks = min of the two) */
+  bool *ownsBlock;                   /* blocks this matrix deletes */
+} BfMatBlockDiag;
+
+/* mat_block_coo.h:34-46 */
+typedef struct BfMatBlockCoo {
+  BfMatBlock super;
+  BfSize numBlocks;
+  BfSize *rowInd;
+  BfSize *colInd;
+  BfSize numBlockRows, numBlockCols;
+  bool *ownsBlock;
+} BfMatBlockCoo;
+
+/* mat_block_dense.h:60-62 */
+typedef struct BfMatBlockDense {
+  BfMatBlock super;
+  BfSize numBlockRows, numBlockCols;
+  bool *ownsBlock;
+} BfMatBlockDense;
+
+/* mat_product.h:28-31: the product F_0 F_1 ... F_{n-1}; applied last
+ * factor first */
+typedef struct BfMatProduct {
+  BfMat super;
+  BfPtrArray factorArr;
+} BfMatProduct;
+
+/* The containers are host objects; bfMatMul on one whose leaves are dense
+ * complex matrices (or containers of them) packs the leaves into a device
+ * product -- one apply stage per factor, the leaves of a factor as that
+ * stage's blocks -- and runs the butterfly's stage kernels; X and Y are
+ * host matrices.  bfhipMatPack returns that device product for repeated
+ * applies (BF_TYPE_MAT_HIP_PRODUCT, bfMatMul / bfMatDelete).  A product
+ * with other factors (a device factorization, say) is applied factor by
+ * factor, each with its own bfMatMul (mat_product.c:211-245). */
+BfMatProduct *bfMatProductNew(void);
+void bfMatProductInit(BfMatProduct *prod);
+void bfMatProductDeinit(BfMatProduct *prod);
+void bfMatProductDealloc(BfMatProduct **prod);
+void bfMatProductDeinitAndDealloc(BfMatProduct **prod);
+BfSize bfMatProductNumFactors(BfMatProduct const *prod);
+BfMat *bfMatProductGetFactor(BfMatProduct *prod, BfSize i);
+BfMat const *bfMatProductGetFactorConst(BfMatProduct const *prod, BfSize i);
+/* appends `mat` (the product takes ownership): prod <- prod * mat */
+void bfMatProductPostMultiply(BfMatProduct *prod, BfMat *mat);
+BfMat *bfMatProductPopLastFactor(BfMatProduct *prod);
+BfMat *bfMatProductToMat(BfMatProduct *prod);
+BfMatProduct *bfMatToMatProduct(BfMat *mat);
+
+BfMatBlockDiag *bfMatBlockDiagNew(void);
+BfMatBlockDiag *bfMatBlockDiagNewFromBlocks(BfPtrArray *blocks, BfPolicy policy);
+void bfMatBlockDiagInit(BfMatBlockDiag *mat, BfSize numBlockRows, BfSize numBlockCols);
+void bfMatBlockDiagDeinit(BfMatBlockDiag *mat);
+void bfMatBlockDiagDealloc(BfMatBlockDiag **mat);
+void bfMatBlockDiagDeinitAndDealloc(BfMatBlockDiag **mat);
+BfSize bfMatBlockDiagNumBlocks(BfMatBlockDiag const *mat);
+BfMat *bfMatBlockDiagGetBlock(BfMatBlockDiag *mat, BfSize i);
+BfMat const *bfMatBlockDiagGetBlockConst(BfMatBlockDiag const *mat, BfSize i);
+/* takes ownership of `block`; the offsets follow the blocks' shapes */
+void bfMatBlockDiagSetBlock(BfMatBlockDiag *mat, BfSize i, BfMat *block);
+BfMat *bfMatBlockDiagToMat(BfMatBlockDiag *mat);
+BfMatBlockDiag *bfMatToMatBlockDiag(BfMat *mat);
+
+BfMatBlockCoo *bfMatBlockCooNew(void);
+/* the caller fills rowOffset, colOffset, rowInd, colInd and block */
+void bfMatBlockCooInit(BfMatBlockCoo *mat, BfSize numBlockRows, BfSize numBlockCols, BfSize numBlocks);
+void bfMatBlockCooDeinit(BfMatBlockCoo *mat);
+void bfMatBlockCooDealloc(BfMatBlockCoo **mat);
+void bfMatBlockCooDeinitAndDealloc(BfMatBlockCoo **mat);
+BfMat *bfMatBlockCooToMat(BfMatBlockCoo *mat);
+BfMatBlockCoo *bfMatToMatBlockCoo(BfMat *mat);
+
+BfMatBlockDense *bfMatBlockDenseNew(void);
+BfMatBlockDense *bfMatBlockDenseNewFromBlocks(BfSize numRowBlocks, BfSize numColBlocks, BfPtrArray *blocks,
+                                              BfPolicy policy);
+void bfMatBlockDenseInit(BfMatBlockDense *mat, BfSize numBlockRows, BfSize numBlockCols);
+void bfMatBlockDenseDeinit(BfMatBlockDense *mat);
+void bfMatBlockDenseDealloc(BfMatBlockDense **mat);
+void bfMatBlockDenseDeinitAndDealloc(BfMatBlockDense **mat);
+BfMat *bfMatBlockDenseGetBlock(BfMatBlockDense *mat, BfSize i, BfSize j);
+/* takes ownership of `block`; the offsets follow the blocks' shapes */
+void bfMatBlockDenseSetBlock(BfMatBlockDense *mat, BfSize i, BfSize j, BfMat *block);
+BfMat *bfMatBlockDenseToMat(BfMatBlockDense *mat);
+BfMatBlockDense *bfMatToMatBlockDense(BfMat *mat);
+
+/* new (no counterpart): the device product of a container of dense
+ * leaves, for repeated applies */
+BfMat *bfhipMatPack(BfMat const *mat);
 
 /* fac.h:6-44.  The trees are quadtrees of this library (BfQuadtree's
  * BfTree super); rows of fed blocks follow the row tree's order and their
