@@ -656,3 +656,89 @@ BfMatDenseComplex *bfMatDenseComplexNewViewFromPtr(BfSize numRows, BfSize numCol
   bfMatDenseComplexInitViewFromPtr(m, numRows, numCols, data);
   return m;
 }
+
+/* ------------------------------------------------------ complex vectors */
+
+/* marks a BfVec as a BfVecComplex (the reference dispatches through a
+ * vector vtable; only complex vectors exist here) */
+static int vecComplexTag;
+#define VEC_VIEW 1
+
+BfVecComplex *bfVecComplexNew(void) {
+  BfVecComplex *v = calloc(1, sizeof(BfVecComplex));
+  if (!v) bfSetError(BF_ERROR_MEMORY_ERROR);
+  return v;
+}
+
+void bfVecComplexInit(BfVecComplex *vec, BfSize size) {
+  if (!vec) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return; }
+  vec->super.vtbl = &vecComplexTag;
+  vec->super.props = 0;
+  vec->super.size = size;
+  vec->stride = 1;
+  vec->data = calloc(size ? size : 1, sizeof(BfComplex));
+  if (!vec->data) bfSetError(BF_ERROR_MEMORY_ERROR);
+}
+
+void bfVecComplexInitView(BfVecComplex *vec, BfSize size, BfSize stride, BfComplex *data) {
+  if (!vec) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return; }
+  vec->super.vtbl = &vecComplexTag;
+  vec->super.props = VEC_VIEW;
+  vec->super.size = size;
+  vec->stride = stride;
+  vec->data = data;
+}
+
+void bfVecComplexDeinit(BfVecComplex *vec) {
+  if (!vec) return;
+  if (!(vec->super.props & VEC_VIEW)) free(vec->data);
+  vec->data = NULL;
+}
+
+void bfVecComplexDeinitAndDealloc(BfVecComplex **vec) {
+  if (!vec || !*vec) return;
+  bfVecComplexDeinit(*vec);
+  free(*vec);
+  *vec = NULL;
+}
+
+BfVec *bfVecComplexToVec(BfVecComplex *vec) { return vec ? &vec->super : NULL; }
+
+BfVecComplex *bfVecToVecComplex(BfVec *vec) {
+  if (!vec || vec->vtbl != &vecComplexTag) { bfSetError(BF_ERROR_TYPE_ERROR); return NULL; }
+  return (BfVecComplex *)vec;
+}
+
+void bfVecDelete(BfVec **vec) {
+  if (!vec || !*vec) return;
+  if ((*vec)->vtbl != &vecComplexTag) { bfSetError(BF_ERROR_TYPE_ERROR); return; }
+  BfVecComplex *v = (BfVecComplex *)*vec;
+  bfVecComplexDeinitAndDealloc(&v);
+  *vec = NULL;
+}
+
+BfVec *bfMatMulVec(BfMat const *mat, BfVec const *vec) {
+  if (!mat || !vec || vec->vtbl != &vecComplexTag) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
+  const BfVecComplex *x = (const BfVecComplex *)vec;
+  BfMatDenseComplex *X = bfMatDenseComplexNew();
+  if (!X) return NULL;
+  bfMatDenseComplexInit(X, x->super.size, 1);
+  if (!X->data) { free(X); return NULL; }
+  for (BfSize i = 0; i < x->super.size; ++i) X->data[i] = x->data[i * x->stride];
+  BfMat *Y = bfMatMul(mat, &X->super);
+  bfMatDenseComplexDeinitAndDealloc(&X);
+  if (!Y) return NULL;
+  if (bfMatGetType(Y) != BF_TYPE_MAT_DENSE_COMPLEX || Y->numCols != 1) {
+    bfMatDelete(&Y);
+    bfSetError(BF_ERROR_TYPE_ERROR);
+    return NULL;
+  }
+  BfMatDenseComplex *Yd = (BfMatDenseComplex *)Y;
+  BfVecComplex *y = bfVecComplexNew();
+  if (!y) { bfMatDelete(&Y); return NULL; }
+  bfVecComplexInit(y, Y->numRows);
+  if (!y->data) { free(y); bfMatDelete(&Y); return NULL; }
+  for (BfSize i = 0; i < Y->numRows; ++i) y->data[i] = Yd->data[i * Yd->rowStride];
+  bfMatDelete(&Y);
+  return &y->super;
+}

@@ -559,6 +559,30 @@ BfMatBlockDense *bfMatToMatBlockDense(BfMat *mat);
  * leaves, for repeated applies */
 BfMat *bfhipMatPack(BfMat const *mat);
 
+/* vec.h:74-78, vec_complex.h:20-34: complex vectors (entry i at
+ * data[i * stride]) */
+typedef struct BfVec {
+  void *vtbl;
+  int props;
+  BfSize size;
+} BfVec;
+typedef struct BfVecComplex {
+  BfVec super;
+  BfSize stride;
+  BfComplex *data;
+} BfVecComplex;
+BfVecComplex *bfVecComplexNew(void);
+void bfVecComplexInit(BfVecComplex *vec, BfSize size);
+void bfVecComplexInitView(BfVecComplex *vec, BfSize size, BfSize stride, BfComplex *data);
+void bfVecComplexDeinit(BfVecComplex *vec);
+void bfVecComplexDeinitAndDealloc(BfVecComplex **vec);
+BfVec *bfVecComplexToVec(BfVecComplex *vec);
+BfVecComplex *bfVecToVecComplex(BfVec *vec);
+void bfVecDelete(BfVec **vec);
+/* mat.h:87: y = A x for a complex vector, through bfMatMul on an n x 1
+ * matrix (so a device factorization or a container applies on the GPU) */
+BfVec *bfMatMulVec(BfMat const *mat, BfVec const *vec);
+
 /* fac.h:6-44.  The trees are quadtrees of this library (BfQuadtree's
  * BfTree super); rows of fed blocks follow the row tree's order and their
  * columns the column tree's. */

@@ -252,3 +252,54 @@ def test_product_with_a_device_factorization(gpu_lib):
     lib.bfMatProductPopLastFactor(p)
     lib.bfMatDelete(C.byref(C.c_void_p(p)))
     lib.bfMatDelete(C.byref(C.c_void_p(wm.ptr)))
+
+
+class VecComplexS(C.Structure):
+    _fields_ = [("vtbl", vp), ("props", C.c_int), ("size", sz), ("stride", sz), ("data", C.POINTER(C.c_double))]
+
+
+def vec_lib():
+    lib = L()
+    for name, res, args in [("bfVecComplexNew", vp, []), ("bfVecComplexInitView", None, [vp, sz, sz, vp]),
+                            ("bfVecComplexInit", None, [vp, sz]), ("bfVecDelete", None, [C.POINTER(vp)]),
+                            ("bfMatMulVec", vp, [vp, vp])]:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def test_complex_vectors_views_and_ownership():
+    lib = vec_lib()
+    data = np.arange(12, dtype=np.complex128)
+    v = lib.bfVecComplexNew()
+    lib.bfVecComplexInitView(v, 4, 3, data.ctypes.data)  # every third entry
+    s = VecComplexS.from_address(v)
+    assert s.size == 4 and s.stride == 3
+    lib.bfVecDelete(C.byref(C.c_void_p(v)))               # a view leaves the data alone
+    assert data[9] == 9
+    w = lib.bfVecComplexNew()
+    lib.bfVecComplexInit(w, 5)
+    assert VecComplexS.from_address(w).size == 5
+    lib.bfVecDelete(C.byref(C.c_void_p(w)))
+    assert lib.bfGetError() == 0
+
+
+@pytest.mark.gpu
+def test_mat_mul_vec(gpu_lib):
+    lib = vec_lib()
+    rng = np.random.default_rng(4)
+    keep = []
+    d, D = make_diag(rng, [(3, 5), (6, 2)], keep)
+    x = np.zeros(14, complex)
+    x[::2] = rand(rng, 7, 1)[:, 0]                          # strided view of 7 entries
+    v = lib.bfVecComplexNew()
+    lib.bfVecComplexInitView(v, 7, 2, x.ctypes.data)
+    y = lib.bfMatMulVec(d, v)
+    assert y, lib.bfGetError()
+    s = VecComplexS.from_address(y)
+    got = np.ctypeslib.as_array(s.data, shape=(2 * s.size,)).view(np.complex128).copy()
+    assert s.size == 9 and rel(got, D @ x[::2]) < 1e-13
+    lib.bfVecDelete(C.byref(C.c_void_p(y)))
+    lib.bfVecDelete(C.byref(C.c_void_p(v)))
+    lib.bfMatDelete(C.byref(C.c_void_p(d)))
