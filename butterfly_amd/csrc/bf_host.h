@@ -59,6 +59,10 @@ int bfk_svd_jacobi(int64_t batch, const int64_t *dAOff, const int32_t *dM, const
 int bfk_svd_jacobi_big(int64_t nb, const int32_t *dBig, const int64_t *dAOff, const int32_t *dM, const int32_t *dN,
                        double *dA, const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig,
                        int maxSweeps, int maxN);
+int bfk_svd_tall_max_n(void);
+int bfk_svd_jacobi_tall(int64_t nb, const int32_t *dIdx, const int64_t *dAOff, const int32_t *dM, const int32_t *dN,
+                        double *dA, double *dS, const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig,
+                        int maxSweeps, int maxN);
 size_t bfk_ortho_units(int rows, int n);
 int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, const int64_t *groupStart,
               const size_t *groupBytes, const BfkOrthoItem *dItems, double *dArena, const double *dRin,

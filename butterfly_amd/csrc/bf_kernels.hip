@@ -50,6 +50,7 @@ static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 static hipEvent_t g_evL0 = nullptr, g_evL1 = nullptr;
 static int g_lPending = 0;
 static size_t g_ldsCap = 0;
+static bool g_tallOk = false; /* k_svd_tall has its LDS */
 static double g_times[8];   /* last measured ms per category */
 static double g_sum[8];     /* accumulated ms per category since the last reset */
 static double g_cnt[8];     /* number of measurements since the last reset */
@@ -1188,6 +1189,191 @@ __global__ __launch_bounds__(256) void k_svd_big_norms(const int32_t *__restrict
   if (threadIdx.x == 0) sig[sOff[b] + c] = sqrt(red[0] + red[1] + red[2] + red[3]);
 }
 
+/* SVD of a tall matrix (m >= 2n, n <= SVD_TALL_MAX_N) by one workgroup:
+ * Householder QR of a copy S of A (global memory; each wave updates every
+ * fourth column with a wave reduction), the triangle R into LDS, one-sided
+ * Jacobi on the columns of R with V accumulated in LDS (the pairs, order,
+ * rotation and stopping rule of k_svd_jacobi), then W = Q [R V; 0] into A:
+ * W = A V, so its columns are sigma_j u_j as k_svd_jacobi's.
+ * The rotations touch n rows instead of m; the per-step launches of
+ * k_svd_step are not needed for these shapes. */
+#define SVD_TALL_MAX_N 70
+#define SVD_TALL_T 1024 /* threads: 16 waves share the column updates of the QR */
+__host__ __device__ __forceinline__ size_t svd_tall_lds(int n) { return (size_t)2 * n * n * 16 + 64; }
+
+__global__ __launch_bounds__(SVD_TALL_T) void k_svd_tall(const int32_t *__restrict__ idx, const int64_t *__restrict__ aOff,
+                                                  const int32_t *__restrict__ mm, const int32_t *__restrict__ nn,
+                                                  double2 *__restrict__ A, double2 *__restrict__ S,
+                                                  const int64_t *__restrict__ vOff, double2 *__restrict__ Vout,
+                                                  const int64_t *__restrict__ sOff, double *__restrict__ sig,
+                                                  int maxSweeps) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[SVD_TALL_T / 64];
+  __shared__ double hinv[SVD_TALL_MAX_N]; /* 1 / (sig (sig + |x_j|)) of reflector j, 0 = none */
+  __shared__ int anyRot;
+  const int b = idx[blockIdx.x];
+  const int m = mm[b], n = nn[b];
+  double2 *a = A + aOff[b], *s = S + aOff[b];
+  double2 *R = lds, *V = lds + n * n; /* column-major n x n */
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const double eps = 2.220446049250313e-16;
+  for (int64_t e = tid; e < (int64_t)m * n; e += SVD_TALL_T) s[e] = a[e];
+  __syncthreads();
+  /* 1. Householder QR of S: column j's reflector v = x + ph sig e_j,
+   * H = I - v v^H / (sig (sig + |x_j|)), R_jj = -ph sig */
+  for (int j = 0; j < n; ++j) {
+    double2 *x = s + (int64_t)j * m;
+    double q = 0;
+    for (int r = j + tid; r < m; r += SVD_TALL_T) q = fma(x[r].x, x[r].x, fma(x[r].y, x[r].y, q));
+    q = group_sum(q, 64);
+    if (lane == 0) red[w] = q;
+    __syncthreads();
+    double tot = 0;
+#pragma unroll
+    for (int q2 = 0; q2 < SVD_TALL_T / 64; ++q2) tot += red[q2];
+    const double sg = sqrt(tot);
+    const double2 x0 = x[j];
+    const double ax0 = hypot(x0.x, x0.y);
+    const double phx = ax0 > 0 ? x0.x / ax0 : 1.0, phy = ax0 > 0 ? x0.y / ax0 : 0.0;
+    __syncthreads();
+    if (tid == 0) hinv[j] = sg > 0 ? 1.0 / (sg * (sg + ax0)) : 0.0;
+    if (sg > 0) {
+      if (tid == 0) x[j] = make_double2(x0.x + phx * sg, x0.y + phy * sg);
+      __syncthreads();
+      const double inv = 1.0 / (sg * (sg + ax0));
+      for (int c = j + 1 + w; c < n; c += SVD_TALL_T / 64) {
+        double2 *y = s + (int64_t)c * m;
+        double dx = 0, dy = 0; /* v^H y */
+        for (int r = j + lane; r < m; r += 64) {
+          const double2 v = x[r], t = y[r];
+          dx = fma(v.x, t.x, fma(v.y, t.y, dx));
+          dy = fma(v.x, t.y, fma(-v.y, t.x, dy));
+        }
+        dx = group_sum(dx, 64) * inv;
+        dy = group_sum(dy, 64) * inv;
+        for (int r = j + lane; r < m; r += 64) {
+          const double2 v = x[r];
+          double2 t = y[r];
+          t.x -= v.x * dx - v.y * dy;
+          t.y -= v.x * dy + v.y * dx;
+          y[r] = t;
+        }
+      }
+      __syncthreads();
+    }
+    /* column j of R: the rows above j are final, R_jj = -ph sig */
+    for (int r = tid; r < n; r += SVD_TALL_T)
+      R[j * n + r] = r < j ? x[r] : (r == j ? (sg > 0 ? make_double2(-phx * sg, -phy * sg) : x0)
+                                            : make_double2(0, 0));
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += SVD_TALL_T) V[e] = make_double2((e % (n + 1)) == 0 ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  /* 2. one-sided Jacobi on the columns of R, V accumulated */
+  const int P = n + (n & 1), npairs = P / 2;
+  int g = 64;
+  while (g > 1 && g * npairs > SVD_TALL_T) g >>= 1;
+  const int slots = SVD_TALL_T / g, slot = tid / g, sub = tid - slot * g;
+  for (int sweep = 0; sweep < maxSweeps && n > 1; ++sweep) {
+    if (tid == 0) anyRot = 0;
+    __syncthreads();
+    for (int step = 0; step < P - 1; ++step) {
+      for (int pb = 0; pb < npairs; pb += slots) {
+        const int pr = pb + slot;
+        int i = -1, j = -1;
+        if (pr < npairs) {
+          const int p0 = rr_row(pr, step, P), p1 = rr_row(P - 1 - pr, step, P);
+          i = min(p0, p1);
+          j = max(p0, p1);
+        }
+        const bool valid = pr < npairs && j < n;
+        double al = 0, be = 0, gx = 0, gy = 0;
+        if (valid) {
+          const double2 *ci = R + i * n, *cj = R + j * n;
+          for (int r = sub; r < n; r += g) {
+            const double2 x = ci[r], y = cj[r];
+            al = fma(x.x, x.x, fma(x.y, x.y, al));
+            be = fma(y.x, y.x, fma(y.y, y.y, be));
+            gx = fma(x.x, y.x, fma(x.y, y.y, gx));
+            gy = fma(x.x, y.y, fma(-x.y, y.x, gy));
+          }
+        }
+        for (int o = 1; o < g; o <<= 1) {
+          al += __shfl_xor(al, o, 64);
+          be += __shfl_xor(be, o, 64);
+          gx += __shfl_xor(gx, o, 64);
+          gy += __shfl_xor(gy, o, 64);
+        }
+        const double ag = sqrt(gx * gx + gy * gy);
+        if (valid && ag > 0 && ag > eps * sqrt(al * be)) {
+          if (sub == 0) anyRot = 1;
+          const double ex = gx / ag, ey = -gy / ag;
+          const double zeta = (be - al) / (2 * ag);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+          const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+          double2 *ci = R + i * n, *cj = R + j * n, *vi = V + i * n, *vj = V + j * n;
+          for (int r = sub; r < n; r += g) {
+            double2 x = ci[r], y = cj[r];
+            double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x;
+            ci[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
+            cj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
+            x = vi[r];
+            y = vj[r];
+            ux = ex * y.x - ey * y.y;
+            uy = ex * y.y + ey * y.x;
+            vi[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
+            vj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!anyRot) break;
+    __syncthreads();
+  }
+  /* 3. sigma from the columns of R V, V out, W = Q [R V; 0] into A (the
+   * reflectors applied last first: each column keeps its orthogonality to
+   * the others relative to its own norm, as in k_svd_jacobi) */
+  double *sv = sig + sOff[b];
+  for (int j = slot; j < n; j += slots) {
+    double q = 0;
+    for (int r = sub; r < n; r += g) q = fma(R[j * n + r].x, R[j * n + r].x, fma(R[j * n + r].y, R[j * n + r].y, q));
+    for (int o = 1; o < g; o <<= 1) q += __shfl_xor(q, o, 64);
+    if (sub == 0) sv[j] = sqrt(q);
+  }
+  double2 *vo = Vout + vOff[b];
+  for (int e = tid; e < n * n; e += SVD_TALL_T) vo[e] = V[e];
+  for (int64_t e = tid; e < (int64_t)m * n; e += SVD_TALL_T) {
+    const int c = (int)(e / m), r = (int)(e - (int64_t)c * m);
+    a[e] = r < n ? R[c * n + r] : make_double2(0, 0);
+  }
+  __syncthreads();
+  for (int j = n - 1; j >= 0; --j) {
+    const double inv = hinv[j];
+    if (inv == 0) continue; /* uniform: hinv is in LDS */
+    const double2 *x = s + (int64_t)j * m;
+    for (int c = w; c < n; c += SVD_TALL_T / 64) {
+      double2 *y = a + (int64_t)c * m;
+      double dx = 0, dy = 0; /* v^H y */
+      for (int r = j + lane; r < m; r += 64) {
+        const double2 v = x[r], t = y[r];
+        dx = fma(v.x, t.x, fma(v.y, t.y, dx));
+        dy = fma(v.x, t.y, fma(-v.y, t.x, dy));
+      }
+      dx = group_sum(dx, 64) * inv;
+      dy = group_sum(dy, 64) * inv;
+      for (int r = j + lane; r < m; r += 64) {
+        const double2 v = x[r];
+        double2 t = y[r];
+        t.x -= v.x * dx - v.y * dy;
+        t.y -= v.x * dy + v.y * dx;
+        y[r] = t;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 /* ---------------------------------------------------- orthogonalization */
 
 /* LDS of one orthogonalization job, in 16-byte units: the stacked column
@@ -1792,6 +1978,12 @@ int bfk_init(void) {
       g_ldsCap = 65536;
     }
   }
+  /* the tall SVD kernel asks only for what its largest case needs (it has
+   * a few static LDS words besides) */
+  g_tallOk = g_ldsCap >= svd_tall_lds(SVD_TALL_MAX_N) + 1024 &&
+             hipFuncSetAttribute((const void *)k_svd_tall, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)svd_tall_lds(SVD_TALL_MAX_N)) == hipSuccess;
+  if (!g_tallOk) (void)hipGetLastError();
   return 0;
 }
 
@@ -2209,6 +2401,27 @@ int bfk_svd_jacobi_big(int64_t nb, const int32_t *dBig, const int64_t *dAOff, co
   (void)hipFree(dAct);
   (void)hipFree(dRot);
   return rc;
+}
+
+int bfk_svd_tall_max_n(void) { return g_tallOk ? SVD_TALL_MAX_N : 0; }
+
+/* tall matrices idx[0 .. nb) of a batch (m >= 2n, n <= bfk_svd_tall_max_n()),
+ * one workgroup each; dS is scratch of the size of dA */
+int bfk_svd_jacobi_tall(int64_t nb, const int32_t *dIdx, const int64_t *dAOff, const int32_t *dM, const int32_t *dN,
+                        double *dA, double *dS, const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig,
+                        int maxSweeps, int maxN) {
+  if (bfk_init()) return -1;
+  if (nb <= 0) return 0;
+  if (!g_tallOk || maxN > SVD_TALL_MAX_N) return -3;
+  const int64_t maxGrid = 1 << 30;
+  for (int64_t b0 = 0; b0 < nb; b0 += maxGrid) {
+    const int64_t k = nb - b0 < maxGrid ? nb - b0 : maxGrid;
+    hipLaunchKernelGGL(k_svd_tall, dim3((unsigned)k), dim3(SVD_TALL_T), svd_tall_lds(maxN), g_stream, dIdx + b0, dAOff, dM,
+                       dN, (double2 *)dA, (double2 *)dS, dVOff, (double2 *)dV, dSOff, dSig, maxSweeps);
+    BFK_CHECK(hipGetLastError());
+  }
+  BFK_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
 }
 
 size_t bfk_ortho_units(int rows, int n) { return ortho_units(rows, n); }

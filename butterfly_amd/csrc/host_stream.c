@@ -202,13 +202,19 @@ static int cmp_sig_desc(const void *a, const void *b) {
 static int jacobi_dispatch(int64_t nj, const int64_t *aOff, const int32_t *M, const int32_t *N, const int64_t *vOff,
                            const int64_t *sOff, void *dAo, void *dM, void *dN, void *dVo, void *dSo, void *dA,
                            void *dV, void *dS) {
-  int32_t *bigIdx = malloc(nz(nj) * sizeof(int32_t));
+  int32_t *bigIdx = malloc(nz(nj) * sizeof(int32_t)), *tallIdx = malloc(nz(nj) * sizeof(int32_t));
   int64_t *sA = malloc(nz(nj) * 8), *sV = malloc(nz(nj) * 8), *sS = malloc(nz(nj) * 8);
   int32_t *sM = malloc(nz(nj) * 4), *sN = malloc(nz(nj) * 4);
-  int64_t nbig = 0, nsmall = 0;
-  int maxN = 0, rc = 0;
+  int64_t nbig = 0, nsmall = 0, ntall = 0, tallEnd = 0;
+  int maxN = 0, maxTallN = 0, rc = 0;
+  const int tallMax = bfk_svd_tall_max_n();
   for (int64_t i = 0; i < nj; ++i) {
-    if ((int64_t)M[i] * N[i] >= SVD_BIG_ENTRIES && N[i] >= 2) {
+    if ((int64_t)M[i] * N[i] >= SVD_BIG_ENTRIES && N[i] >= 2 && N[i] <= tallMax && M[i] >= 2 * N[i]) {
+      /* tall: QR first, the Jacobi on the n x n triangle in LDS */
+      tallIdx[ntall++] = (int32_t)i;
+      if (N[i] > maxTallN) maxTallN = N[i];
+      if (aOff[i] + (int64_t)M[i] * N[i] > tallEnd) tallEnd = aOff[i] + (int64_t)M[i] * N[i];
+    } else if ((int64_t)M[i] * N[i] >= SVD_BIG_ENTRIES && N[i] >= 2) {
       bigIdx[nbig++] = (int32_t)i;
       if (N[i] > maxN) maxN = N[i];
     } else {
@@ -220,13 +226,23 @@ static int jacobi_dispatch(int64_t nj, const int64_t *aOff, const int32_t *M, co
       ++nsmall;
     }
   }
-  if (nbig) {
+  if (ntall) {
+    void *dT = bfk_malloc((size_t)ntall * 4), *dScr = bfk_malloc((size_t)tallEnd * 16);
+    if (!dT || !dScr) rc = -2;
+    if (!rc) rc = bfk_h2d(dT, tallIdx, (size_t)ntall * 4);
+    if (!rc) rc = bfk_svd_jacobi_tall(ntall, dT, dAo, dM, dN, dA, dScr, dVo, dV, dSo, dS, 60, maxTallN);
+    bfk_free(dT);
+    bfk_free(dScr);
+  }
+  if (nbig && !rc) {
     void *dB = bfk_malloc((size_t)nbig * 4);
     if (!dB) rc = -2;
     if (!rc) rc = bfk_h2d(dB, bigIdx, (size_t)nbig * 4);
     if (!rc) rc = bfk_svd_jacobi_big(nbig, dB, dAo, dM, dN, dA, dVo, dV, dSo, dS, 60, maxN);
     bfk_free(dB);
-    if (!rc && nsmall) { /* the small ones' tables */
+  }
+  if ((nbig || ntall) && !rc) {
+    if (nsmall) { /* the small ones' tables */
       rc = bfk_h2d(dAo, sA, (size_t)nsmall * 8);
       if (!rc) rc = bfk_h2d(dVo, sV, (size_t)nsmall * 8);
       if (!rc) rc = bfk_h2d(dSo, sS, (size_t)nsmall * 8);
@@ -235,7 +251,7 @@ static int jacobi_dispatch(int64_t nj, const int64_t *aOff, const int32_t *M, co
     }
   }
   if (!rc && nsmall) rc = bfk_svd_jacobi(nsmall, dAo, dM, dN, dA, dVo, dV, dSo, dS, 60);
-  free(bigIdx); free(sA); free(sV); free(sS); free(sM); free(sN);
+  free(bigIdx); free(tallIdx); free(sA); free(sV); free(sS); free(sM); free(sN);
   return rc;
 }
 

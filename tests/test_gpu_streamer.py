@@ -21,7 +21,11 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("m,n", [(40, 8), (300, 32), (17, 64), (128, 128), (5, 1), (1000, 20), (600, 300)])
+# shapes for every path of the batched SVD: one workgroup per matrix (small),
+# QR then Jacobi in LDS (tall: m >= 2n, n <= 70, >= 2^15 entries), one launch
+# per Jacobi step (large)
+@pytest.mark.parametrize("m,n", [(40, 8), (300, 32), (17, 64), (128, 128), (5, 1), (1000, 20), (600, 300),
+                                 (2048, 64), (700, 50), (64, 1500)])
 def test_svd_batched_matches_numpy(gpu_lib, m, n):
     bf = gpu_lib
     rng = np.random.default_rng(m * 1000 + n)
