@@ -1189,6 +1189,52 @@ __global__ __launch_bounds__(256) void k_svd_big_norms(const int32_t *__restrict
   if (threadIdx.x == 0) sig[sOff[b] + c] = sqrt(red[0] + red[1] + red[2] + red[3]);
 }
 
+/* y -= v (v^H y) inv over rows [j, m) of one column by one wave, four
+ * rows per lane in flight */
+__device__ __forceinline__ void house_apply_wave(const double2 *__restrict__ x, double2 *__restrict__ y, int j, int m,
+                                                 double inv, int lane) {
+  double dx = 0, dy = 0, ex = 0, ey = 0; /* v^H y in two chains */
+  int r = j + lane;
+  for (; r + 192 < m; r += 256) {
+    double2 v[4], t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { v[u] = x[r + 64 * u]; t[u] = y[r + 64 * u]; }
+#pragma unroll
+    for (int u = 0; u < 4; u += 2) {
+      dx = fma(v[u].x, t[u].x, fma(v[u].y, t[u].y, dx));
+      dy = fma(v[u].x, t[u].y, fma(-v[u].y, t[u].x, dy));
+      ex = fma(v[u + 1].x, t[u + 1].x, fma(v[u + 1].y, t[u + 1].y, ex));
+      ey = fma(v[u + 1].x, t[u + 1].y, fma(-v[u + 1].y, t[u + 1].x, ey));
+    }
+  }
+  for (; r < m; r += 64) {
+    const double2 v = x[r], t = y[r];
+    dx = fma(v.x, t.x, fma(v.y, t.y, dx));
+    dy = fma(v.x, t.y, fma(-v.y, t.x, dy));
+  }
+  dx = group_sum(dx + ex, 64) * inv;
+  dy = group_sum(dy + ey, 64) * inv;
+  r = j + lane;
+  for (; r + 192 < m; r += 256) {
+    double2 v[4], t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { v[u] = x[r + 64 * u]; t[u] = y[r + 64 * u]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      t[u].x -= v[u].x * dx - v[u].y * dy;
+      t[u].y -= v[u].x * dy + v[u].y * dx;
+      y[r + 64 * u] = t[u];
+    }
+  }
+  for (; r < m; r += 64) {
+    const double2 v = x[r];
+    double2 t = y[r];
+    t.x -= v.x * dx - v.y * dy;
+    t.y -= v.x * dy + v.y * dx;
+    y[r] = t;
+  }
+}
+
 /* SVD of a tall matrix (m >= 2n, n <= SVD_TALL_MAX_N) by one workgroup:
  * Householder QR of a copy S of A (global memory; each wave updates every
  * fourth column with a wave reduction), the triangle R into LDS, one-sided
@@ -1243,21 +1289,7 @@ __global__ __launch_bounds__(SVD_TALL_T) void k_svd_tall(const int32_t *__restri
       const double inv = 1.0 / (sg * (sg + ax0));
       for (int c = j + 1 + w; c < n; c += SVD_TALL_T / 64) {
         double2 *y = s + (int64_t)c * m;
-        double dx = 0, dy = 0; /* v^H y */
-        for (int r = j + lane; r < m; r += 64) {
-          const double2 v = x[r], t = y[r];
-          dx = fma(v.x, t.x, fma(v.y, t.y, dx));
-          dy = fma(v.x, t.y, fma(-v.y, t.x, dy));
-        }
-        dx = group_sum(dx, 64) * inv;
-        dy = group_sum(dy, 64) * inv;
-        for (int r = j + lane; r < m; r += 64) {
-          const double2 v = x[r];
-          double2 t = y[r];
-          t.x -= v.x * dx - v.y * dy;
-          t.y -= v.x * dy + v.y * dx;
-          y[r] = t;
-        }
+        house_apply_wave(x, y, j, m, inv, lane);
       }
       __syncthreads();
     }
@@ -1354,21 +1386,7 @@ __global__ __launch_bounds__(SVD_TALL_T) void k_svd_tall(const int32_t *__restri
     const double2 *x = s + (int64_t)j * m;
     for (int c = w; c < n; c += SVD_TALL_T / 64) {
       double2 *y = a + (int64_t)c * m;
-      double dx = 0, dy = 0; /* v^H y */
-      for (int r = j + lane; r < m; r += 64) {
-        const double2 v = x[r], t = y[r];
-        dx = fma(v.x, t.x, fma(v.y, t.y, dx));
-        dy = fma(v.x, t.y, fma(-v.y, t.x, dy));
-      }
-      dx = group_sum(dx, 64) * inv;
-      dy = group_sum(dy, 64) * inv;
-      for (int r = j + lane; r < m; r += 64) {
-        const double2 v = x[r];
-        double2 t = y[r];
-        t.x -= v.x * dx - v.y * dy;
-        t.y -= v.x * dy + v.y * dx;
-        y[r] = t;
-      }
+      house_apply_wave(x, y, j, m, inv, lane);
     }
     __syncthreads();
   }
