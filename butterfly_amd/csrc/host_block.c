@@ -121,7 +121,8 @@ static BfSize num_blocks(BfMat const *m) {
 
 /* the dense leaves of `m` placed at (i0, j0), as column-major copies
  * (kept in `bufs` until the product is built); -1 for a leaf that is not
- * dense complex or a container of such, -2 out of memory */
+ * dense complex or a container of such (or a block whose shape is not its
+ * cell's), -2 out of memory, -4 a block index outside the grid */
 static int collect(BfMat const *m, int64_t i0, int64_t j0, RefVec *refs, BufVec *bufs) {
   if (!m) return -1;
   const BfType t = bfMatGetType(m);
@@ -143,7 +144,7 @@ static int collect(BfMat const *m, int64_t i0, int64_t j0, RefVec *refs, BufVec 
   const BfSize nb = num_blocks(m);
   for (BfSize b = 0; b < nb; ++b) {
     BfSize r, c;
-    if (!block_at(m, b, &r, &c)) return -1;
+    if (!block_at(m, b, &r, &c)) return -4; /* a block index outside the grid */
     if (!B->block[b]) continue; /* an unset block is zero */
     const int64_t bi = (int64_t)B->rowOffset[r], bj = (int64_t)B->colOffset[c];
     if (B->block[b]->numRows != B->rowOffset[r + 1] - B->rowOffset[r] ||
@@ -190,7 +191,11 @@ static BfMat *pack(BfMat const *m) {
   }
   BfMat *P = NULL;
   if (!rc) P = bfk_product_from_blocks(nf, rows, cols, nb, bl);
-  else bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : rc == -3 ? BF_ERROR_INCOMPATIBLE_SHAPES : BF_ERROR_NOT_IMPLEMENTED);
+  else
+    bfSetError(rc == -2   ? BF_ERROR_MEMORY_ERROR
+               : rc == -3 ? BF_ERROR_INCOMPATIBLE_SHAPES
+               : rc == -4 ? BF_ERROR_INVALID_ARGUMENTS
+                          : BF_ERROR_NOT_IMPLEMENTED);
   for (size_t i = 0; i < bufs.n; ++i) free(bufs.a[i]);
   BF_VEC_FREE(bufs);
   for (size_t f = 0; f < nf; ++f) BF_VEC_FREE(refs[f]);

@@ -112,7 +112,8 @@ static float *standardize(BfhipRankModel const *m, BfSize n, float const *x) {
 
 int bfhipRankModelTrain(BfhipRankModel *model, BfSize n, float const *features, float const *ranks, BfSize epochs,
                         BfSize batch, float lr, float *lastLoss) {
-  if (!model || !features || !ranks || n == 0 || batch == 0 || batch > (1u << 30)) {
+  /* sample indices are 32-bit on the device */
+  if (!model || !features || !ranks || n == 0 || n > (BfSize)INT32_MAX || batch == 0 || batch > (1u << 30)) {
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return -1;
   }
