@@ -1603,8 +1603,9 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 
-/* Launch shape of the 1-RHS apply: waves (segments) per block and whether
- * a lane runs four independent FMA chains; set by bfk_set_apply_config. */
+/* Launch shape of the 1-RHS apply: waves (segments) per block and the lane
+ * map (0 = rows = segment height, 1 = the next power of two); set by
+ * bfk_set_apply_config. */
 static int g_applyWaves = 1;
 static int g_applyUnroll = 0;
 /* right-hand-side tiles of 16 per wave in the MFMA apply (0 = off) */
@@ -1618,10 +1619,11 @@ __constant__ int g_foldMaskDev = 0;
 
 /* One stage of the apply for one right-hand side.  Each wave of the block
  * owns one output row segment: its lanes split into R rows x G = 64/R
- * column groups (R the next power of two >= the segment height, at most
- * 64); a lane walks its columns of every term (column-major blocks, so the
- * R lanes of a group read R consecutive entries), then the G partial sums
- * meet in a shuffle reduction and the row is written once.  With LO the
+ * column groups (R = the segment height, at most 64; the compensated FP32
+ * form and the UNROLL variant use the next power of two); a lane walks its
+ * columns of every term (column-major blocks, so the R lanes of a group
+ * read R consecutive entries), then the G partial sums meet through lane
+ * reads and the row is written once.  With LO the
  * factors are an FP32 pair hi + lo.  With COMP (FP32 only) every lane
  * sums its FP32 products in FP64 and the lanes of a row meet in FP64, so
  * the rounding of a stage no longer grows with the length of its rows. */
@@ -1697,47 +1699,49 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
     }
     return;
   }
-  for (int rb = 0; rb < m; rb += R) {
-    const int i = rb + r;
-    V a0 = czero<V>(), a1 = czero<V>(), a2 = czero<V>(), a3 = czero<V>();
-    if (i < m) {
+  /* the tight lane map: R = m rows (not the next power of two), G = 64 / m
+   * column groups, so a segment of 17 rows keeps 51 lanes busy instead of
+   * 34 (4.4% faster at N = 2^20); the groups meet through lane reads.
+   * UNROLL selects the power-of-two map with xor reductions instead. */
+  constexpr bool TIGHT = !UNROLL;
+  const int Rt = TIGHT ? (m < 64 ? m : 64) : R;
+  const int Gt = TIGHT ? 64 / Rt : G;
+  const int rt = TIGHT ? lane % Rt : r;
+  const int gt = TIGHT ? lane / Rt : g;
+  for (int rb = 0; rb < m; rb += Rt) {
+    const int i = rb + rt;
+    V a0 = czero<V>();
+    if (i < m && gt < Gt) {
       for (int t = S.t0; t < S.t1; ++t) {
         const BfkTerm T = terms[t];
         const V *x = bufs.in[T.xbuf] + T.xOff;
         if (T.aOff < 0) {
-          if (g == 0) a0 = cadd(a0, x[i]);
+          if (gt == 0) a0 = cadd(a0, x[i]);
           continue;
         }
         const int64_t lda = T.lda;
         const V *A = hi + T.aOff + T.rowOff + i;
         const int n = T.n;
-        int j = g;
-        /* two independent chains: two factor loads in flight per lane at
-         * the register count of one (occupancy stays at 8 waves/SIMD) */
-        if (UNROLL) for (; j + G < n; j += 2 * G) {
-          const V m0 = A[(int64_t)j * lda], m1 = A[(int64_t)(j + G) * lda];
-          const V x0 = x[j], x1 = x[j + G];
-          a0 = cfmaT(m0, x0, a0);
-          a1 = cfmaT(m1, x1, a1);
-          if (LO) {
-            const V *B = lo + T.aOff + T.rowOff + i;
-            a0 = cfmaT(B[(int64_t)j * lda], x0, a0);
-            a1 = cfmaT(B[(int64_t)(j + G) * lda], x1, a1);
-          }
-        }
-        for (; j < n; j += G) {
+        for (int j = gt; j < n; j += Gt) {
           V xj = x[j];
           a0 = cfmaT(A[(int64_t)j * lda], xj, a0);
           if (LO) a0 = cfmaT(lo[T.aOff + T.rowOff + i + (int64_t)j * lda], xj, a0);
         }
       }
     }
-    V acc = cadd(cadd(a0, a1), cadd(a2, a3));
-    for (int o = R; o < 64; o <<= 1) {
-      acc.x += __shfl_xor(acc.x, o, 64);
-      acc.y += __shfl_xor(acc.y, o, 64);
+    V acc = a0;
+    if (TIGHT) {
+      for (int k = 1; k < Gt; ++k) { /* lanes rt + k Rt hold the other groups */
+        acc.x += __shfl(a0.x, rt + k * Rt, 64);
+        acc.y += __shfl(a0.y, rt + k * Rt, 64);
+      }
+    } else {
+      for (int o = R; o < 64; o <<= 1) {
+        acc.x += __shfl_xor(acc.x, o, 64);
+        acc.y += __shfl_xor(acc.y, o, 64);
+      }
     }
-    if (g == 0 && i < m) y[i] = acc;
+    if (gt == 0 && i < m) y[i] = acc;
   }
 }
 

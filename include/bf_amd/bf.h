@@ -388,8 +388,9 @@ void bfhipGetKernelTimes(double out[8]);
 void bfhipGetKernelTimeSums(double sums[8], double counts[8]);
 void bfhipResetKernelTimes(void);
 /* new: launch shapes of the apply, for tuning: the 1-RHS apply's waves per
- * block in {1, 2, 4} (one row segment per wave) and four independent FMA
- * chains per lane when `unroll`; the many-RHS apply's right-hand-side
+ * block in {1, 2, 4} (one row segment per wave) and, when `unroll` is 1,
+ * the power-of-two lane map instead of the default map whose rows match
+ * the segment height; the many-RHS apply's right-hand-side
  * tiles of 16 per wave on the matrix cores in {2, 4, 8}, or 0 for the
  * vector-unit kernel */
 int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
