@@ -1644,16 +1644,18 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
   V *y = bufs.out[S.ybuf] + S.yOff;
   if constexpr (COMP) {
     /* FP32 products summed in FP64 (two chains), reduced across the
-     * column groups in FP64, rounded to FP32 once */
-    for (int rb = 0; rb < m; rb += R) {
-      const int i = rb + r;
+     * column groups in FP64, rounded to FP32 once; the tight lane map
+     * below (rows = segment height) */
+    const int Rc = m < 64 ? m : 64, Gc = 64 / Rc, rc = lane % Rc, gc = lane / Rc;
+    for (int rb = 0; rb < m; rb += Rc) {
+      const int i = rb + rc;
       double sx0 = 0, sy0 = 0, sx1 = 0, sy1 = 0;
-      if (i < m) {
+      if (i < m && gc < Gc) {
         for (int t = S.t0; t < S.t1; ++t) {
           const BfkTerm T = terms[t];
           const V *x = bufs.in[T.xbuf] + T.xOff;
           if (T.aOff < 0) {
-            if (g == 0) {
+            if (gc == 0) {
               sx0 += (double)x[i].x;
               sy0 += (double)x[i].y;
             }
@@ -1663,13 +1665,13 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
           const V *A = hi + T.aOff + T.rowOff + i;
           const V *B = lo + T.aOff + T.rowOff + i;
           const int n = T.n;
-          int j = g;
-          for (; j + G < n; j += 2 * G) {
-            const V x0 = x[j], x1 = x[j + G];
-            V p0 = cmulf(A[(int64_t)j * lda], x0), p1 = cmulf(A[(int64_t)(j + G) * lda], x1);
+          int j = gc;
+          for (; j + Gc < n; j += 2 * Gc) {
+            const V x0 = x[j], x1 = x[j + Gc];
+            V p0 = cmulf(A[(int64_t)j * lda], x0), p1 = cmulf(A[(int64_t)(j + Gc) * lda], x1);
             if (LO) {
               p0 = cfmaf(B[(int64_t)j * lda], x0, p0);
-              p1 = cfmaf(B[(int64_t)(j + G) * lda], x1, p1);
+              p1 = cfmaf(B[(int64_t)(j + Gc) * lda], x1, p1);
             }
             sx0 += (double)p0.x;
             sy0 += (double)p0.y;
@@ -1685,12 +1687,13 @@ __global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs,
           }
         }
       }
-      double ax = sx0 + sx1, ay = sy0 + sy1;
-      for (int o = R; o < 64; o <<= 1) {
-        ax += __shfl_xor(ax, o, 64);
-        ay += __shfl_xor(ay, o, 64);
+      const double px = sx0 + sx1, py = sy0 + sy1;
+      double ax = px, ay = py;
+      for (int k = 1; k < Gc; ++k) { /* lanes rc + k Rc hold the other groups */
+        ax += __shfl(px, rc + k * Rc, 64);
+        ay += __shfl(py, rc + k * Rc, 64);
       }
-      if (g == 0 && i < m) {
+      if (gc == 0 && i < m) {
         V out;
         out.x = (decltype(out.x))ax;
         out.y = (decltype(out.y))ay;
