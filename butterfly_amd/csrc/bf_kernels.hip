@@ -1628,7 +1628,7 @@ __constant__ int g_foldMaskDev = 0;
  * sums its FP32 products in FP64 and the lanes of a row meet in FP64, so
  * the rounding of a stage no longer grows with the length of its rows. */
 template <typename V, bool LO, bool UNROLL, bool COMP = false>
-__global__ __launch_bounds__(256) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
                                                 const BfkTerm *__restrict__ terms, const V *__restrict__ hi,
                                                 const V *__restrict__ lo, typename BufsOf<V>::T bufs) {
   const int lane = threadIdx.x & 63;
