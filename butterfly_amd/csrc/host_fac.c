@@ -497,10 +497,13 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
     }
   if (!rc) rc = bfk_lstsq_begin(bigBytes, numBig);
   const int begun = rc == 0;
-  const size_t chunk = (size_t)1 << 22; /* jobs per upload */
+  /* jobs per upload: small at first so that the kernels start early, then
+   * doubling up to 4M */
+  size_t chunk = (size_t)1 << 18;
   size_t up = 0, launched = 0, g = 0;
   while (!rc && up < nj) {
     const size_t n = nj - up < chunk ? nj - up : chunk;
+    if (chunk < ((size_t)1 << 22)) chunk *= 2;
     rc = bfk_h2d_side(dL + up, sorted + up, n * sizeof(BfkLstsqJob));
     up += n;
     /* what is uploaded: the rest of the current class, the classes that
