@@ -1190,9 +1190,11 @@ __global__ __launch_bounds__(256) void k_svd_big_norms(const int32_t *__restrict
 }
 
 /* y -= v (v^H y) inv over rows [j, m) of one column by one wave, four
- * rows per lane in flight */
-__device__ __forceinline__ void house_apply_wave(const double2 *__restrict__ x, double2 *__restrict__ y, int j, int m,
-                                                 double inv, int lane) {
+ * rows per lane in flight; with NORM, returns |y|^2 over rows (j, m) after
+ * the update (the wave's total in every lane) */
+template <bool NORM = false>
+__device__ __forceinline__ double house_apply_wave(const double2 *__restrict__ x, double2 *__restrict__ y, int j, int m,
+                                                   double inv, int lane) {
   double dx = 0, dy = 0, ex = 0, ey = 0; /* v^H y in two chains */
   int r = j + lane;
   for (; r + 192 < m; r += 256) {
@@ -1215,6 +1217,7 @@ __device__ __forceinline__ void house_apply_wave(const double2 *__restrict__ x, 
   dx = group_sum(dx + ex, 64) * inv;
   dy = group_sum(dy + ey, 64) * inv;
   r = j + lane;
+  double nq = 0;
   for (; r + 192 < m; r += 256) {
     double2 v[4], t[4];
 #pragma unroll
@@ -1224,6 +1227,7 @@ __device__ __forceinline__ void house_apply_wave(const double2 *__restrict__ x, 
       t[u].x -= v[u].x * dx - v[u].y * dy;
       t[u].y -= v[u].x * dy + v[u].y * dx;
       y[r + 64 * u] = t[u];
+      if (NORM && r + 64 * u > j) nq = fma(t[u].x, t[u].x, fma(t[u].y, t[u].y, nq));
     }
   }
   for (; r < m; r += 64) {
@@ -1232,162 +1236,199 @@ __device__ __forceinline__ void house_apply_wave(const double2 *__restrict__ x, 
     t.x -= v.x * dx - v.y * dy;
     t.y -= v.x * dy + v.y * dx;
     y[r] = t;
+    if (NORM && r > j) nq = fma(t.x, t.x, fma(t.y, t.y, nq));
   }
+  return NORM ? group_sum(nq, 64) : 0.0;
 }
 
-/* SVD of a tall matrix (m >= 2n, n <= SVD_TALL_MAX_N) by one workgroup:
- * Householder QR of a copy S of A (global memory; each wave updates every
- * fourth column with a wave reduction), the triangle R into LDS, one-sided
- * Jacobi on the columns of R with V accumulated in LDS (the pairs, order,
- * rotation and stopping rule of k_svd_jacobi), then W = Q [R V; 0] into A:
- * W = A V, so its columns are sigma_j u_j as k_svd_jacobi's.
- * The rotations touch n rows instead of m; the per-step launches of
- * k_svd_step are not needed for these shapes. */
+/* SVD of a tall matrix (m >= 2n, n <= SVD_TALL_MAX_N), in three launches:
+ *   k_tall_qr      Householder QR with column pivoting of a copy S of A
+ *                  (global memory; each wave takes every 16th column of a
+ *                  step), A P = Q R; R, the pivots and the reflectors' scales
+ *                  go to a per-matrix scratch slot
+ *   k_tall_jacobi  one wave: one-sided Jacobi on the rows of [R | I] in LDS
+ *                  (jacobi_rows, as for the re-expansion blocks): G R = R'
+ *                  with orthogonal rows sigma_i v_i^H, G accumulated
+ *   k_tall_w       sigma_i = |row i of R'|, V = P [conj(R') / sigma]^T, and
+ *                  W = Q [sigma_i conj(G[i, :]); 0] (reflectors applied last
+ *                  first), so A V = W and W's columns keep their mutual
+ *                  orthogonality relative to their own norms
+ * The pivoted QR grades R's rows, so the row Jacobi needs few sweeps; the
+ * rotations touch n-row operands instead of m. */
 #define SVD_TALL_MAX_N 70
-#define SVD_TALL_T 1024 /* threads: 16 waves share the column updates of the QR */
-__host__ __device__ __forceinline__ size_t svd_tall_lds(int n) { return (size_t)2 * n * n * 16 + 64; }
+#define SVD_TALL_T 1024 /* threads of the QR and W launches */
+__host__ __device__ __forceinline__ int tall_ld(int n) { return 2 * n + 1; }
+__host__ __device__ __forceinline__ size_t svd_tall_lds(int n) { return (size_t)n * tall_ld(n) * 16; }
+/* scratch slot of one matrix: [R | I] then [R' | G], n x (2n + 1) */
+#define TALL_SLOT ((int64_t)SVD_TALL_MAX_N * (2 * SVD_TALL_MAX_N + 1))
 
-__global__ __launch_bounds__(SVD_TALL_T) void k_svd_tall(const int32_t *__restrict__ idx, const int64_t *__restrict__ aOff,
-                                                  const int32_t *__restrict__ mm, const int32_t *__restrict__ nn,
-                                                  double2 *__restrict__ A, double2 *__restrict__ S,
-                                                  const int64_t *__restrict__ vOff, double2 *__restrict__ Vout,
-                                                  const int64_t *__restrict__ sOff, double *__restrict__ sig,
-                                                  int maxSweeps) {
-  extern __shared__ double2 lds[];
-  __shared__ double red[SVD_TALL_T / 64];
-  __shared__ double hinv[SVD_TALL_MAX_N]; /* 1 / (sig (sig + |x_j|)) of reflector j, 0 = none */
-  __shared__ int anyRot;
-  const int b = idx[blockIdx.x];
+__global__ __launch_bounds__(SVD_TALL_T) void k_tall_qr(const int32_t *__restrict__ idx,
+                                                        const int64_t *__restrict__ aOff,
+                                                        const int32_t *__restrict__ mm,
+                                                        const int32_t *__restrict__ nn, const double2 *__restrict__ A,
+                                                        double2 *__restrict__ S, double2 *__restrict__ RG,
+                                                        int32_t *__restrict__ pivg, double *__restrict__ hinvg) {
+  __shared__ double cn[SVD_TALL_MAX_N];
+  __shared__ double2 rdiag[SVD_TALL_MAX_N];
+  __shared__ int pv[SVD_TALL_MAX_N];
+  __shared__ int qsel;
+  __shared__ double qnorm;
+  const int t = blockIdx.x, b = idx[t];
   const int m = mm[b], n = nn[b];
-  double2 *a = A + aOff[b], *s = S + aOff[b];
-  double2 *R = lds, *V = lds + n * n; /* column-major n x n */
+  const double2 *a = A + aOff[b];
+  double2 *s = S + aOff[b];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const double eps = 2.220446049250313e-16;
+  constexpr int NW = SVD_TALL_T / 64;
   for (int64_t e = tid; e < (int64_t)m * n; e += SVD_TALL_T) s[e] = a[e];
+  for (int j = tid; j < n; j += SVD_TALL_T) pv[j] = j;
   __syncthreads();
-  /* 1. Householder QR of S: column j's reflector v = x + ph sig e_j,
-   * H = I - v v^H / (sig (sig + |x_j|)), R_jj = -ph sig */
-  for (int j = 0; j < n; ++j) {
-    double2 *x = s + (int64_t)j * m;
+  /* squared norms of the columns over rows j.. : taken here for j = 0, then
+   * by the updates of step j - 1 */
+  for (int c = w; c < n; c += NW) {
+    const double2 *col = s + (int64_t)c * m;
     double q = 0;
-    for (int r = j + tid; r < m; r += SVD_TALL_T) q = fma(x[r].x, x[r].x, fma(x[r].y, x[r].y, q));
+    for (int r = lane; r < m; r += 64) q = fma(col[r].x, col[r].x, fma(col[r].y, col[r].y, q));
     q = group_sum(q, 64);
-    if (lane == 0) red[w] = q;
+    if (lane == 0) cn[c] = q;
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      int best = j;
+      for (int c = j + 1; c < n; ++c)
+        if (cn[c] > cn[best]) best = c;
+      qsel = best;
+      qnorm = cn[best];
+    }
     __syncthreads();
-    double tot = 0;
-#pragma unroll
-    for (int q2 = 0; q2 < SVD_TALL_T / 64; ++q2) tot += red[q2];
-    const double sg = sqrt(tot);
+    const int q = qsel;
+    if (q != j) { /* whole columns: the rows above j hold R */
+      double2 *cj = s + (int64_t)j * m, *cq = s + (int64_t)q * m;
+      for (int r = tid; r < m; r += SVD_TALL_T) {
+        const double2 u = cj[r];
+        cj[r] = cq[r];
+        cq[r] = u;
+      }
+      if (tid == 0) {
+        const int u = pv[j];
+        pv[j] = pv[q];
+        pv[q] = u;
+        cn[q] = cn[j];
+      }
+    }
+    __syncthreads();
+    double2 *x = s + (int64_t)j * m;
+    const double sg = sqrt(qnorm);
     const double2 x0 = x[j];
     const double ax0 = hypot(x0.x, x0.y);
     const double phx = ax0 > 0 ? x0.x / ax0 : 1.0, phy = ax0 > 0 ? x0.y / ax0 : 0.0;
     __syncthreads();
-    if (tid == 0) hinv[j] = sg > 0 ? 1.0 / (sg * (sg + ax0)) : 0.0;
+    if (tid == 0) {
+      hinvg[(int64_t)t * SVD_TALL_MAX_N + j] = sg > 0 ? 1.0 / (sg * (sg + ax0)) : 0.0;
+      rdiag[j] = sg > 0 ? make_double2(-phx * sg, -phy * sg) : x0;
+    }
     if (sg > 0) {
       if (tid == 0) x[j] = make_double2(x0.x + phx * sg, x0.y + phy * sg);
       __syncthreads();
       const double inv = 1.0 / (sg * (sg + ax0));
-      for (int c = j + 1 + w; c < n; c += SVD_TALL_T / 64) {
-        double2 *y = s + (int64_t)c * m;
-        house_apply_wave(x, y, j, m, inv, lane);
+      for (int c = j + 1 + w; c < n; c += NW) {
+        const double q2 = house_apply_wave<true>(x, s + (int64_t)c * m, j, m, inv, lane);
+        if (lane == 0) cn[c] = q2;
       }
-      __syncthreads();
+    } else {
+      for (int c = j + 1 + w; c < n; c += NW) /* no reflector: drop row j from the norms */
+        if (lane == 0) cn[c] = fmax(cn[c] - (s[(int64_t)c * m + j].x * s[(int64_t)c * m + j].x +
+                                             s[(int64_t)c * m + j].y * s[(int64_t)c * m + j].y), 0.0);
     }
-    /* column j of R: the rows above j are final, R_jj = -ph sig */
-    for (int r = tid; r < n; r += SVD_TALL_T)
-      R[j * n + r] = r < j ? x[r] : (r == j ? (sg > 0 ? make_double2(-phx * sg, -phy * sg) : x0)
-                                            : make_double2(0, 0));
     __syncthreads();
   }
-  for (int e = tid; e < n * n; e += SVD_TALL_T) V[e] = make_double2((e % (n + 1)) == 0 ? 1.0 : 0.0, 0.0);
+  /* [R | I], row-major with leading dimension 2n + 1 */
+  const int ld = tall_ld(n);
+  double2 *rg = RG + (int64_t)t * TALL_SLOT;
+  for (int e = tid; e < n * 2 * n; e += SVD_TALL_T) {
+    const int i = e / (2 * n), c = e - i * 2 * n;
+    double2 v;
+    if (c < n) v = c > i ? s[(int64_t)c * m + i] : (c == i ? rdiag[i] : make_double2(0, 0));
+    else v = make_double2(c - n == i ? 1.0 : 0.0, 0.0);
+    rg[i * ld + c] = v;
+  }
+  for (int j = tid; j < n; j += SVD_TALL_T) pivg[(int64_t)t * SVD_TALL_MAX_N + j] = pv[j];
+}
+
+__global__ __launch_bounds__(64) void k_tall_jacobi(const int32_t *__restrict__ idx, const int32_t *__restrict__ nn,
+                                                    double2 *__restrict__ RG) {
+  extern __shared__ double2 lds[];
+  const int t = blockIdx.x, n = nn[idx[t]], ld = tall_ld(n);
+  double2 *rg = RG + (int64_t)t * TALL_SLOT;
+  for (int e = threadIdx.x; e < n * ld; e += 64) lds[e] = rg[e];
   __syncthreads();
-  /* 2. one-sided Jacobi on the columns of R, V accumulated */
-  const int P = n + (n & 1), npairs = P / 2;
-  int g = 64;
-  while (g > 1 && g * npairs > SVD_TALL_T) g >>= 1;
-  const int slots = SVD_TALL_T / g, slot = tid / g, sub = tid - slot * g;
-  for (int sweep = 0; sweep < maxSweeps && n > 1; ++sweep) {
-    if (tid == 0) anyRot = 0;
-    __syncthreads();
-    for (int step = 0; step < P - 1; ++step) {
-      for (int pb = 0; pb < npairs; pb += slots) {
-        const int pr = pb + slot;
-        int i = -1, j = -1;
-        if (pr < npairs) {
-          const int p0 = rr_row(pr, step, P), p1 = rr_row(P - 1 - pr, step, P);
-          i = min(p0, p1);
-          j = max(p0, p1);
-        }
-        const bool valid = pr < npairs && j < n;
-        double al = 0, be = 0, gx = 0, gy = 0;
-        if (valid) {
-          const double2 *ci = R + i * n, *cj = R + j * n;
-          for (int r = sub; r < n; r += g) {
-            const double2 x = ci[r], y = cj[r];
-            al = fma(x.x, x.x, fma(x.y, x.y, al));
-            be = fma(y.x, y.x, fma(y.y, y.y, be));
-            gx = fma(x.x, y.x, fma(x.y, y.y, gx));
-            gy = fma(x.x, y.y, fma(-x.y, y.x, gy));
-          }
-        }
-        for (int o = 1; o < g; o <<= 1) {
-          al += __shfl_xor(al, o, 64);
-          be += __shfl_xor(be, o, 64);
-          gx += __shfl_xor(gx, o, 64);
-          gy += __shfl_xor(gy, o, 64);
-        }
-        const double ag = sqrt(gx * gx + gy * gy);
-        if (valid && ag > 0 && ag > eps * sqrt(al * be)) {
-          if (sub == 0) anyRot = 1;
-          const double ex = gx / ag, ey = -gy / ag;
-          const double zeta = (be - al) / (2 * ag);
-          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
-          const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
-          double2 *ci = R + i * n, *cj = R + j * n, *vi = V + i * n, *vj = V + j * n;
-          for (int r = sub; r < n; r += g) {
-            double2 x = ci[r], y = cj[r];
-            double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x;
-            ci[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
-            cj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
-            x = vi[r];
-            y = vj[r];
-            ux = ex * y.x - ey * y.y;
-            uy = ex * y.y + ey * y.x;
-            vi[r] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
-            vj[r] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
-          }
-        }
-        __syncthreads();
-      }
-    }
-    if (!anyRot) break;
-    __syncthreads();
+  const int npairs = (n + (n & 1)) / 2;
+  int Lp = 64;
+  while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
+  switch (Lp) {
+    case 8: jacobi_rows<8>(lds, n, 2 * n, ld); break;
+    case 4: jacobi_rows<4>(lds, n, 2 * n, ld); break;
+    case 2: jacobi_rows<2>(lds, n, 2 * n, ld); break;
+    case 1: jacobi_rows<1>(lds, n, 2 * n, ld); break;
+    case 16: jacobi_rows<16>(lds, n, 2 * n, ld); break;
+    case 32: jacobi_rows<32>(lds, n, 2 * n, ld); break;
+    default: jacobi_rows<64>(lds, n, 2 * n, ld); break;
   }
-  /* 3. sigma from the columns of R V, V out, W = Q [R V; 0] into A (the
-   * reflectors applied last first: each column keeps its orthogonality to
-   * the others relative to its own norm, as in k_svd_jacobi) */
-  double *sv = sig + sOff[b];
-  for (int j = slot; j < n; j += slots) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * ld; e += 64) rg[e] = lds[e];
+}
+
+__global__ __launch_bounds__(SVD_TALL_T) void k_tall_w(const int32_t *__restrict__ idx,
+                                                       const int64_t *__restrict__ aOff,
+                                                       const int32_t *__restrict__ mm, const int32_t *__restrict__ nn,
+                                                       double2 *__restrict__ A, const double2 *__restrict__ S,
+                                                       const double2 *__restrict__ RG, const int32_t *__restrict__ pivg,
+                                                       const double *__restrict__ hinvg,
+                                                       const int64_t *__restrict__ vOff, double2 *__restrict__ Vout,
+                                                       const int64_t *__restrict__ sOff, double *__restrict__ sig) {
+  __shared__ double sgm[SVD_TALL_MAX_N];
+  const int t = blockIdx.x, b = idx[t];
+  const int m = mm[b], n = nn[b], ld = tall_ld(n);
+  double2 *a = A + aOff[b];
+  const double2 *s = S + aOff[b], *rg = RG + (int64_t)t * TALL_SLOT;
+  const int32_t *pv = pivg + (int64_t)t * SVD_TALL_MAX_N;
+  const double *hinv = hinvg + (int64_t)t * SVD_TALL_MAX_N;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr int NW = SVD_TALL_T / 64;
+  for (int i = w; i < n; i += NW) {
     double q = 0;
-    for (int r = sub; r < n; r += g) q = fma(R[j * n + r].x, R[j * n + r].x, fma(R[j * n + r].y, R[j * n + r].y, q));
-    for (int o = 1; o < g; o <<= 1) q += __shfl_xor(q, o, 64);
-    if (sub == 0) sv[j] = sqrt(q);
+    for (int c = lane; c < n; c += 64) q = fma(rg[i * ld + c].x, rg[i * ld + c].x, fma(rg[i * ld + c].y, rg[i * ld + c].y, q));
+    q = group_sum(q, 64);
+    if (lane == 0) {
+      sgm[i] = sqrt(q);
+      sig[sOff[b] + i] = sqrt(q);
+    }
   }
+  __syncthreads();
+  /* V column i = P conj(row i of R') / sigma_i */
   double2 *vo = Vout + vOff[b];
-  for (int e = tid; e < n * n; e += SVD_TALL_T) vo[e] = V[e];
+  for (int e = tid; e < n * n; e += SVD_TALL_T) {
+    const int i = e / n, j = e - i * n;
+    const double inv = sgm[i] > 0 ? 1.0 / sgm[i] : 0.0;
+    const double2 r = rg[i * ld + j];
+    vo[(int64_t)i * n + pv[j]] = make_double2(r.x * inv, -r.y * inv);
+  }
+  /* W = Q [sigma_i conj(G[i, :]); 0] */
   for (int64_t e = tid; e < (int64_t)m * n; e += SVD_TALL_T) {
     const int c = (int)(e / m), r = (int)(e - (int64_t)c * m);
-    a[e] = r < n ? R[c * n + r] : make_double2(0, 0);
+    double2 v = make_double2(0, 0);
+    if (r < n) {
+      const double2 g = rg[c * ld + n + r];
+      v = make_double2(sgm[c] * g.x, -sgm[c] * g.y);
+    }
+    a[e] = v;
   }
   __syncthreads();
   for (int j = n - 1; j >= 0; --j) {
     const double inv = hinv[j];
-    if (inv == 0) continue; /* uniform: hinv is in LDS */
+    if (inv == 0) continue; /* uniform */
     const double2 *x = s + (int64_t)j * m;
-    for (int c = w; c < n; c += SVD_TALL_T / 64) {
-      double2 *y = a + (int64_t)c * m;
-      house_apply_wave(x, y, j, m, inv, lane);
-    }
+    for (int c = w; c < n; c += NW) house_apply_wave(x, a + (int64_t)c * m, j, m, inv, lane);
     __syncthreads();
   }
 }
@@ -2005,8 +2046,8 @@ int bfk_init(void) {
   }
   /* the tall SVD kernel asks only for what its largest case needs (it has
    * a few static LDS words besides) */
-  g_tallOk = g_ldsCap >= svd_tall_lds(SVD_TALL_MAX_N) + 1024 &&
-             hipFuncSetAttribute((const void *)k_svd_tall, hipFuncAttributeMaxDynamicSharedMemorySize,
+  g_tallOk = g_ldsCap >= svd_tall_lds(SVD_TALL_MAX_N) &&
+             hipFuncSetAttribute((const void *)k_tall_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)svd_tall_lds(SVD_TALL_MAX_N)) == hipSuccess;
   if (!g_tallOk) (void)hipGetLastError();
   return 0;
@@ -2435,18 +2476,35 @@ int bfk_svd_tall_max_n(void) { return g_tallOk ? SVD_TALL_MAX_N : 0; }
 int bfk_svd_jacobi_tall(int64_t nb, const int32_t *dIdx, const int64_t *dAOff, const int32_t *dM, const int32_t *dN,
                         double *dA, double *dS, const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig,
                         int maxSweeps, int maxN) {
+  (void)maxSweeps;
   if (bfk_init()) return -1;
   if (nb <= 0) return 0;
-  if (!g_tallOk || maxN > SVD_TALL_MAX_N) return -3;
-  const int64_t maxGrid = 1 << 30;
-  for (int64_t b0 = 0; b0 < nb; b0 += maxGrid) {
-    const int64_t k = nb - b0 < maxGrid ? nb - b0 : maxGrid;
-    hipLaunchKernelGGL(k_svd_tall, dim3((unsigned)k), dim3(SVD_TALL_T), svd_tall_lds(maxN), g_stream, dIdx + b0, dAOff, dM,
-                       dN, (double2 *)dA, (double2 *)dS, dVOff, (double2 *)dV, dSOff, dSig, maxSweeps);
-    BFK_CHECK(hipGetLastError());
+  if (!g_tallOk || maxN > SVD_TALL_MAX_N || nb > (1 << 30)) return -3;
+  double2 *rg = nullptr;
+  int32_t *piv = nullptr;
+  double *hinv = nullptr;
+  BFK_CHECK(hipMalloc((void **)&rg, (size_t)nb * TALL_SLOT * 16));
+  BFK_CHECK(hipMalloc((void **)&piv, (size_t)nb * SVD_TALL_MAX_N * 4));
+  BFK_CHECK(hipMalloc((void **)&hinv, (size_t)nb * SVD_TALL_MAX_N * 8));
+  int rc = 0;
+  hipLaunchKernelGGL(k_tall_qr, dim3((unsigned)nb), dim3(SVD_TALL_T), 0, g_stream, dIdx, dAOff, dM, dN,
+                     (const double2 *)dA, (double2 *)dS, rg, piv, hinv);
+  if (hipGetLastError() != hipSuccess) rc = -1;
+  if (!rc) {
+    hipLaunchKernelGGL(k_tall_jacobi, dim3((unsigned)nb), dim3(64), svd_tall_lds(maxN), g_stream, dIdx, dN, rg);
+    if (hipGetLastError() != hipSuccess) rc = -1;
   }
-  BFK_CHECK(hipStreamSynchronize(g_stream));
-  return 0;
+  if (!rc) {
+    hipLaunchKernelGGL(k_tall_w, dim3((unsigned)nb), dim3(SVD_TALL_T), 0, g_stream, dIdx, dAOff, dM, dN,
+                       (double2 *)dA, (const double2 *)dS, (const double2 *)rg, piv, hinv, dVOff, (double2 *)dV, dSOff,
+                       dSig);
+    if (hipGetLastError() != hipSuccess) rc = -1;
+  }
+  if (hipStreamSynchronize(g_stream) != hipSuccess) rc = -1;
+  (void)hipFree(rg);
+  (void)hipFree(piv);
+  (void)hipFree(hinv);
+  return rc;
 }
 
 size_t bfk_ortho_units(int rows, int n) { return ortho_units(rows, n); }
