@@ -50,7 +50,7 @@ static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 static hipEvent_t g_evL0 = nullptr, g_evL1 = nullptr;
 static int g_lPending = 0;
 static size_t g_ldsCap = 0;
-static bool g_tallOk = false; /* k_svd_tall has its LDS */
+static bool g_tallOk = false; /* k_tall_jacobi has its LDS */
 static double g_times[8];   /* last measured ms per category */
 static double g_sum[8];     /* accumulated ms per category since the last reset */
 static double g_cnt[8];     /* number of measurements since the last reset */
@@ -2471,8 +2471,10 @@ int bfk_svd_jacobi_big(int64_t nb, const int32_t *dBig, const int64_t *dAOff, co
 
 int bfk_svd_tall_max_n(void) { return g_tallOk ? SVD_TALL_MAX_N : 0; }
 
-/* tall matrices idx[0 .. nb) of a batch (m >= 2n, n <= bfk_svd_tall_max_n()),
- * one workgroup each; dS is scratch of the size of dA */
+/* tall matrices idx[0 .. nb) of a batch (m >= 2n, n <= bfk_svd_tall_max_n()):
+ * k_tall_qr, k_tall_jacobi, k_tall_w, one workgroup per matrix each; dS is
+ * scratch of the size of dA; maxSweeps is not used (the row Jacobi stops by
+ * its own test, at most 40 sweeps) */
 int bfk_svd_jacobi_tall(int64_t nb, const int32_t *dIdx, const int64_t *dAOff, const int32_t *dM, const int32_t *dN,
                         double *dA, double *dS, const int64_t *dVOff, double *dV, const int64_t *dSOff, double *dSig,
                         int maxSweeps, int maxN) {
