@@ -4,6 +4,7 @@
  */
 #define _DEFAULT_SOURCE 1
 #include <math.h>
+#include <pthread.h>
 #include <time.h>
 
 #include "bf_host.h"
@@ -175,26 +176,34 @@ typedef struct {
   NodeVec nodes;
   BfPoint2 const *pts;
   BfSize *perm;
+  uint8_t *qc; /* quadrant of the point at each tree position, per split */
   BfSize leafSize;
+  int topLevels;  /* levels split here before the rest goes to threads */
+  BfI64Vec defer; /* nodes left for the threads */
 } TreeBuilder;
 
 /* Partition perm[lo:hi) by quadrant, visiting the index range in the same
  * order as the reference's in-place sift so that the tree order of points
  * inside every node matches it: for each of the first three quadrants,
  * skip the leading run that already belongs to it, then swap every later
- * member forward onto the boundary. */
+ * member forward onto the boundary.  The quadrants are computed once into
+ * qc (one gather of the points per split) and swapped along with perm. */
 static void sift(TreeBuilder *b, BfSize lo, BfSize hi, const BfReal *split, BfSize off[5]) {
   off[0] = lo;
   off[4] = hi;
   BfSize *perm = b->perm;
+  uint8_t *qc = b->qc;
+  for (BfSize j = lo; j < hi; ++j) qc[j] = (uint8_t)quadrant_of(b->pts[perm[j]], split);
   for (int q = 0; q < 3; ++q) {
     BfSize bd = lo;
-    while (bd < hi && quadrant_of(b->pts[perm[bd]], split) == q) ++bd;
+    while (bd < hi && qc[bd] == q) ++bd;
     for (BfSize j = bd + 1; j < hi; ++j) {
-      if (quadrant_of(b->pts[perm[j]], split) == q) {
-        BfSize t = perm[bd];
+      if (qc[j] == q) {
+        const BfSize t = perm[bd];
         perm[bd] = perm[j];
         perm[j] = t;
+        qc[j] = qc[bd];
+        qc[bd] = (uint8_t)q;
         ++bd;
       }
     }
@@ -230,8 +239,73 @@ static void build_node(TreeBuilder *b, size_t ni) {
     BF_VEC_PUSH(b->nodes, c);
     size_t ci = b->nodes.n - 1;
     b->nodes.a[ni].child[q] = (int32_t)ci;
-    if (cnt > b->leafSize) build_node(b, ci);
+    if (cnt > b->leafSize) {
+      if (c.depth < b->topLevels) build_node(b, ci);
+      else if (b->topLevels > 0) BF_VEC_PUSH(b->defer, (int64_t)ci);
+      else build_node(b, ci);
+    }
   }
+}
+
+/* The subtrees below the top levels are disjoint index ranges of perm and
+ * qc: each is built by its own builder (node 0 = its root) on a thread,
+ * then appended to the main node list.  The final breadth-first numbering
+ * depends only on the tree's shape, so the result is the serial one. */
+typedef struct {
+  TreeBuilder b;
+} SubtreeTask;
+
+static void *subtree_task(void *arg) {
+  SubtreeTask *t = arg;
+  build_node(&t->b, 0);
+  return NULL;
+}
+
+static int build_deferred(TreeBuilder *b) {
+  const size_t nd = b->defer.n;
+  if (!nd) return 0;
+  SubtreeTask *tasks = calloc(nd, sizeof(SubtreeTask));
+  pthread_t *th = malloc(nd * sizeof(pthread_t));
+  int *ok = calloc(nd, sizeof(int));
+  if (!tasks || !th || !ok) {
+    free(tasks); free(th); free(ok);
+    return -1;
+  }
+  for (size_t i = 0; i < nd; ++i) {
+    TreeBuilder *sb = &tasks[i].b;
+    sb->pts = b->pts;
+    sb->perm = b->perm;
+    sb->qc = b->qc;
+    sb->leafSize = b->leafSize;
+    BF_VEC_PUSH(sb->nodes, b->nodes.a[b->defer.a[i]]);
+  }
+  for (size_t i = 0; i < nd; ++i) ok[i] = pthread_create(&th[i], NULL, subtree_task, &tasks[i]) == 0;
+  for (size_t i = 0; i < nd; ++i) {
+    if (ok[i]) pthread_join(th[i], NULL);
+    else subtree_task(&tasks[i]);
+  }
+  for (size_t i = 0; i < nd; ++i) {
+    NodeVec *sn = &tasks[i].b.nodes;
+    const int32_t root = (int32_t)b->defer.a[i];
+    const int32_t base = (int32_t)b->nodes.n - 1; /* local k >= 1 goes to base + k */
+    for (size_t k = 0; k < sn->n; ++k) {
+      BfQuadtreeNode nd2 = sn->a[k];
+      for (int q = 0; q < 4; ++q)
+        if (nd2.child[q] >= 0) nd2.child[q] = nd2.child[q] + base;
+      if (k == 0) {
+        nd2.parent = b->nodes.a[root].parent;
+        b->nodes.a[root] = nd2;
+      } else {
+        nd2.parent = nd2.parent == 0 ? root : nd2.parent + base;
+        BF_VEC_PUSH(b->nodes, nd2);
+      }
+    }
+    BF_VEC_FREE(*sn);
+  }
+  free(tasks);
+  free(th);
+  free(ok);
+  return 0;
 }
 
 BfQuadtree *bfQuadtreeNew(void) {
@@ -256,7 +330,15 @@ void bfQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points, BfVec
   b.pts = (BfPoint2 const *)points->data;
   b.leafSize = leafSize;
   b.perm = malloc(n * sizeof(BfSize));
-  if (!b.perm) { bfSetError(BF_ERROR_MEMORY_ERROR); return; }
+  b.qc = malloc(n);
+  if (!b.perm || !b.qc) {
+    free(b.perm);
+    free(b.qc);
+    bfSetError(BF_ERROR_MEMORY_ERROR);
+    return;
+  }
+  /* large trees: two levels here, up to 16 subtrees on threads */
+  b.topLevels = n >= ((BfSize)1 << 16) ? 2 : 0;
   for (BfSize i = 0; i < n; ++i) b.perm[i] = i;
 
   /* root box: bounding box stretched to a square about its centre
@@ -290,6 +372,9 @@ void bfQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points, BfVec
   root.index = -1;
   BF_VEC_PUSH(b.nodes, root);
   build_node(&b, 0);
+  if (build_deferred(&b)) { bfSetError(BF_ERROR_MEMORY_ERROR); return; }
+  BF_VEC_FREE(b.defer);
+  free(b.qc);
 
   /* renumber breadth-first, children in quadrant order (the reference's
    * LR level order, tree_level_iter.c:10-43) */

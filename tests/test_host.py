@@ -63,6 +63,19 @@ def test_quadtree_perm_matches_oracle(leaf):
         assert t.num_nodes == len(nodes)
 
 
+@pytest.mark.parametrize("leaf", [16, 64])
+def test_quadtree_threaded_build_matches_oracle(leaf):
+    """From 2^16 points on, the subtrees below the second level are built on
+    threads and stitched: same permutation and node count as the serial
+    oracle."""
+    rng = np.random.default_rng(leaf)
+    for pts in (curve_problem(1 << 17)[0], rng.uniform(-1, 1, ((1 << 16) + 3, 2))):
+        t = bf.Quadtree(pts, leaf)
+        perm, nodes, _ = orc.tree(pts, leaf)
+        assert np.array_equal(t.perm, perm)
+        assert t.num_nodes == len(nodes)
+
+
 def test_quadtree_rejects_bad_input():
     with pytest.raises(bf.BfError):
         bf.Quadtree(np.zeros((1, 2)), 1)
