@@ -203,3 +203,5 @@ extern BfMatVtable bfMatHipButterflyVtable;
 
 /* timing */
 double bfWallTime(void);
+/* prints "what" and the ms since `since` to stderr when BF_AMD_TRACE is set */
+void bfTrace(const char *what, double since);

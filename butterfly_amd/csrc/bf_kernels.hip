@@ -154,9 +154,7 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *_
  *   pts  2p + n   test, equiv, orig points (sampling only)
  *   red  8        block reductions
  *   piv  p int32  column pivots (p/4 units rounded up) */
-__host__ __device__ __forceinline__ size_t lstsq_units(int p, int n) {
-  return (size_t)p * (p + n + 1) + p + p + (2 * p + n) + 8 + (p + 3) / 4;
-}
+__host__ __device__ __forceinline__ size_t lstsq_units(int p, int n) { return BF_LSTSQ_UNITS(p, n); }
 
 /* Round-robin pairing: position 0 is fixed, the others rotate by `step`. */
 __device__ __forceinline__ int rr_row(int pos, int step, int P) {

@@ -19,6 +19,11 @@ extern "C" {
  * source-tree order.  X is written column-major at `out`.  Each job
  * carries its wavenumber k and pseudo-inverse cut rcond, so the jobs of
  * independent problems can share a launch. */
+/* 16-byte units of one re-expansion job's LDS (or scratch) workspace:
+ * M p x (p + n + 1), w p, v p, points 2p + n, reductions 8, pivots p/4 */
+#define BF_LSTSQ_UNITS(p, n) \
+  ((size_t)(p) * ((p) + (n) + 1) + (size_t)(p) + (size_t)(p) + (size_t)(2 * (p) + (n)) + 8 + (size_t)(((p) + 3) / 4))
+
 typedef struct BfkLstsqJob {
   double ecx, ecy, er;
   double tcx, tcy, tr;

@@ -5,6 +5,8 @@
 #define _DEFAULT_SOURCE 1
 #include <math.h>
 #include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <time.h>
 
 #include "bf_host.h"
@@ -13,6 +15,18 @@ double bfWallTime(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
   return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* BF_AMD_TRACE=1 in the environment: the host steps of a factorization
+ * print their wall times to stderr */
+static int g_trace = -1;
+
+void bfTrace(const char *what, double since) {
+  if (g_trace < 0) {
+    const char *e = getenv("BF_AMD_TRACE");
+    g_trace = e && e[0] && e[0] != '0';
+  }
+  if (g_trace) fprintf(stderr, "[bf] %-28s %9.2f ms\n", what, 1e3 * (bfWallTime() - since));
 }
 
 /* ------------------------------------------------------------ errors */

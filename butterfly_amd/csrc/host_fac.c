@@ -409,40 +409,101 @@ static void *apply_plan_task(void *arg) {
   return NULL;
 }
 
+/* Parallel stable counting sort of the re-expansion jobs by LDS class,
+ * largest class first, writing the jobs themselves (no order array): each
+ * of NT threads classes a contiguous range and counts it, the class starts
+ * are summed per thread, then each thread scatters its range.  Within a
+ * class the jobs keep plan order, as group_by_class(..., 1) with a gather
+ * would give. */
+enum { CLASS_NT = 16 };
+
 typedef struct {
   const BfkLstsqJob *src;
-  const int32_t *order;
   BfkLstsqJob *dst;
+  uint32_t *cls;
   size_t i0, i1;
-} GatherTask;
+  uint32_t maxCls;
+  size_t *pos; /* per class: the next slot of this thread (phase 3), counts (phase 2) */
+  int phase;
+} ClassTask;
 
-static void *gather_task(void *arg) {
-  GatherTask *g = arg;
-  for (size_t i = g->i0; i < g->i1; ++i) g->dst[i] = g->src[g->order[i]];
+static void *class_task(void *arg) {
+  ClassTask *t = arg;
+  if (t->phase == 1) {
+    uint32_t mx = 0;
+    for (size_t i = t->i0; i < t->i1; ++i) {
+      const uint32_t c = (uint32_t)((BF_LSTSQ_UNITS(t->src[i].p, t->src[i].n) * 16 + 4095) >> 12);
+      t->cls[i] = c;
+      if (c > mx) mx = c;
+    }
+    t->maxCls = mx;
+  } else if (t->phase == 2) {
+    for (size_t i = t->i0; i < t->i1; ++i) ++t->pos[t->cls[i]];
+  } else {
+    for (size_t i = t->i0; i < t->i1; ++i) t->dst[t->pos[t->cls[i]]++] = t->src[i];
+  }
   return NULL;
 }
 
-/* dst[i] = src[order[i]] on up to 16 threads */
-static void gather_jobs(const BfkLstsqJob *src, const int32_t *order, BfkLstsqJob *dst, size_t n) {
-  enum { NT = 16 };
-  GatherTask t[NT];
-  pthread_t th[NT];
-  int ok[NT] = {0};
-  const int nt = n < ((size_t)1 << 16) ? 1 : NT;
-  for (int q = 0; q < nt; ++q) {
-    t[q].src = src;
-    t[q].order = order;
-    t[q].dst = dst;
-    t[q].i0 = n * (size_t)q / (size_t)nt;
-    t[q].i1 = n * (size_t)(q + 1) / (size_t)nt;
-  }
+static void class_phase(ClassTask *t, int nt, int phase) {
+  pthread_t th[CLASS_NT];
+  int ok[CLASS_NT] = {0};
+  for (int q = 0; q < nt; ++q) t[q].phase = phase;
   for (int q = 1; q < nt; ++q) {
-    ok[q] = pthread_create(&th[q], NULL, gather_task, &t[q]) == 0;
-    if (!ok[q]) gather_task(&t[q]);
+    ok[q] = pthread_create(&th[q], NULL, class_task, &t[q]) == 0;
+    if (!ok[q]) class_task(&t[q]);
   }
-  gather_task(&t[0]);
+  class_task(&t[0]);
   for (int q = 1; q < nt; ++q)
     if (ok[q]) pthread_join(th[q], NULL);
+}
+
+/* dst gets src's jobs class by class (largest first); gs gets each
+ * non-empty class's first position plus a final nj, gb its byte size */
+static int class_sort_jobs(const BfkLstsqJob *src, size_t nj, BfkLstsqJob *dst, BfI64Vec *gs, BfSizeVec *gb) {
+  ClassTask t[CLASS_NT];
+  const int nt = nj < ((size_t)1 << 16) ? 1 : CLASS_NT;
+  uint32_t *cls = malloc((nj ? nj : 1) * sizeof(uint32_t));
+  if (!cls) return -2;
+  memset(t, 0, sizeof(t));
+  for (int q = 0; q < nt; ++q) {
+    t[q].src = src;
+    t[q].dst = dst;
+    t[q].cls = cls;
+    t[q].i0 = nj * (size_t)q / (size_t)nt;
+    t[q].i1 = nj * (size_t)(q + 1) / (size_t)nt;
+  }
+  class_phase(t, nt, 1);
+  uint32_t maxCls = 0;
+  for (int q = 0; q < nt; ++q)
+    if (t[q].maxCls > maxCls) maxCls = t[q].maxCls;
+  size_t *pos = calloc((size_t)nt * (maxCls + 1), sizeof(size_t));
+  if (!pos) {
+    free(cls);
+    return -2;
+  }
+  for (int q = 0; q < nt; ++q) t[q].pos = pos + (size_t)q * (maxCls + 1);
+  class_phase(t, nt, 2);
+  /* counts -> each thread's first slot per class, classes descending */
+  gs->n = gb->n = 0;
+  size_t run = 0;
+  for (int64_t c = maxCls; c >= 0; --c) {
+    const size_t first = run;
+    for (int q = 0; q < nt; ++q) {
+      const size_t cnt = t[q].pos[c];
+      t[q].pos[c] = run;
+      run += cnt;
+    }
+    if (run > first) {
+      BF_VEC_PUSH(*gs, (int64_t)first);
+      BF_VEC_PUSH(*gb, (size_t)c << 12);
+    }
+  }
+  BF_VEC_PUSH(*gs, (int64_t)nj);
+  class_phase(t, nt, 3);
+  free(pos);
+  free(cls);
+  return 0;
 }
 
 /* Sampling, then the compression: the re-expansion jobs are grouped by
@@ -453,40 +514,43 @@ static void gather_jobs(const BfkLstsqJob *src, const int32_t *order, BfkLstsqJo
 static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   int rc;
   *tApplyPlan = 0;
+  const double tk = bfWallTime();
   size_t arenaBytes = (size_t)F->arenaEntries * 16;
   F->dArena = bfk_malloc(arenaBytes);
   if (!F->dArena) return -2;
+  bfTrace("factor: arena allocated", tk);
   void *dK = NULL;
   if ((rc = upload(&dK, F->kernelJobs.a, F->kernelJobs.n * sizeof(BfkKernelJob)))) return rc;
   const BfkLayer layer = layer_of(&F->helm);
   rc = bfk_sample(dK, (int64_t)F->kernelJobs.n, F->dSrcXY, F->dTgtXY, F->dSrcN, layer, F->dArena);
   bfk_free(dK);
   if (rc) return rc;
+  bfTrace("factor: sampling launched", tk);
   const size_t nj = F->lstsqJobs.n;
-  uint32_t *cls = malloc((nj ? nj : 1) * sizeof(uint32_t));
-  for (size_t i = 0; i < nj; ++i)
-    cls[i] = (uint32_t)((bfk_lstsq_lds_bytes(F->lstsqJobs.a[i].p, F->lstsqJobs.a[i].n) + 4095) >> 12);
-  int32_t *order = malloc((nj ? nj : 1) * sizeof(int32_t));
+  /* the jobs in class order (largest first) */
   BfI64Vec gs;
   BfSizeVec gb;
   memset(&gs, 0, sizeof(gs));
   memset(&gb, 0, sizeof(gb));
-  group_by_class(cls, nj, order, &gs, &gb, 1);
-  free(cls);
+  BfkLstsqJob *sorted = malloc((nj ? nj : 1) * sizeof(BfkLstsqJob));
+  if (!sorted || class_sort_jobs(F->lstsqJobs.a, nj, sorted, &gs, &gb)) {
+    free(sorted);
+    BF_VEC_FREE(gs);
+    BF_VEC_FREE(gb);
+    return -2;
+  }
+  bfTrace("factor: jobs sorted by class", tk);
   /* the apply plan is host work: build it on a helper thread while this
    * thread uploads the jobs and the kernels run */
   ApplyPlanTask task = {F, 0, 0};
   pthread_t th;
   const int threaded = pthread_create(&th, NULL, apply_plan_task, &task) == 0;
   if (!threaded) apply_plan_task(&task);
-  /* the jobs in class order, uploaded in chunks on the copy stream; each
-   * uploaded range is launched at once, so the upload (several GB for
-   * large or batched problems) overlaps the kernels */
-  BfkLstsqJob *sorted = malloc((nj ? nj : 1) * sizeof(BfkLstsqJob));
-  gather_jobs(F->lstsqJobs.a, order, sorted, nj);
-  free(order);
+  /* uploaded in chunks on the copy stream; each uploaded range is launched
+   * at once, so the upload (several GB for large or batched problems)
+   * overlaps the kernels */
   BfkLstsqJob *dL = bfk_malloc((nj ? nj : 1) * sizeof(BfkLstsqJob));
-  if (!dL || !sorted) rc = -2;
+  if (!dL) rc = -2;
   /* jobs above the LDS capacity share one scratch buffer */
   size_t bigBytes = 0;
   int64_t numBig = 0;
@@ -526,6 +590,7 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
       if (launched == classEnd) g = gEnd;
     }
   }
+  bfTrace("factor: compression launched", tk);
   /* close the launch sequence even after a failed launch, so the side
    * streams are joined and the scratch is released */
   if (begun) {
@@ -534,6 +599,7 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   }
   free(sorted);
   if (threaded) pthread_join(th, NULL);
+  bfTrace("factor: apply plan joined", tk);
   *tApplyPlan = task.seconds;
   if (!rc && task.rc) rc = task.rc;
   /* the launches are still running: release their tables after the next sync */
@@ -552,6 +618,7 @@ static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, d
   /* the normal-dependent layers read the normals in every sampling and
    * compression job: no kernel runs without them */
   int rc = (F->helm.layerPot != BF_LAYER_POTENTIAL_SINGLE && !snrm) ? -2 : 0;
+  bfTrace("factor: coordinates (since plan)", t1);
   if (!rc) rc = upload((void **)&F->dSrcXY, sxy, 2 * F->numSrc * sizeof(double));
   if (!rc) rc = upload((void **)&F->dTgtXY, txy, 2 * F->numTgt * sizeof(double));
   if (!rc && snrm) rc = upload((void **)&F->dSrcN, snrm, 2 * F->numSrc * sizeof(double));
@@ -562,6 +629,7 @@ static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, d
   if (!rc) rc = run_factor_kernels(F, &tApplyPlan);
   if (!rc) rc = bfk_sync();
   double t2 = bfWallTime();
+  bfTrace("factor: kernels done (since plan)", t1);
   if (!rc) rc = upload((void **)&F->dSegs, F->segs.a, F->segs.n * sizeof(BfkSeg));
   if (!rc) rc = upload((void **)&F->dTerms, F->terms.a, F->terms.n * sizeof(BfkTerm));
   if (rc) {
@@ -570,6 +638,8 @@ static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, d
     return NULL;
   }
   double t3 = bfWallTime();
+  bfTrace("factor: apply tables up", t2);
+  bfTrace("factor: total", t0);
   double ktimes[8];
   bfk_get_times(ktimes);
   int64_t nL = 0, nK = 0, nI = 0, nD = 0, nB = 0;
@@ -646,6 +716,7 @@ BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const
     return NULL;
   }
   double t1 = bfWallTime();
+  bfTrace("factor: plan", t0);
   return factor_from_plan(F, tree_coords(srcTree), tree_coords(tgtTree),
                           helm->layerPot != BF_LAYER_POTENTIAL_SINGLE ? tree_normals(srcTree) : NULL, t0, t1);
 }
