@@ -300,6 +300,14 @@ static int upload(void **dst, const void *src, size_t bytes) {
   return 0;
 }
 
+/* the same on the copy stream: does not wait for the queued kernels */
+static int upload_side(void **dst, const void *src, size_t bytes) {
+  *dst = bfk_malloc(bytes);
+  if (!*dst) return -2;
+  if (bytes && bfk_h2d_side(*dst, src, bytes)) return -1;
+  return 0;
+}
+
 /* the apply of a factorization whose plan and arena are in place (a loaded
  * one): stage segments and terms, uploaded */
 int bfMatHipButterflyFinish(BfMatHipButterfly *F) {
@@ -509,8 +517,8 @@ static int class_sort_jobs(const BfkLstsqJob *src, size_t nj, BfkLstsqJob *dst, 
 /* Sampling, then the compression: the re-expansion jobs are grouped by
  * the LDS they need (4 KiB classes, launched largest first over four
  * streams) and uploaded in chunks that are launched as they arrive; the
- * apply plan is built on a helper thread meanwhile.  Returns with the
- * kernels running (bfk_sync waits). */
+ * apply plan is built on a helper thread meanwhile and its tables go up
+ * on the copy stream.  Returns with the kernels running (bfk_sync waits). */
 static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   int rc;
   *tApplyPlan = 0;
@@ -602,6 +610,10 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   bfTrace("factor: apply plan joined", tk);
   *tApplyPlan = task.seconds;
   if (!rc && task.rc) rc = task.rc;
+  /* the apply tables go up while the compression runs */
+  if (!rc) rc = upload_side((void **)&F->dSegs, F->segs.a, F->segs.n * sizeof(BfkSeg));
+  if (!rc) rc = upload_side((void **)&F->dTerms, F->terms.a, F->terms.n * sizeof(BfkTerm));
+  bfTrace("factor: apply tables up", tk);
   /* the launches are still running: release their tables after the next sync */
   bfk_free_async(dL);
   BF_VEC_FREE(gs);
@@ -610,8 +622,8 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
 }
 
 /* Everything after planning: coordinates up, sampling and compression
- * launched, the apply plan built on the host meanwhile, then the apply
- * tables up and the statistics.  Takes ownership of sxy / txy (tree-ordered
+ * launched, the apply plan built on the host and its tables uploaded
+ * meanwhile, then the statistics.  Takes ownership of sxy / txy (tree-ordered
  * coordinates) and snrm (tree-ordered source normals or NULL); on failure
  * deletes F and returns NULL. */
 static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, double *snrm, double t0, double t1) {
@@ -630,15 +642,12 @@ static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, d
   if (!rc) rc = bfk_sync();
   double t2 = bfWallTime();
   bfTrace("factor: kernels done (since plan)", t1);
-  if (!rc) rc = upload((void **)&F->dSegs, F->segs.a, F->segs.n * sizeof(BfkSeg));
-  if (!rc) rc = upload((void **)&F->dTerms, F->terms.a, F->terms.n * sizeof(BfkTerm));
   if (rc) {
     bfMatHipButterflyDelete((BfMat **)&F);
     bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : BF_ERROR_RUNTIME_ERROR);
     return NULL;
   }
   double t3 = bfWallTime();
-  bfTrace("factor: apply tables up", t2);
   bfTrace("factor: total", t0);
   double ktimes[8];
   bfk_get_times(ktimes);

@@ -687,7 +687,9 @@ int bfPlanBuildAt(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const 
   pool.F = F;
   pool.srcPts = srcPts;
   pool.tgtPts = tgtPts;
+  const double tp = bfWallTime();
   run_pool(&pool, 0, nthreads);
+  bfTrace("plan: chunks planned", tp);
   int64_t nl = (int64_t)F->leaves.n, nf = (int64_t)F->factors.n, nb = (int64_t)F->blocks.n;
   int64_t no = (int64_t)F->offs.n, njl = (int64_t)F->lstsqJobs.n, njk = (int64_t)F->kernelJobs.n;
   int64_t na = F->arenaEntries;
@@ -716,6 +718,7 @@ int bfPlanBuildAt(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const 
   RESERVE(F->kernelJobs, njk);
   F->arenaEntries = na;
   run_pool(&pool, 1, nthreads);
+  bfTrace("plan: chunks merged", tp);
   free(chunks);
   BF_VEC_FREE(items);
   tree_view_free(&P.src);
