@@ -203,5 +203,7 @@ extern BfMatVtable bfMatHipButterflyVtable;
 
 /* timing */
 double bfWallTime(void);
+/* madvise(MADV_HUGEPAGE) over [p, p + bytes) for tables of 32 MiB and up */
+void bfHugePages(void *p, size_t bytes);
 /* prints "what" and the ms since `since` to stderr when BF_AMD_TRACE is set */
 void bfTrace(const char *what, double since);

@@ -473,6 +473,7 @@ static int class_sort_jobs(const BfkLstsqJob *src, size_t nj, BfkLstsqJob *dst, 
   const int nt = nj < ((size_t)1 << 16) ? 1 : CLASS_NT;
   uint32_t *cls = malloc((nj ? nj : 1) * sizeof(uint32_t));
   if (!cls) return -2;
+  bfHugePages(cls, nj * sizeof(uint32_t));
   memset(t, 0, sizeof(t));
   for (int q = 0; q < nt; ++q) {
     t[q].src = src;
@@ -541,6 +542,7 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   memset(&gs, 0, sizeof(gs));
   memset(&gb, 0, sizeof(gb));
   BfkLstsqJob *sorted = malloc((nj ? nj : 1) * sizeof(BfkLstsqJob));
+  bfHugePages(sorted, nj * sizeof(BfkLstsqJob));
   if (!sorted || class_sort_jobs(F->lstsqJobs.a, nj, sorted, &gs, &gb)) {
     free(sorted);
     BF_VEC_FREE(gs);

@@ -534,40 +534,47 @@ static void build_chunk(PlanPool *pool, PlanChunk *c) {
  * target point bases; then frees T's tables */
 static void copy_tables(BfMatHipButterfly *F, BfMatHipButterfly *T, const PlanChunk *c, int64_t srcPts,
                         int64_t tgtPts) {
-  COPY_TABLE(F->leaves, T->leaves, c->leafBase);
+  /* one pass per table: each entry is copied and shifted on the way (the
+   * job and block tables run to gigabytes at 2^20 points) */
+  BfHipLeaf *L = F->leaves.a + c->leafBase;
   for (size_t i = 0; i < T->leaves.n; ++i) {
-    BfHipLeaf *l = &F->leaves.a[c->leafBase + (int64_t)i];
-    l->firstFactor += c->factorBase;
-    l->row0 += tgtPts;
-    l->col0 += srcPts;
-    if (l->denseOff >= 0) l->denseOff += c->arenaBase;
+    BfHipLeaf l = T->leaves.a[i];
+    l.firstFactor += c->factorBase;
+    l.row0 += tgtPts;
+    l.col0 += srcPts;
+    if (l.denseOff >= 0) l.denseOff += c->arenaBase;
+    L[i] = l;
   }
-  COPY_TABLE(F->factors, T->factors, c->factorBase);
+  BfHipFactor *Fa = F->factors.a + c->factorBase;
   for (size_t i = 0; i < T->factors.n; ++i) {
-    BfHipFactor *f = &F->factors.a[c->factorBase + (int64_t)i];
-    f->firstBlock += c->blockBase;
-    f->rowOffStart += c->offsBase;
-    f->colOffStart += c->offsBase;
+    BfHipFactor f = T->factors.a[i];
+    f.firstBlock += c->blockBase;
+    f.rowOffStart += c->offsBase;
+    f.colOffStart += c->offsBase;
+    Fa[i] = f;
   }
-  COPY_TABLE(F->blocks, T->blocks, c->blockBase);
+  BfHipBlock *B = F->blocks.a + c->blockBase;
   for (size_t i = 0; i < T->blocks.n; ++i) {
-    BfHipBlock *b = &F->blocks.a[c->blockBase + (int64_t)i];
-    if (b->dataOff >= 0) b->dataOff += c->arenaBase;
-    if (b->job >= 0) b->job += (int32_t)(b->kind == BF_BLOCK_LSTSQ ? c->lBase : c->kBase);
+    BfHipBlock b = T->blocks.a[i];
+    if (b.dataOff >= 0) b.dataOff += c->arenaBase;
+    if (b.job >= 0) b.job += (int32_t)(b.kind == BF_BLOCK_LSTSQ ? c->lBase : c->kBase);
+    B[i] = b;
   }
   COPY_TABLE(F->offs, T->offs, c->offsBase);
-  COPY_TABLE(F->lstsqJobs, T->lstsqJobs, c->lBase);
+  BfkLstsqJob *JL = F->lstsqJobs.a + c->lBase;
   for (size_t i = 0; i < T->lstsqJobs.n; ++i) {
-    BfkLstsqJob *J = &F->lstsqJobs.a[c->lBase + (int64_t)i];
-    J->out += c->arenaBase;
-    if (J->origOff >= 0) J->origOff += srcPts;
+    BfkLstsqJob J = T->lstsqJobs.a[i];
+    J.out += c->arenaBase;
+    if (J.origOff >= 0) J.origOff += srcPts;
+    JL[i] = J;
   }
-  COPY_TABLE(F->kernelJobs, T->kernelJobs, c->kBase);
+  BfkKernelJob *JK = F->kernelJobs.a + c->kBase;
   for (size_t i = 0; i < T->kernelJobs.n; ++i) {
-    BfkKernelJob *J = &F->kernelJobs.a[c->kBase + (int64_t)i];
-    J->out += c->arenaBase;
-    if (J->srcOff >= 0) J->srcOff += srcPts;
-    J->tgtOff += tgtPts;
+    BfkKernelJob J = T->kernelJobs.a[i];
+    J.out += c->arenaBase;
+    if (J.srcOff >= 0) J.srcOff += srcPts;
+    J.tgtOff += tgtPts;
+    JK[i] = J;
   }
   BF_VEC_FREE(T->leaves);
   BF_VEC_FREE(T->factors);
@@ -611,6 +618,7 @@ static void run_pool(PlanPool *pool, int phase, int nthreads) {
       size_t nc_ = 2 * (v).cap > (size_t)(total) ? 2 * (v).cap : (size_t)(total); \
       void *na_ = realloc((v).a, (nc_ ? nc_ : 1) * sizeof(*(v).a));    \
       if (!na_) abort();                                               \
+      bfHugePages(na_, nc_ * sizeof(*(v).a));                          \
       (v).a = na_;                                                     \
       (v).cap = nc_;                                                   \
     }                                                                  \
