@@ -164,6 +164,7 @@ typedef struct BfMatHipButterfly {
   size_t bufBytes;
 
   double stats[16];
+  int statsCounted;    /* stats 4-8 and 10 are filled in */
   int orthogonal;      /* the factors have been orthogonalized */
   double orthoSeconds; /* wall time of the orthogonalization */
 } BfMatHipButterfly;

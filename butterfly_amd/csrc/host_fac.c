@@ -333,6 +333,7 @@ int bfMatHipButterflyFinish(BfMatHipButterfly *F) {
   F->stats[8] = (double)nB;
   F->stats[9] = (double)F->numStages;
   F->stats[15] = bfWallTime() - t0;
+  F->statsCounted = 1;
   return rc;
 }
 
@@ -653,32 +654,15 @@ static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, d
   bfTrace("factor: total", t0);
   double ktimes[8];
   bfk_get_times(ktimes);
-  int64_t nL = 0, nK = 0, nI = 0, nD = 0, nB = 0;
-  double h0 = 0;
-  for (size_t b = 0; b < F->blocks.n; ++b) {
-    int kd = F->blocks.a[b].kind;
-    nL += kd == BF_BLOCK_LSTSQ;
-    nK += kd == BF_BLOCK_KERNEL;
-    nI += kd == BF_BLOCK_IDENTITY;
-  }
-  for (size_t i = 0; i < F->lstsqJobs.n; ++i)
-    h0 += (double)F->lstsqJobs.a[i].p * (F->lstsqJobs.a[i].p + F->lstsqJobs.a[i].n);
-  for (size_t i = 0; i < F->kernelJobs.n; ++i) h0 += (double)F->kernelJobs.a[i].m * F->kernelJobs.a[i].n;
-  for (size_t l = 0; l < F->leaves.n; ++l) {
-    nB += F->leaves.a[l].isButterfly;
-    nD += !F->leaves.a[l].isButterfly;
-  }
+  /* the block and evaluation counts (stats 4-8, 10) are taken when the
+   * statistics are first read: passes over tens of millions of entries
+   * that the factorization itself does not need */
+  F->statsCounted = 0;
   F->stats[0] = t3 - t0;
   F->stats[1] = t1 - t0;
   F->stats[2] = t2 - t1;
   F->stats[3] = (double)F->arenaEntries;
-  F->stats[4] = (double)nL;
-  F->stats[5] = (double)nK;
-  F->stats[6] = (double)nI;
-  F->stats[7] = (double)nD;
-  F->stats[8] = (double)nB;
   F->stats[9] = (double)F->numStages;
-  F->stats[10] = h0;
   F->stats[13] = ktimes[0];
   F->stats[14] = ktimes[1];
   F->stats[15] = tApplyPlan + (t3 - t2);
@@ -1171,8 +1155,35 @@ BfSize bfhipFacLeafNumFactors(BfMat const *fac, BfSize leaf) {
   return F->leaves.a[leaf].isButterfly ? (BfSize)F->leaves.a[leaf].numFactors : 0;
 }
 
+/* stats 4-8 (block kinds, leaf kinds) and 10 (kernel evaluations) */
+static void count_stats(BfMatHipButterfly *F) {
+  int64_t nL = 0, nK = 0, nI = 0, nD = 0, nB = 0;
+  double h0 = 0;
+  for (size_t b = 0; b < F->blocks.n; ++b) {
+    const int kd = F->blocks.a[b].kind;
+    nL += kd == BF_BLOCK_LSTSQ;
+    nK += kd == BF_BLOCK_KERNEL;
+    nI += kd == BF_BLOCK_IDENTITY;
+  }
+  for (size_t i = 0; i < F->lstsqJobs.n; ++i)
+    h0 += (double)F->lstsqJobs.a[i].p * (F->lstsqJobs.a[i].p + F->lstsqJobs.a[i].n);
+  for (size_t i = 0; i < F->kernelJobs.n; ++i) h0 += (double)F->kernelJobs.a[i].m * F->kernelJobs.a[i].n;
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    nB += F->leaves.a[l].isButterfly;
+    nD += !F->leaves.a[l].isButterfly;
+  }
+  F->stats[4] = (double)nL;
+  F->stats[5] = (double)nK;
+  F->stats[6] = (double)nI;
+  F->stats[7] = (double)nD;
+  F->stats[8] = (double)nB;
+  F->stats[10] = h0;
+  F->statsCounted = 1;
+}
+
 void bfhipFacGetStats(BfMat const *fac, double out[16]) {
-  BfMatHipButterfly const *F = (BfMatHipButterfly const *)fac;
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac; /* the counts are cached on first read */
+  if (!F->statsCounted) count_stats(F);
   memcpy(out, F->stats, 16 * sizeof(double));
 }
 
