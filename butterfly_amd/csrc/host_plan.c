@@ -611,14 +611,16 @@ static void run_pool(PlanPool *pool, int phase, int nthreads) {
 }
 
 /* size a table to `total` entries, growing its capacity geometrically so
- * that repeated appends copy each entry a bounded number of times */
+ * that repeated appends copy each entry a bounded number of times; a first
+ * allocation of a large table gets huge-page advice (not a regrown one:
+ * appended problems of a batch would then fault into compaction stalls) */
 #define RESERVE(v, total)                                              \
   do {                                                                 \
     if ((v).cap < (size_t)(total)) {                                   \
       size_t nc_ = 2 * (v).cap > (size_t)(total) ? 2 * (v).cap : (size_t)(total); \
       void *na_ = realloc((v).a, (nc_ ? nc_ : 1) * sizeof(*(v).a));    \
       if (!na_) abort();                                               \
-      bfHugePages(na_, nc_ * sizeof(*(v).a));                          \
+      if (!(v).cap) bfHugePages(na_, nc_ * sizeof(*(v).a));            \
       (v).a = na_;                                                     \
       (v).cap = nc_;                                                   \
     }                                                                  \
