@@ -1248,15 +1248,17 @@ __device__ __forceinline__ double house_apply_wave(const double2 *__restrict__ x
  *                  (jacobi_rows, as for the re-expansion blocks): G R = R'
  *                  with orthogonal rows sigma_i v_i^H, G accumulated
  *   k_tall_w       sigma_i = |row i of R'|, V = P [conj(R') / sigma]^T, and
- *                  W = Q [sigma_i conj(G[i, :]); 0] (reflectors applied last
- *                  first), so A V = W and W's columns keep their mutual
- *                  orthogonality relative to their own norms
+ *                  W = Q [sigma_i conj(G[i, :]); 0] with Q in compact WY
+ *                  form (one pass over the reflectors), so A V = W and W's
+ *                  columns keep their mutual orthogonality relative to
+ *                  their own norms
  * The pivoted QR grades R's rows, so the row Jacobi needs few sweeps; the
  * rotations touch n-row operands instead of m. */
 #define SVD_TALL_MAX_N 70
 #define SVD_TALL_T 1024 /* threads of the QR and W launches */
 __host__ __device__ __forceinline__ int tall_ld(int n) { return 2 * n + 1; }
 __host__ __device__ __forceinline__ size_t svd_tall_lds(int n) { return (size_t)n * tall_ld(n) * 16; }
+__host__ __device__ __forceinline__ size_t svd_tall_w_lds(int n) { return (size_t)2 * n * n * 16; }
 /* scratch slot of one matrix: [R | I] then [R' | G], n x (2n + 1) */
 #define TALL_SLOT ((int64_t)SVD_TALL_MAX_N * (2 * SVD_TALL_MAX_N + 1))
 
@@ -1385,6 +1387,7 @@ __global__ __launch_bounds__(SVD_TALL_T) void k_tall_w(const int32_t *__restrict
                                                        const int64_t *__restrict__ vOff, double2 *__restrict__ Vout,
                                                        const int64_t *__restrict__ sOff, double *__restrict__ sig) {
   __shared__ double sgm[SVD_TALL_MAX_N];
+  extern __shared__ double2 lds[]; /* 2 n^2: T^{-1} and Z */
   const int t = blockIdx.x, b = idx[t];
   const int m = mm[b], n = nn[b], ld = tall_ld(n);
   double2 *a = A + aOff[b];
@@ -1411,24 +1414,116 @@ __global__ __launch_bounds__(SVD_TALL_T) void k_tall_w(const int32_t *__restrict
     const double2 r = rg[i * ld + j];
     vo[(int64_t)i * n + pv[j]] = make_double2(r.x * inv, -r.y * inv);
   }
-  /* W = Q [sigma_i conj(G[i, :]); 0] */
-  for (int64_t e = tid; e < (int64_t)m * n; e += SVD_TALL_T) {
-    const int c = (int)(e / m), r = (int)(e - (int64_t)c * m);
-    double2 v = make_double2(0, 0);
-    if (r < n) {
-      const double2 g = rg[c * ld + n + r];
-      v = make_double2(sgm[c] * g.x, -sgm[c] * g.y);
+  /* W = Q X, X = [sigma_c conj(G[c, :])^T; 0] (m x n, zero below row n),
+   * Q = H_0 ... H_{n-1}, H_j = I - tau_j v_j v_j^H, v_j = column j of S on
+   * rows >= j, tau_j = hinv[j] (0: no reflector).  Compact WY form:
+   * Q = I - Y T Y^H with T^{-1} = diag(1/tau) + the strict upper part of
+   * Y^H Y, so W = X - Y Z' with T^{-1} Z' = Y^H X.  Y^H X only needs the
+   * first n rows of Y; each column of W goes through the same linear steps
+   * on its own, so it keeps its accuracy relative to its own norm. */
+  double2 *Ti = lds, *Z = lds + n * n; /* row-major n x n each */
+#define TALL_X(r, c) make_double2(sgm[c] * rg[(c) * ld + n + (r)].x, -sgm[c] * rg[(c) * ld + n + (r)].y)
+  for (int e = tid; e < n * n; e += SVD_TALL_T) { /* Z = Y^H X */
+    const int j = e / n, c = e - j * n;
+    double zx = 0, zy = 0;
+    if (hinv[j] != 0)
+      for (int r = j; r < n; ++r) {
+        const double2 v = s[(int64_t)j * m + r], x = TALL_X(r, c);
+        zx = fma(v.x, x.x, fma(v.y, x.y, zx)); /* conj(v) x */
+        zy = fma(v.x, x.y, fma(-v.y, x.x, zy));
+      }
+    Z[e] = make_double2(zx, zy);
+    const int i = j, jj = c; /* diagonal and lower part of T^{-1} */
+    if (i >= jj) Ti[e] = make_double2(i == jj ? (hinv[i] != 0 ? 1.0 / hinv[i] : 1.0) : 0.0, 0.0);
+  }
+  { /* strict upper part of Y^H Y: 4 x 4 tiles of (i, j), one wave each */
+    const int nb4 = (n + 3) / 4, ntiles = nb4 * (nb4 + 1) / 2;
+    for (int tl = w; tl < ntiles; tl += NW) {
+      int ib = 0, rem = tl; /* tile tl -> (ib, jb), ib <= jb, row by row */
+      while (rem >= nb4 - ib) {
+        rem -= nb4 - ib;
+        ++ib;
+      }
+      const int jb = ib + rem, i0 = 4 * ib, j0 = 4 * jb;
+      double ax[4][4], ay[4][4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ax[p][q] = ay[p][q] = 0;
+      for (int r = j0 + lane; r < m; r += 64) {
+        double2 vi[4], vj[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int i = i0 + p, j = j0 + p;
+          vi[p] = (i < n && r >= i) ? s[(int64_t)i * m + r] : make_double2(0, 0);
+          vj[p] = (j < n && r >= j) ? s[(int64_t)j * m + r] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            ax[p][q] = fma(vi[p].x, vj[q].x, fma(vi[p].y, vj[q].y, ax[p][q]));
+            ay[p][q] = fma(vi[p].x, vj[q].y, fma(-vi[p].y, vj[q].x, ay[p][q]));
+          }
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double gx = group_sum(ax[p][q], 64), gy = group_sum(ay[p][q], 64);
+          const int i = i0 + p, j = j0 + q;
+          if (lane == 0 && i < j && j < n)
+            Ti[i * n + j] = (hinv[i] != 0 && hinv[j] != 0) ? make_double2(gx, gy) : make_double2(0, 0);
+        }
     }
-    a[e] = v;
   }
   __syncthreads();
-  for (int j = n - 1; j >= 0; --j) {
-    const double inv = hinv[j];
-    if (inv == 0) continue; /* uniform */
-    const double2 *x = s + (int64_t)j * m;
-    for (int c = w; c < n; c += NW) house_apply_wave(x, a + (int64_t)c * m, j, m, inv, lane);
-    __syncthreads();
+  /* T^{-1} Z' = Z, upper triangular with a real diagonal: one thread per
+   * column of Z, in place */
+  for (int c = tid; c < n; c += SVD_TALL_T)
+    for (int j = n - 1; j >= 0; --j) {
+      double zx = Z[j * n + c].x, zy = Z[j * n + c].y;
+      for (int k = j + 1; k < n; ++k) {
+        const double2 tk = Ti[j * n + k], zk = Z[k * n + c];
+        zx -= tk.x * zk.x - tk.y * zk.y;
+        zy -= tk.x * zk.y + tk.y * zk.x;
+      }
+      const double d = 1.0 / Ti[j * n + j].x;
+      Z[j * n + c] = make_double2(zx * d, zy * d);
+    }
+  __syncthreads();
+  /* W = X - Y Z': a thread per (row, chunk of TALL_CH columns); the lanes
+   * of a wave share the column chunk, so the Z' reads are broadcasts */
+  constexpr int TALL_CH = 14;
+  const int nch = (n + TALL_CH - 1) / TALL_CH;
+  for (int64_t e = tid; e < (int64_t)m * nch; e += SVD_TALL_T) {
+    const int ch = (int)(e / m), r = (int)(e - (int64_t)ch * m), c0 = ch * TALL_CH;
+    double accx[TALL_CH], accy[TALL_CH];
+#pragma unroll
+    for (int q = 0; q < TALL_CH; ++q) accx[q] = accy[q] = 0;
+    const int jmax = r < n - 1 ? r : n - 1;
+    for (int j = 0; j <= jmax; ++j) {
+      if (hinv[j] == 0) continue;
+      const double2 v = s[(int64_t)j * m + r];
+      const double2 *zr = Z + j * n + c0;
+#pragma unroll
+      for (int q = 0; q < TALL_CH; ++q)
+        if (c0 + q < n) {
+          const double2 z = zr[q];
+          accx[q] = fma(v.x, z.x, fma(-v.y, z.y, accx[q]));
+          accy[q] = fma(v.x, z.y, fma(v.y, z.x, accy[q]));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TALL_CH; ++q) {
+      const int c = c0 + q;
+      if (c < n) {
+        const double2 x = r < n ? TALL_X(r, c) : make_double2(0, 0);
+        a[(int64_t)c * m + r] = make_double2(x.x - accx[q], x.y - accy[q]);
+      }
+    }
   }
+#undef TALL_X
 }
 
 /* ---------------------------------------------------- orthogonalization */
@@ -2044,9 +2139,13 @@ int bfk_init(void) {
   }
   /* the tall SVD kernel asks only for what its largest case needs (it has
    * a few static LDS words besides) */
+  /* k_tall_w also holds sigma in static LDS (SVD_TALL_MAX_N doubles) */
   g_tallOk = g_ldsCap >= svd_tall_lds(SVD_TALL_MAX_N) &&
+             g_ldsCap >= svd_tall_w_lds(SVD_TALL_MAX_N) + SVD_TALL_MAX_N * 8 &&
              hipFuncSetAttribute((const void *)k_tall_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)svd_tall_lds(SVD_TALL_MAX_N)) == hipSuccess;
+                                 (int)svd_tall_lds(SVD_TALL_MAX_N)) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_tall_w, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)svd_tall_w_lds(SVD_TALL_MAX_N)) == hipSuccess;
   if (!g_tallOk) (void)hipGetLastError();
   return 0;
 }
@@ -2495,7 +2594,8 @@ int bfk_svd_jacobi_tall(int64_t nb, const int32_t *dIdx, const int64_t *dAOff, c
     if (hipGetLastError() != hipSuccess) rc = -1;
   }
   if (!rc) {
-    hipLaunchKernelGGL(k_tall_w, dim3((unsigned)nb), dim3(SVD_TALL_T), 0, g_stream, dIdx, dAOff, dM, dN,
+    hipLaunchKernelGGL(k_tall_w, dim3((unsigned)nb), dim3(SVD_TALL_T), svd_tall_w_lds(maxN), g_stream, dIdx, dAOff,
+                       dM, dN,
                        (double2 *)dA, (const double2 *)dS, (const double2 *)rg, piv, hinv, dVOff, (double2 *)dV, dSOff,
                        dSig);
     if (hipGetLastError() != hipSuccess) rc = -1;
