@@ -50,6 +50,29 @@ def test_svd_batched_matches_numpy(gpu_lib, m, n):
             assert err <= (s[k] if k < len(s) else 0) * 1.0001 + 1e-13 * max(1, n / 64) * s[0]
 
 
+@pytest.mark.parametrize("m,n", [(900, 40), (40, 900)])
+def test_svd_batched_rank_deficient_with_zero_columns(gpu_lib, m, n):
+    """Tall path (as is, or on the conjugate transpose) with exact rank 10
+    and five exactly zero columns (rows): the pivoted QR ends with steps that
+    have no reflector, which the compact WY step of the tall SVD skips."""
+    bf = gpu_lib
+    rng = np.random.default_rng(m + n)
+    r = 10
+    A = (rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r))) @ \
+        (rng.standard_normal((r, n)) + 1j * rng.standard_normal((r, n)))
+    zero = rng.choice(min(m, n), 5, replace=False)
+    if m >= n:
+        A[:, zero] = 0
+    else:
+        A[zero, :] = 0
+    (U, W, k), = bf.svd_batched([A], 1e-8)
+    s = np.linalg.svd(A, compute_uv=False)
+    assert k == int(np.sum(s >= 1e-8 * s[0])) == r
+    assert np.abs(U.conj().T @ U - np.eye(k)).max() < 1e-12
+    assert np.allclose(np.linalg.norm(W, axis=1), s[:k], rtol=1e-11)
+    assert np.linalg.norm(U @ W - A, 2) <= 1e-13 * s[0]
+
+
 def _nodes(tree):
     import butterfly_amd as bf
     qs = bf.QuadtreeStruct.from_address(tree.ptr)
