@@ -155,6 +155,56 @@ def test_plan_config0_census():
     assert (lsq[:, 2] == 16).all()
 
 
+def _census(lt, ft, bt):
+    """(first, middle, last) block counts, stored entries and the ranks of
+    the re-expansion blocks of a plan export."""
+    f = b = 0
+    first = mid = last = stored = 0
+    ps = []
+    for leaf in lt:
+        if leaf[0] == 0:
+            stored += int(leaf[3] * leaf[4])
+            continue
+        nf = int(leaf[5])
+        for i in range(nf):
+            nb = int(ft[f, 3])
+            blk = bt[b:b + nb]
+            if i == 0:
+                first += nb
+            elif i == nf - 1:
+                last += nb
+            else:
+                mid += nb
+            stored += int((blk[:, 2] * blk[:, 3])[blk[:, 4] != 2].sum())
+            ps += list(blk[blk[:, 4] == 0][:, 2])
+            b += nb
+            f += 1
+    return first, mid, last, stored, np.array(ps)
+
+
+def test_plan_curve_census_reference_rules():
+    """Curve -> circle at N = 2^16, k for 10 points per wavelength, under the
+    reference rules (leaf 1, rank estimates at 1e-15, dense below 128^2,
+    bfFacHelm2Prepare's level choice, fac_helm2.c:551-651): 188 butterflies
+    and 8 dense blocks; 1170 first, 7816 middle and 1374 last blocks;
+    1.777e9 stored entries; 8986 re-expansion blocks with p from 31 (median
+    100) to 6252.  These counts come from an independent restatement of the
+    planning rules; a plan one level off changes every one of them.  Both
+    the oracle and the product's planner must give them."""
+    src, tgt, k = curve_problem(1 << 16)
+    O = orc.Fac(k, src, tgt, plan_only=True)
+    ts, tt = bf.Quadtree(src, 1), bf.Quadtree(tgt, 1)
+    F = bf.FacHelm2.plan_only(bf.Helm2(k), ts, tt)
+    for lt, ft, bt in (O.plan(), F.plan()):
+        assert len(lt) == 196 and int(lt[:, 0].sum()) == 188
+        first, mid, last, stored, ps = _census(lt, ft, bt)
+        assert (first, mid, last) == (1170, 7816, 1374)
+        assert round(stored / 1e6) == 1777
+        assert len(ps) == 8986 and ps.min() == 31 and int(np.median(ps)) == 100 and ps.max() == 6252
+    for a, b in zip(O.plan(), F.plan()):
+        assert np.array_equal(a, b)
+
+
 def test_plan_covers_every_entry_once():
     # the top-level leaves tile the N_tgt x N_src index square exactly
     src, tgt, k = curve_problem(8192)

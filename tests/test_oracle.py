@@ -47,6 +47,66 @@ def test_uniform_and_randn_statistics():
     assert abs(np.mean(np.abs(z) ** 2) - 2) < 0.05
 
 
+def _xoshiro_uniforms(seed, count):
+    """(x >> 11) 2^-53 of xoshiro256+ seeded through splitmix64, restated in
+    Python from the published generators (rand.c:19-46)."""
+    M = (1 << 64) - 1
+    z = seed & M
+    s = []
+    for _ in range(4):
+        z = (z + 0x9E3779B97F4A7C15) & M
+        w = z
+        w = ((w ^ (w >> 30)) * 0xBF58476D1CE4E5B9) & M
+        w = ((w ^ (w >> 27)) * 0x94D049BB133111EB) & M
+        s.append(w ^ (w >> 31))
+    out = []
+    for _ in range(count):
+        out.append(((s[0] + s[3]) & M) >> 11)
+        t = (s[1] << 17) & M
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t
+        s[3] = ((s[3] << 45) | (s[3] >> 19)) & M
+    return [u * 2.0 ** -53 for u in out]
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 10])
+def test_box_muller_pairing_is_the_textbook_transform(n):
+    """bfRealRandn (rand.c:52-72): consecutive uniforms (u0, u1) become
+    (sqrt(-2 log u0) cos 2 pi u1, sqrt(-2 log u0) sin 2 pi u1); an odd
+    length draws two more uniforms for the cosine deviate alone.  Swapping
+    cos and sin, or the roles of u0 and u1, fails this bit for bit."""
+    import math
+    u = _xoshiro_uniforms(3, n + 2)
+    want = []
+    for i in range(n // 2):
+        rad = math.sqrt(-2 * math.log(u[2 * i]))
+        want += [rad * math.cos(2 * math.pi * u[2 * i + 1]), rad * math.sin(2 * math.pi * u[2 * i + 1])]
+    if n % 2:
+        j = 2 * (n // 2)
+        want.append(math.sqrt(-2 * math.log(u[j])) * math.cos(2 * math.pi * u[j + 1]))
+    orc.seed(3)
+    got = np.zeros(n)
+    orc.lib().orc_randn_fill(n, orc._dp(got))
+    assert got.tolist() == want
+    # bfComplexRandn is the real stream of length 2n read as (re, im) pairs
+    orc.seed(3)
+    z = orc.complex_randn(n // 2 + 1)
+    orc.seed(3)
+    r = np.zeros(2 * (n // 2 + 1))
+    orc.lib().orc_randn_fill(len(r), orc._dp(r))
+    assert np.array_equal(z.view(np.float64), r)
+
+
+def test_box_sampling_draws_x_then_y():
+    """bfPoint2SampleUniformlyFromBoundingBox (points.c:25-28): point i takes
+    uniform 2i for x and 2i + 1 for y, each scaled to its side."""
+    u = _xoshiro_uniforms(11, 8)
+    orc.seed(11)
+    P = orc.sample_box(4, 2.0, 3.0, -1.0, 0.5)
+    for i in range(4):
+        assert P[i, 0] == (3.0 - 2.0) * u[2 * i] + 2.0
+        assert P[i, 1] == (0.5 - -1.0) * u[2 * i + 1] + -1.0
+
+
 def test_h0_matches_mpmath_table():
     tab = json.load(open(os.path.join(HERE, "golden", "h0_table.json")))["rows"]
     x = np.array([float(r[0]) for r in tab])
