@@ -73,6 +73,41 @@ class FacOptionsStruct(C.Structure):
                 ("maxDenseSize", C.c_size_t), ("passThrough", C.c_bool), ("pinvRcond", C.c_double)]
 
 
+class PtrArrayStruct(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_void_p)), ("capacity", C.c_size_t), ("num_elts", C.c_size_t),
+                ("isView", C.c_bool)]
+
+
+class TreeLevelIterStruct(C.Structure):
+    """tree_level_iter.h:7-12"""
+    _fields_ = [("traversal", C.c_int), ("nodes", PtrArrayStruct), ("levelNodes", C.POINTER(PtrArrayStruct)),
+                ("aux", C.c_void_p)]
+
+
+class MatStruct(C.Structure):
+    """mat.h:187-196"""
+    _fields_ = [("vtbl", C.c_void_p), ("props", C.c_int), ("numRows", C.c_size_t), ("numCols", C.c_size_t)]
+
+
+class MatBlockStruct(C.Structure):
+    """mat_block.h:39-66"""
+    _fields_ = [("super", MatStruct), ("vtbl", C.c_void_p), ("block", C.POINTER(C.c_void_p)),
+                ("rowOffset", C.POINTER(C.c_size_t)), ("colOffset", C.POINTER(C.c_size_t))]
+
+
+class MatBlockCooStruct(C.Structure):
+    """mat_block_coo.h:34-46"""
+    _fields_ = [("super", MatBlockStruct), ("numBlocks", C.c_size_t), ("rowInd", C.POINTER(C.c_size_t)),
+                ("colInd", C.POINTER(C.c_size_t)), ("numBlockRows", C.c_size_t), ("numBlockCols", C.c_size_t),
+                ("ownsBlock", C.c_void_p)]
+
+
+class MatBlockDiagStruct(C.Structure):
+    """mat_block_diag.h:33-37"""
+    _fields_ = [("super", MatBlockStruct), ("numBlockRows", C.c_size_t), ("numBlockCols", C.c_size_t),
+                ("ownsBlock", C.c_void_p)]
+
+
 class MatDenseComplexStruct(C.Structure):
     _fields_ = [("vtbl", C.c_void_p), ("props", C.c_int), ("numRows", C.c_size_t),
                 ("numCols", C.c_size_t), ("rowStride", C.c_size_t), ("colStride", C.c_size_t),
@@ -111,6 +146,20 @@ def lib():
         "bfFacHelm2MakeMultilevel": (vp, [vp, vp, vp]),
         "bfFacHelm2MakeMultilevelWithOptions": (vp, [vp, vp, vp, vp]),
         "bfhipFacHelm2Plan": (vp, [vp, vp, vp, vp]),
+        "bfQuadtreeNodeGetQuadtreeConst": (vp, [vp]),
+        "bfTreeLevelIterInit": (None, [vp, C.c_int, vp]),
+        "bfTreeLevelIterDeinit": (None, [vp]),
+        "bfTreeLevelIterCurrentDepth": (sz, [vp]),
+        "bfTreeLevelIterIsDone": (C.c_bool, [vp]),
+        "bfTreeLevelIterNext": (None, [vp]),
+        "bfTreeLevelIterGetNumPoints": (sz, [vp]),
+        "bfTreeLevelIterCurrentLevelIsInternal": (C.c_bool, [vp]),
+        "bfFacHelm2Prepare": (sz, [vp, vp, vp, vp, vp]),
+        "bfFacHelm2Make": (vp, [vp, vp, vp, vp, vp, sz]),
+        "bfFacHelm2MakeSingleLevel": (vp, [vp, vp, vp]),
+        "bfFacGetMatProduct": (vp, [vp, C.c_int]),
+        "bfMatProductNumFactors": (sz, [vp]),
+        "bfMatProductGetFactor": (vp, [vp, sz]),
         "bfhipFacHelm2MakeBatched": (vp, [sz, vp, C.POINTER(vp), C.POINTER(vp), vp]),
         "bfMatGetType": (C.c_int, [vp]),
         "bfMatGetNumRows": (sz, [vp]),
@@ -231,8 +280,10 @@ def sample_box(n, x0, x1, y0, y1):
 
 class Helm2:
     """Kernel parameters (helm2.h:10-15): the single layer, the double layer
-    (BF_LAYER_POTENTIAL_PV_DOUBLE) or the combined field alpha S + beta D
-    (BF_LAYER_POTENTIAL_COMBINED_FIELD)."""
+    (BF_LAYER_POTENTIAL_PV_DOUBLE), the combined field alpha S + beta D
+    (BF_LAYER_POTENTIAL_COMBINED_FIELD) or the normal derivative of the
+    single layer S' (BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_SINGLE, target
+    normals)."""
 
     def __init__(self, k, layer_pot=None, alpha=1.0, beta=1.0):
         if layer_pot is None:
@@ -244,19 +295,24 @@ class Helm2:
     def k(self):
         return self.s.k
 
-    def get_kernel_matrix(self, src, tgt, src_normals=None):
+    def get_kernel_matrix(self, src, tgt, src_normals=None, tgt_normals=None):
         """bfHelm2GetKernelMatrix: dense (tgt x src) kernel, sampled on the GPU
-        (source unit normals for the double layer and the combined field)."""
+        (source unit normals for the double layer and the combined field,
+        target unit normals for S')."""
         src = np.ascontiguousarray(src, dtype=np.float64)
         tgt = np.ascontiguousarray(tgt, dtype=np.float64)
         L = lib()
         ps = L.bfPoints2NewViewFromPtr(_dp(src), len(src))
         pt = L.bfPoints2NewViewFromPtr(_dp(tgt), len(tgt))
-        nv = None
+        nv = tv = None
         if src_normals is not None:
             nrm = np.ascontiguousarray(src_normals, dtype=np.float64)
             nv = Vectors2Struct(nrm.ctypes.data, len(nrm), len(nrm), True)
-        m = L.bfHelm2GetKernelMatrix(C.byref(self.s), ps, pt, C.byref(nv) if nv is not None else None, None)
+        if tgt_normals is not None:
+            tnrm = np.ascontiguousarray(tgt_normals, dtype=np.float64)
+            tv = Vectors2Struct(tnrm.ctypes.data, len(tnrm), len(tnrm), True)
+        m = L.bfHelm2GetKernelMatrix(C.byref(self.s), ps, pt, C.byref(nv) if nv is not None else None,
+                                     C.byref(tv) if tv is not None else None)
         L.bfPoints2DeinitAndDealloc(C.byref(C.c_void_p(ps)))
         L.bfPoints2DeinitAndDealloc(C.byref(C.c_void_p(pt)))
         check()
@@ -268,7 +324,8 @@ class Quadtree:
 
     def __init__(self, points, leaf_size=1, normals=None):
         """`normals`: unit normals of the points (original order), needed by
-        the double layer and the combined field."""
+        the double layer and the combined field on a source tree and by S' on
+        a target tree."""
         self.points = np.ascontiguousarray(points, dtype=np.float64)
         L = lib()
         self._pts = L.bfPoints2NewViewFromPtr(_dp(self.points), len(self.points))
@@ -294,6 +351,21 @@ class Quadtree:
     def num_nodes(self):
         return QuadtreeStruct.from_address(self.ptr).numNodes
 
+    def node_ptr(self, i):
+        """Address of node i (breadth-first order) for the node-level API."""
+        q = QuadtreeStruct.from_address(self.ptr)
+        if not 0 <= i < q.numNodes:
+            raise IndexError(i)
+        return q.nodes + i * C.sizeof(QuadtreeNodeStruct)
+
+    def node(self, i):
+        return QuadtreeNodeStruct.from_address(self.node_ptr(i))
+
+    def level(self, d):
+        """Indices of the nodes of depth d."""
+        q = QuadtreeStruct.from_address(self.ptr)
+        return list(range(q.levelOffset[d], q.levelOffset[d + 1]))
+
     def __del__(self):
         try:
             L = lib()
@@ -312,6 +384,17 @@ def options(tol=1e-15, rank_const=1.0, max_dense=128 * 128, pass_through=False, 
     return o
 
 
+def dense_numpy(ptr):
+    """A copy of the BfMatDenseComplex at `ptr` (not taking ownership)."""
+    d = MatDenseComplexStruct.from_address(ptr)
+    n = d.numRows * d.numCols
+    if n == 0:
+        return np.zeros((d.numRows, d.numCols), np.complex128)
+    span = (d.numRows - 1) * d.rowStride + (d.numCols - 1) * d.colStride + 1
+    a = np.ctypeslib.as_array(C.cast(d.data, C.POINTER(C.c_double)), shape=(2 * span,)).copy().view(np.complex128)
+    return np.lib.stride_tricks.as_strided(a, (d.numRows, d.numCols), (16 * d.rowStride, 16 * d.colStride)).copy()
+
+
 class Mat:
     """A BfMat handle owned by Python (freed with bfMatDelete)."""
 
@@ -328,10 +411,7 @@ class Mat:
         return (L.bfMatGetNumRows(self.ptr), L.bfMatGetNumCols(self.ptr))
 
     def to_numpy(self):
-        d = MatDenseComplexStruct.from_address(self.ptr)
-        n = d.numRows * d.numCols
-        a = np.ctypeslib.as_array(C.cast(d.data, C.POINTER(C.c_double)), shape=(2 * n,))
-        return a.copy().view(np.complex128).reshape(d.numRows, d.numCols)
+        return dense_numpy(self.ptr)
 
     def __matmul__(self, x):
         """bfMatMul(self, X): X (N_src x nrhs) in source-tree order."""
@@ -459,6 +539,17 @@ class FacHelm2:
         if not p:
             raise BfError("batched factorization failed")
         return Factorization(p, keep=(tuple(src_trees), tuple(tgt_trees)))
+
+    @staticmethod
+    def make_single_level(helm, src_tree, src_node, tgt_tree, tgt_node):
+        """bfFacHelm2MakeSingleLevel (fac_helm2.c:706-729) for node indices
+        src_node / tgt_node of the two trees: a host BfMatProduct of block
+        factors (the reference's containers) from the source node's points
+        to the target node's points, both in tree order."""
+        p = lib().bfFacHelm2MakeSingleLevel(C.byref(helm.s), src_tree.node_ptr(src_node),
+                                            tgt_tree.node_ptr(tgt_node))
+        check()
+        return Mat(p, keep=(src_tree, tgt_tree))
 
     @staticmethod
     def plan_only(helm, src_tree, tgt_tree, opts=None):
@@ -672,6 +763,10 @@ class Fac:
     def mat(self):
         """bfFacGetMat: the device product Psi W0 ... W_L."""
         return Mat(lib().bfFacGetMat(self.ptr, 1), keep=self._owner)
+
+    def mat_product(self):
+        """bfFacGetMatProduct: the same product as host block containers."""
+        return Mat(lib().bfFacGetMatProduct(self.ptr, 1), keep=self._owner)
 
 
 class FacStreamer:

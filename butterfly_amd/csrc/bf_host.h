@@ -42,12 +42,12 @@ void bfk_reset_times(void);
 int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles);
 int bfk_set_compensation(int on);
 int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY,
-               const double *dSrcN, BfkLayer layer, double *dArena);
+               const double *dSrcN, const double *dTgtN, BfkLayer layer, double *dArena);
 int bfk_lstsq_begin(size_t maxOversizeBytes, int64_t numOversize);
 int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t numJobs, size_t ldsBytes,
                      const double *dSrcXY, const double *dSrcN, BfkLayer layer, double *dArena);
 int bfk_lstsq_end(void);
-int bfk_pair_green(int64_t n, const double *dA, const double *dB, double k, double *dOut);
+int bfk_pair_green(int64_t n, const double *dA, const double *dB, const double *dNA, double k, double *dOut);
 int bfk_scale_rows(int64_t n, int nrhs, const double *dW, const void *dX, void *dU, int fp32);
 int bfk_bie_post(int64_t n, int nrhs, int ncorr, double alpha, const int32_t *dCol, const double *dC,
                  const void *dX, const void *dU, void *dY, int fp32);
@@ -158,6 +158,7 @@ typedef struct BfMatHipButterfly {
   float *dArena32, *dArena32lo;
   double *dSrcXY, *dTgtXY;
   double *dSrcN; /* source unit normals in tree order (double layer, combined field) */
+  double *dTgtN; /* target unit normals in tree order (S') */
   BfkSeg *dSegs;
   BfkTerm *dTerms;
   void *dPing, *dPong;
@@ -174,6 +175,9 @@ int bfPlanBuild(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *t
 /* plans one more problem straight into F as its next diagonal block */
 int bfPlanBuildAt(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt, int64_t srcPts,
                   int64_t tgtPts);
+/* one butterfly on given source / target levels (node indices) */
+int bfPlanButterfly(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt, const int32_t *const *ls,
+                    const size_t *nls, const int32_t *const *lt, const size_t *nlt, int D);
 /* apply plan (fac.c) */
 int bfApplyPlanBuild(BfMatHipButterfly *F);
 /* apply plan and uploads for a plan + arena already in place (fac.c) */

@@ -98,6 +98,16 @@ __device__ __forceinline__ double2 green_layer(const BfkLayer &L, double k, doub
   return make_double2(L.ar * g.x - L.ai * g.y + L.br * dr - L.bi * di, L.ar * g.y + L.ai * g.x + L.br * di + L.bi * dr);
 }
 
+/* S'(t <- s) = (i/4) k H1(k r) nt.(t - s)/r with the target normal nt, 0 for
+ * r < 1e-15 (helm2.c:45-53, :127-175): the double-layer expression with the
+ * target's normal in place of the source's */
+__device__ __forceinline__ double2 green_tnormal(double k, double2 t, double2 s, double2 nt) {
+  const double dx = t.x - s.x, dy = t.y - s.y;
+  double dr, di;
+  bf_helm2_double(k, hypot(dx, dy), nt.x * dx + nt.y * dy, &dr, &di);
+  return make_double2(dr, di);
+}
+
 /* unit normal of a point on a proxy circle (circle.c: the radial direction) */
 __device__ __forceinline__ double2 circle_normal(double cx, double cy, double r, double2 pt) {
   return make_double2((pt.x - cx) / r, (pt.y - cy) / r);
@@ -118,7 +128,8 @@ __device__ __attribute__((noinline)) double2 green_call(BfkLayer L, double k, do
 __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *__restrict__ jobs,
                                                            const double2 *__restrict__ srcXY,
                                                            const double2 *__restrict__ tgtXY,
-                                                           const double2 *__restrict__ srcN, BfkLayer L,
+                                                           const double2 *__restrict__ srcN,
+                                                           const double2 *__restrict__ tgtN, BfkLayer L,
                                                            double2 *__restrict__ arena) {
   __shared__ double2 spts[SAMPLE_CHUNK];
   __shared__ double2 snrm[SAMPLE_CHUNK];
@@ -130,7 +141,7 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *_
     for (int j = threadIdx.x; j < cn; j += SAMPLE_THREADS) {
       const double2 pt = J.srcOff >= 0 ? srcXY[J.srcOff + c0 + j] : circle_pt(J.scx, J.scy, J.sr, J.n, c0 + j);
       spts[j] = pt;
-      if (L.layer != 0)
+      if (L.layer == 1 || L.layer == 2)
         snrm[j] = J.srcOff >= 0 ? srcN[J.srcOff + c0 + j] : circle_normal(J.scx, J.scy, J.sr, pt);
     }
     __syncthreads();
@@ -139,8 +150,9 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *_
     for (int64_t e = threadIdx.x; e < cnt; e += SAMPLE_THREADS) {
       int j = (int)(e / m);
       int i = (int)(e - (int64_t)j * m);
-      out[e] = L.layer == 0 ? green(J.k, tgtXY[J.tgtOff + i], spts[j])
-                            : green_layer(L, J.k, tgtXY[J.tgtOff + i], spts[j], snrm[j]);
+      out[e] = L.layer == 0   ? green(J.k, tgtXY[J.tgtOff + i], spts[j])
+               : L.layer == 3 ? green_tnormal(J.k, tgtXY[J.tgtOff + i], spts[j], tgtN[J.tgtOff + i])
+                              : green_layer(L, J.k, tgtXY[J.tgtOff + i], spts[j], snrm[j]);
     }
   }
 }
@@ -891,11 +903,12 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
 
 /* ------------------------------------ boundary integral operator, GMRES */
 
-/* G(a_i <- b_i) for a list of point pairs (the quadrature corrections) */
-__global__ void k_pair_green(int64_t n, const double2 *__restrict__ a, const double2 *__restrict__ b, double k,
-                             double2 *__restrict__ out) {
+/* G(a_i <- b_i) for a list of point pairs (the quadrature corrections), or
+ * S'(a_i <- b_i) with the normal na_i at a_i when na is given */
+__global__ void k_pair_green(int64_t n, const double2 *__restrict__ a, const double2 *__restrict__ b,
+                             const double2 *__restrict__ na, double k, double2 *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = green(k, a[i], b[i]);
+  if (i < n) out[i] = na ? green_tnormal(k, a[i], b[i], na[i]) : green(k, a[i], b[i]);
 }
 
 /* u = w .* x, row i scaled by w[i], nrhs columns (row-major) */
@@ -2293,7 +2306,7 @@ static int time_end(int slot) {
 
 /* kernel blocks; jobs already in device memory */
 int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY, const double *dTgtXY,
-               const double *dSrcN, BfkLayer layer, double *dArena) {
+               const double *dSrcN, const double *dTgtN, BfkLayer layer, double *dArena) {
   if (bfk_init()) return -1;
   if (numJobs == 0) {
     g_times[0] = 0;
@@ -2304,8 +2317,8 @@ int bfk_sample(const BfkKernelJob *dJobs, int64_t numJobs, const double *dSrcXY,
   for (int64_t j0 = 0; j0 < numJobs; j0 += maxGrid) {
     int64_t nj = numJobs - j0 < maxGrid ? numJobs - j0 : maxGrid;
     hipLaunchKernelGGL(k_sample, dim3((unsigned)nj), dim3(SAMPLE_THREADS), 0, g_stream, dJobs + j0,
-                       (const double2 *)dSrcXY, (const double2 *)dTgtXY, (const double2 *)dSrcN, layer,
-                       (double2 *)dArena);
+                       (const double2 *)dSrcXY, (const double2 *)dTgtXY, (const double2 *)dSrcN,
+                       (const double2 *)dTgtN, layer, (double2 *)dArena);
     BFK_CHECK(hipGetLastError());
   }
   return time_end(0);
@@ -2390,11 +2403,11 @@ int bfk_lstsq_end(void) {
 
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-int bfk_pair_green(int64_t n, const double *dA, const double *dB, double k, double *dOut) {
+int bfk_pair_green(int64_t n, const double *dA, const double *dB, const double *dNA, double k, double *dOut) {
   if (bfk_init()) return -1;
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_pair_green, dim3(blocks_for(n, 256)), dim3(256), 0, g_stream, n, (const double2 *)dA,
-                     (const double2 *)dB, k, (double2 *)dOut);
+                     (const double2 *)dB, (const double2 *)dNA, k, (double2 *)dOut);
   BFK_CHECK(hipGetLastError());
   return 0;
 }

@@ -49,7 +49,9 @@ typedef struct BfkKernelJob {
 
 /* The layer potential of a factorization (uniform over its jobs): 0 the
  * single layer S, 1 the double layer D (source normals), 2 the combined
- * field alpha S + beta D; alpha = ar + i ai, beta = br + i bi. */
+ * field alpha S + beta D; alpha = ar + i ai, beta = br + i bi; 3 the
+ * normal derivative S' of the single layer (target normals; sampling only:
+ * its re-expansions use the single layer, layer_pot.h:63-69). */
 typedef struct BfkLayer {
   int32_t layer, pad;
   double ar, ai, br, bi;

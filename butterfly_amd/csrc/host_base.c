@@ -334,6 +334,37 @@ static int build_deferred(TreeBuilder *b) {
   return 0;
 }
 
+/* Live quadtrees, so that a node can name its tree
+ * (bfQuadtreeNodeGetQuadtreeConst): nodes live in one array per tree and
+ * carry no pointer back to it.  Single-threaded, like the rest of the
+ * host API. */
+static BF_VEC(BfQuadtree const *) g_trees;
+
+static void tree_register(BfQuadtree const *t) {
+  for (size_t i = 0; i < g_trees.n; ++i)
+    if (g_trees.a[i] == t) return;
+  BF_VEC_PUSH(g_trees, t);
+}
+
+static void tree_unregister(BfQuadtree const *t) {
+  for (size_t i = 0; i < g_trees.n; ++i)
+    if (g_trees.a[i] == t) {
+      g_trees.a[i] = g_trees.a[--g_trees.n];
+      return;
+    }
+}
+
+BfQuadtree const *bfQuadtreeNodeGetQuadtreeConst(BfQuadtreeNode const *node) {
+  for (size_t i = 0; node && i < g_trees.n; ++i) {
+    BfQuadtree const *t = g_trees.a[i];
+    if (t->nodes && node >= t->nodes && node < t->nodes + t->numNodes) return t;
+  }
+  bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+  return NULL;
+}
+
+BfTreeNode *bfQuadtreeNodeToTreeNode(BfQuadtreeNode *node) { return node; }
+
 BfQuadtree *bfQuadtreeNew(void) {
   BfQuadtree *t = calloc(1, sizeof(BfQuadtree));
   if (!t) bfSetError(BF_ERROR_MEMORY_ERROR);
@@ -443,6 +474,7 @@ void bfQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points, BfVec
   tree->super.perm = calloc(1, sizeof(BfPerm));
   tree->super.perm->index = b.perm;
   tree->super.perm->size = n;
+  tree_register(tree);
 }
 
 void bfQuadtreeInit(BfQuadtree *tree, BfPoints2 const *points, BfVectors2 const *unitNormals) {
@@ -452,6 +484,7 @@ void bfQuadtreeInit(BfQuadtree *tree, BfPoints2 const *points, BfVectors2 const 
 
 void bfQuadtreeDeinit(BfQuadtree *tree) {
   if (!tree) return;
+  tree_unregister(tree);
   free(tree->nodes);
   free(tree->levelOffset);
   if (tree->super.perm) {
@@ -507,3 +540,108 @@ BfSize bfHelm2RankEstForTwoCircles(BfHelm2 const *helm, BfCircle const *circ1, B
   BfReal p = helm->k * r1 * r2 / d - C * log10(eps);
   return (BfSize)ceil(p);
 }
+
+/* ------------------------------------------------- tree level iterators */
+
+/* tree_level_iter.c:10-330: the nodes of the subtree under a node in LR
+ * level order (breadth first, children in quadrant order), one level at a
+ * time, from the top (LR_LEVEL_ORDER) or from the deepest level up
+ * (LR_REVERSE_LEVEL_ORDER).  `levelNodes` views the current level; Next
+ * past the last level leaves the view on it (IsDone turns true going
+ * down; going up the view stays on the top level). */
+typedef struct {
+  BfSize current, numLevels;
+  BfSize *offsets;
+  BfPtrArray view;
+} LevelIterAux;
+
+static void level_view(BfTreeLevelIter *it) {
+  LevelIterAux *a = it->aux;
+  const BfSize lv = a->current < a->numLevels ? a->current : a->numLevels - 1;
+  a->view.data = it->nodes.data + a->offsets[lv];
+  a->view.num_elts = a->view.capacity = a->offsets[lv + 1] - a->offsets[lv];
+  a->view.isView = true;
+  it->levelNodes = &a->view;
+}
+
+void bfTreeLevelIterInit(BfTreeLevelIter *it, BfTreeTraversal traversal, BfTreeNode *node) {
+  memset(it, 0, sizeof(*it));
+  it->traversal = traversal == BF_TREE_TRAVERSAL_UNKNOWN ? BF_TREE_TRAVERSAL_LR_LEVEL_ORDER : traversal;
+  BfQuadtree const *t = bfQuadtreeNodeGetQuadtreeConst(node);
+  if (!t || (it->traversal != BF_TREE_TRAVERSAL_LR_LEVEL_ORDER &&
+             it->traversal != BF_TREE_TRAVERSAL_LR_REVERSE_LEVEL_ORDER)) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return;
+  }
+  bfInitPtrArrayWithDefaultCapacity(&it->nodes);
+  bfPtrArrayAppend(&it->nodes, node);
+  for (BfSize i = 0; i < it->nodes.num_elts; ++i) {
+    BfQuadtreeNode const *v = it->nodes.data[i];
+    for (int q = 0; q < 4; ++q)
+      if (v->child[q] >= 0) bfPtrArrayAppend(&it->nodes, (void *)&t->nodes[v->child[q]]);
+  }
+  LevelIterAux *a = calloc(1, sizeof(LevelIterAux));
+  const BfQuadtreeNode *first = it->nodes.data[0], *last = it->nodes.data[it->nodes.num_elts - 1];
+  a->numLevels = (BfSize)(last->depth - first->depth + 1);
+  a->offsets = calloc(a->numLevels + 1, sizeof(BfSize));
+  BfSize lv = 1;
+  for (BfSize i = 1; i < it->nodes.num_elts; ++i)
+    if (((BfQuadtreeNode const *)it->nodes.data[i])->depth != ((BfQuadtreeNode const *)it->nodes.data[i - 1])->depth)
+      a->offsets[lv++] = i;
+  a->offsets[a->numLevels] = it->nodes.num_elts;
+  a->current = it->traversal == BF_TREE_TRAVERSAL_LR_LEVEL_ORDER ? 0 : a->numLevels - 1;
+  it->aux = a;
+  level_view(it);
+}
+
+void bfTreeLevelIterDeinit(BfTreeLevelIter *it) {
+  if (!it) return;
+  bfPtrArrayDeinit(&it->nodes);
+  LevelIterAux *a = it->aux;
+  if (a) {
+    free(a->offsets);
+    free(a);
+  }
+  it->aux = NULL;
+  it->levelNodes = NULL;
+}
+
+BfSize bfTreeLevelIterCurrentDepth(BfTreeLevelIter const *it) {
+  if (!it || !it->levelNodes || it->levelNodes->num_elts == 0) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return BF_SIZE_BAD_VALUE;
+  }
+  return (BfSize)((BfQuadtreeNode const *)it->levelNodes->data[0])->depth;
+}
+
+bool bfTreeLevelIterIsDone(BfTreeLevelIter const *it) {
+  LevelIterAux const *a = it->aux;
+  return a->current == a->numLevels;
+}
+
+void bfTreeLevelIterNext(BfTreeLevelIter *it) {
+  LevelIterAux *a = it->aux;
+  if (it->traversal == BF_TREE_TRAVERSAL_LR_LEVEL_ORDER) {
+    if (a->current < a->numLevels) ++a->current;
+  } else if (a->current > 0) {
+    --a->current;
+  }
+  if (a->current < a->numLevels) level_view(it);
+}
+
+BfSize bfTreeLevelIterGetNumPoints(BfTreeLevelIter const *it) {
+  BfSize n = 0;
+  for (BfSize i = 0; i < it->levelNodes->num_elts; ++i)
+    n += bfQuadtreeNodeGetNumPoints(it->levelNodes->data[i]);
+  return n;
+}
+
+bool bfTreeLevelIterCurrentLevelIsInternal(BfTreeLevelIter const *it) {
+  for (BfSize i = 0; i < it->levelNodes->num_elts; ++i) {
+    BfQuadtreeNode const *v = it->levelNodes->data[i];
+    if (v->child[0] < 0 && v->child[1] < 0 && v->child[2] < 0 && v->child[3] < 0) return false;
+  }
+  return true;
+}
+
+BfPtrArray *bfTreeLevelIterGetLevelNodes(BfTreeLevelIter *it) { return it ? it->levelNodes : NULL; }

@@ -72,9 +72,13 @@ BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points
   }
   /* the corrections treat the logarithmic singularity of the single layer:
    * none for the (smooth) double layer, alpha times them for the combined
-   * field alpha S + beta D */
-  const BfLayerPotential lp = ((BfMatHipButterfly const *)fac)->helm.layerPot;
-  if (lp == BF_LAYER_POTENTIAL_PV_DOUBLE && krOrder != 0) {
+   * field alpha S + beta D; for S' they are S' values at the neighbours with
+   * the target point's normal (krComplexKernel, helm2.c:372-378), read from
+   * the factorization's target normals (the boundary's own) */
+  BfMatHipButterfly const *Fb = (BfMatHipButterfly const *)fac;
+  const BfLayerPotential lp = Fb->helm.layerPot;
+  const int sprime = lp == BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_SINGLE;
+  if ((lp == BF_LAYER_POTENTIAL_PV_DOUBLE && krOrder != 0) || (sprime && krOrder != 0 && !Fb->dTgtN)) {
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return NULL;
   }
@@ -97,6 +101,10 @@ BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points
   int32_t *col = malloc((size_t)(np ? np : 1) * sizeof(int32_t));
   double *pa = malloc((size_t)(np ? np : 1) * 2 * sizeof(double)), *pb = malloc((size_t)(np ? np : 1) * 2 * sizeof(double));
   double *gam = malloc((size_t)(np ? np : 1) * sizeof(double));
+  /* S': the normal at each row's point (tree order on the device) */
+  double *tn = sprime && np ? malloc((size_t)n * 16) : NULL, *pn = sprime && np ? malloc((size_t)np * 16) : NULL;
+  int rc = 0;
+  if (tn) rc = bfk_d2h(tn, Fb->dTgtN, (size_t)n * 16);
   for (int64_t i = 0; i < n; ++i) {
     const int64_t oi = (int64_t)perm->index[i];
     for (int p = 0; p < ord; ++p)
@@ -109,21 +117,26 @@ BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points
         pb[2 * q] = points->data[oj][0];
         pb[2 * q + 1] = points->data[oj][1];
         gam[q] = g[p];
+        if (pn && tn) {
+          pn[2 * q] = tn[2 * i];
+          pn[2 * q + 1] = tn[2 * i + 1];
+        }
       }
   }
-  int rc = 0;
-  void *dA = NULL, *dB = NULL;
+  void *dA = NULL, *dB = NULL, *dNA = NULL;
   A->dW = bfk_malloc((size_t)n * 8);
   A->dCol = bfk_malloc((size_t)(np ? np : 1) * 4);
   A->dC = bfk_malloc((size_t)(np ? np : 1) * 16);
   dA = bfk_malloc((size_t)(np ? np : 1) * 16);
   dB = bfk_malloc((size_t)(np ? np : 1) * 16);
-  if (!A->dW || !A->dCol || !A->dC || !dA || !dB) rc = -2;
+  if (pn) dNA = bfk_malloc((size_t)np * 16);
+  if (!A->dW || !A->dCol || !A->dC || !dA || !dB || (sprime && np && (!tn || !pn || !dNA))) rc = rc ? rc : -2;
+  if (!rc && dNA) rc = bfk_h2d(dNA, pn, (size_t)np * 16);
   if (!rc) rc = bfk_h2d(A->dW, w, (size_t)n * 8);
   if (!rc && np) rc = bfk_h2d(A->dCol, col, (size_t)np * 4);
   if (!rc && np) rc = bfk_h2d(dA, pa, (size_t)np * 16);
   if (!rc && np) rc = bfk_h2d(dB, pb, (size_t)np * 16);
-  if (!rc && np) rc = bfk_pair_green(np, dA, dB, helm->k, A->dC);
+  if (!rc && np) rc = bfk_pair_green(np, dA, dB, (const double *)dNA, helm->k, A->dC);
   if (!rc && np) { /* scale by the correction weights on the host */
     double *c = malloc((size_t)np * 16);
     rc = bfk_d2h(c, A->dC, (size_t)np * 16);
@@ -137,7 +150,8 @@ BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points
   }
   bfk_free(dA);
   bfk_free(dB);
-  free(rev); free(w); free(col); free(pa); free(pb); free(gam);
+  bfk_free(dNA);
+  free(rev); free(w); free(col); free(pa); free(pb); free(gam); free(tn); free(pn);
   if (rc) {
     BfMat *m = &A->super;
     bie_delete(&m);

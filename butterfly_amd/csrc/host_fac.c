@@ -53,13 +53,14 @@ static void free_device(BfMatHipButterfly *F) {
   bfk_free(F->dSrcXY);
   bfk_free(F->dTgtXY);
   bfk_free(F->dSrcN);
+  bfk_free(F->dTgtN);
   bfk_free(F->dSegs);
   bfk_free(F->dTerms);
   bfk_free(F->dPing);
   bfk_free(F->dPong);
   F->dArena = NULL;
   F->dArena32 = F->dArena32lo = NULL;
-  F->dSrcXY = F->dTgtXY = F->dSrcN = NULL;
+  F->dSrcXY = F->dTgtXY = F->dSrcN = F->dTgtN = NULL;
   F->dSegs = NULL;
   F->dTerms = NULL;
   F->dPing = F->dPong = NULL;
@@ -361,16 +362,39 @@ static double *tree_normals(BfQuadtree const *t) {
   return nv;
 }
 
-/* the kernels' form of the layer potential (the proxies use the same one,
- * layer_pot.h:63-69, for S, D and the combined field) */
+/* the layers that read source normals (D and the combined field) and the
+ * one that reads target normals (S'), layer_pot.h:40-48 */
+static int uses_src_normals(BfLayerPotential lp) {
+  return lp == BF_LAYER_POTENTIAL_PV_DOUBLE || lp == BF_LAYER_POTENTIAL_COMBINED_FIELD;
+}
+static int uses_tgt_normals(BfLayerPotential lp) { return lp == BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_SINGLE; }
+
+/* the kernels' form of a layer potential: the kernel blocks (last factors,
+ * values blocks, dense leaves) sample the layer itself */
 static BfkLayer layer_of(BfHelm2 const *h) {
   BfkLayer L;
   memset(&L, 0, sizeof(L));
-  L.layer = h->layerPot == BF_LAYER_POTENTIAL_PV_DOUBLE ? 1 : h->layerPot == BF_LAYER_POTENTIAL_COMBINED_FIELD ? 2 : 0;
+  switch (h->layerPot) {
+  case BF_LAYER_POTENTIAL_PV_DOUBLE: L.layer = 1; break;
+  case BF_LAYER_POTENTIAL_COMBINED_FIELD: L.layer = 2; break;
+  case BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_SINGLE: L.layer = 3; break;
+  default: L.layer = 0;
+  }
   L.ar = creal(h->alpha);
   L.ai = cimag(h->alpha);
   L.br = creal(h->beta);
   L.bi = cimag(h->beta);
+  return L;
+}
+
+/* the layer of the re-expansions (BF_PROXY_LAYER_POT, layer_pot.h:63-69):
+ * the layer itself for S, D and the combined field; S' drops its target
+ * normal derivative and re-expands with the single layer (its proxies hold
+ * single-layer charges, fac_helm2.c:58-59, :230-231), so the target normals
+ * enter only through the kernel blocks */
+static BfkLayer proxy_layer_of(BfHelm2 const *h) {
+  BfkLayer L = layer_of(h);
+  if (L.layer == 3) L.layer = 0;
   return L;
 }
 
@@ -531,8 +555,8 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   bfTrace("factor: arena allocated", tk);
   void *dK = NULL;
   if ((rc = upload(&dK, F->kernelJobs.a, F->kernelJobs.n * sizeof(BfkKernelJob)))) return rc;
-  const BfkLayer layer = layer_of(&F->helm);
-  rc = bfk_sample(dK, (int64_t)F->kernelJobs.n, F->dSrcXY, F->dTgtXY, F->dSrcN, layer, F->dArena);
+  const BfkLayer layer = layer_of(&F->helm), proxyLayer = proxy_layer_of(&F->helm);
+  rc = bfk_sample(dK, (int64_t)F->kernelJobs.n, F->dSrcXY, F->dTgtXY, F->dSrcN, F->dTgtN, layer, F->dArena);
   bfk_free(dK);
   if (rc) return rc;
   bfTrace("factor: sampling launched", tk);
@@ -595,7 +619,7 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
       }
       const size_t classEnd = (size_t)gs.a[gEnd];
       const size_t end = classEnd < up ? classEnd : up;
-      rc = bfk_lstsq_launch(dL + launched, NULL, (int64_t)(end - launched), lds, F->dSrcXY, F->dSrcN, layer,
+      rc = bfk_lstsq_launch(dL + launched, NULL, (int64_t)(end - launched), lds, F->dSrcXY, F->dSrcN, proxyLayer,
                             F->dArena);
       launched = end;
       if (launched == classEnd) g = gEnd;
@@ -627,19 +651,23 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
 /* Everything after planning: coordinates up, sampling and compression
  * launched, the apply plan built on the host and its tables uploaded
  * meanwhile, then the statistics.  Takes ownership of sxy / txy (tree-ordered
- * coordinates) and snrm (tree-ordered source normals or NULL); on failure
- * deletes F and returns NULL. */
-static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, double *snrm, double t0, double t1) {
+ * coordinates), snrm and tnrm (tree-ordered source / target normals or
+ * NULL); on failure deletes F and returns NULL. */
+static BfMat *factor_from_plan(BfMatHipButterfly *F, double *sxy, double *txy, double *snrm, double *tnrm,
+                               double t0, double t1) {
   /* the normal-dependent layers read the normals in every sampling and
    * compression job: no kernel runs without them */
-  int rc = (F->helm.layerPot != BF_LAYER_POTENTIAL_SINGLE && !snrm) ? -2 : 0;
+  const BfLayerPotential lp = F->helm.layerPot;
+  int rc = ((uses_src_normals(lp) && !snrm) || (uses_tgt_normals(lp) && !tnrm)) ? -2 : 0;
   bfTrace("factor: coordinates (since plan)", t1);
   if (!rc) rc = upload((void **)&F->dSrcXY, sxy, 2 * F->numSrc * sizeof(double));
   if (!rc) rc = upload((void **)&F->dTgtXY, txy, 2 * F->numTgt * sizeof(double));
   if (!rc && snrm) rc = upload((void **)&F->dSrcN, snrm, 2 * F->numSrc * sizeof(double));
+  if (!rc && tnrm) rc = upload((void **)&F->dTgtN, tnrm, 2 * F->numTgt * sizeof(double));
   free(sxy);
   free(txy);
   free(snrm);
+  free(tnrm);
   double tApplyPlan = 0;
   if (!rc) rc = run_factor_kernels(F, &tApplyPlan);
   if (!rc) rc = bfk_sync();
@@ -675,15 +703,20 @@ static int problem_ok(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return 0;
   }
-  /* the single layer, the double layer and the combined field; the
-   * target-normal layers have no re-expansion (helm2.c:341-342) */
-  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE && helm->layerPot != BF_LAYER_POTENTIAL_PV_DOUBLE &&
-      helm->layerPot != BF_LAYER_POTENTIAL_COMBINED_FIELD) {
+  /* S, D, the combined field and S' (re-expanded with the single layer,
+   * layer_pot.h:63-69); the normal derivative of the double layer is the
+   * hypersingular operator, which the reference does not sample either
+   * (helm2.c:282-319 has no case for it) */
+  const BfLayerPotential lp = helm->layerPot;
+  if (lp != BF_LAYER_POTENTIAL_SINGLE && !uses_src_normals(lp) && !uses_tgt_normals(lp)) {
     bfSetError(BF_ERROR_NOT_IMPLEMENTED);
     return 0;
   }
-  if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE &&
-      (!srcTree->unitNormals || srcTree->unitNormals->size != srcTree->points->size)) {
+  if (uses_src_normals(lp) && (!srcTree->unitNormals || srcTree->unitNormals->size != srcTree->points->size)) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return 0;
+  }
+  if (uses_tgt_normals(lp) && (!tgtTree->unitNormals || tgtTree->unitNormals->size != tgtTree->points->size)) {
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return 0;
   }
@@ -713,7 +746,8 @@ BfMat *bfFacHelm2MakeMultilevelWithOptions(BfHelm2 const *helm, BfQuadtree const
   double t1 = bfWallTime();
   bfTrace("factor: plan", t0);
   return factor_from_plan(F, tree_coords(srcTree), tree_coords(tgtTree),
-                          helm->layerPot != BF_LAYER_POTENTIAL_SINGLE ? tree_normals(srcTree) : NULL, t0, t1);
+                          uses_src_normals(helm->layerPot) ? tree_normals(srcTree) : NULL,
+                          uses_tgt_normals(helm->layerPot) ? tree_normals(tgtTree) : NULL, t0, t1);
 }
 
 BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree const *const *srcTrees,
@@ -731,7 +765,7 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
       return NULL;
     }
   }
-  const int withNormals = helms[0].layerPot != BF_LAYER_POTENTIAL_SINGLE;
+  const int withNormals = uses_src_normals(helms[0].layerPot), withTgtNormals = uses_tgt_normals(helms[0].layerPot);
   if (bfk_init()) {
     bfSetError(BF_ERROR_RUNTIME_ERROR);
     return NULL;
@@ -748,8 +782,9 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
   bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
   double *sxy = malloc(2 * F->numSrc * sizeof(double)), *txy = malloc(2 * F->numTgt * sizeof(double));
   double *snrm = withNormals ? malloc(2 * F->numSrc * sizeof(double)) : NULL;
+  double *tnrm = withTgtNormals ? malloc(2 * F->numTgt * sizeof(double)) : NULL;
   int64_t sBase = 0, tBase = 0;
-  int bad = !sxy || !txy;
+  int bad = !sxy || !txy || (withNormals && !snrm) || (withTgtNormals && !tnrm);
   for (BfSize i = 0; i < count && !bad; ++i) {
     /* the planner reads the problem's wavenumber and options from F */
     F->helm = helms[i];
@@ -768,6 +803,11 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
         memcpy(snrm + 2 * sBase, c, 2 * ns * sizeof(double));
         free(c);
       }
+      if (tnrm) {
+        double *c = tree_normals(tgtTrees[i]);
+        memcpy(tnrm + 2 * tBase, c, 2 * nt * sizeof(double));
+        free(c);
+      }
       sBase += (int64_t)ns;
       tBase += (int64_t)nt;
     }
@@ -778,12 +818,13 @@ BfMat *bfhipFacHelm2MakeBatched(BfSize count, BfHelm2 const *helms, BfQuadtree c
     free(sxy);
     free(txy);
     free(snrm);
+    free(tnrm);
     bfMatHipButterflyDelete((BfMat **)&F);
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return NULL;
   }
   double t1 = bfWallTime();
-  return factor_from_plan(F, sxy, txy, snrm, t0, t1);
+  return factor_from_plan(F, sxy, txy, snrm, tnrm, t0, t1);
 }
 
 BfMat *bfhipFacHelm2Plan(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
@@ -815,6 +856,221 @@ BfMat *bfFacHelm2MakeMultilevel(BfHelm2 const *helm, BfQuadtree const *srcTree, 
   BfFacHelm2Options opts;
   bfFacHelm2OptionsInitDefault(&opts);
   return bfFacHelm2MakeMultilevelWithOptions(helm, srcTree, tgtTree, &opts);
+}
+
+/* ---------------------------------------- one butterfly, host containers */
+
+/* allRankEstimatesAreOK (fac_helm2.c:511-530) */
+static int ranks_ok(BfHelm2 const *helm, BfQuadtreeNode const *tgtNode, BfPtrArray const *lv) {
+  const BfCircle tc = bfQuadtreeNodeGetBoundingCircle(tgtNode);
+  for (BfSize i = 0; i < lv->num_elts; ++i) {
+    BfQuadtreeNode const *sn = lv->data[i];
+    const BfCircle sc = bfQuadtreeNodeGetBoundingCircle(sn);
+    const double R = hypot(sc.center[0] - tc.center[0], sc.center[1] - tc.center[1]);
+    if (!(R - tc.r - sc.r > 0)) return 0; /* not separated: no estimate */
+    if (bfHelm2RankEstForTwoCircles(helm, &tc, &sc, 1, 1e-15) > bfQuadtreeNodeGetNumPoints(sn)) return 0;
+  }
+  return 1;
+}
+
+BfSize bfFacHelm2Prepare(BfHelm2 const *helm, BfQuadtreeNode const *srcNode, BfQuadtreeNode const *tgtNode,
+                         BfTreeLevelIter *srcLevelIter, BfTreeLevelIter *tgtLevelIter) {
+  if (!helm || !(helm->k > 0) || !srcNode || !tgtNode || !srcLevelIter || !tgtLevelIter) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return 0;
+  }
+  const BfSize numSrc = bfQuadtreeNodeGetNumPoints(srcNode);
+  bfTreeLevelIterInit(srcLevelIter, BF_TREE_TRAVERSAL_LR_REVERSE_LEVEL_ORDER, (BfTreeNode *)srcNode);
+  bfTreeLevelIterInit(tgtLevelIter, BF_TREE_TRAVERSAL_LR_LEVEL_ORDER, (BfTreeNode *)tgtNode);
+  if (!srcLevelIter->aux || !tgtLevelIter->aux) {
+    bfTreeLevelIterDeinit(srcLevelIter);
+    bfTreeLevelIterDeinit(tgtLevelIter);
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return 0;
+  }
+  /* the deepest fully internal level below the target node */
+  BfSize maxAllow = bfTreeLevelIterCurrentDepth(tgtLevelIter);
+  bfTreeLevelIterNext(tgtLevelIter);
+  while (!bfTreeLevelIterIsDone(tgtLevelIter) && bfTreeLevelIterCurrentLevelIsInternal(tgtLevelIter)) {
+    ++maxAllow;
+    bfTreeLevelIterNext(tgtLevelIter);
+  }
+  bfTreeLevelIterDeinit(tgtLevelIter);
+  bfTreeLevelIterInit(tgtLevelIter, BF_TREE_TRAVERSAL_LR_LEVEL_ORDER, (BfTreeNode *)tgtNode);
+  const BfSize top = (BfSize)srcNode->depth, tgtDepth = (BfSize)tgtNode->depth;
+  BfSize depth = bfTreeLevelIterCurrentDepth(srcLevelIter);
+  /* up to that depth, then to a complete, then to an internal level, then
+   * until every rank estimate fits its node's points (never above the
+   * source node itself) */
+  while (depth > maxAllow && depth > top) { bfTreeLevelIterNext(srcLevelIter); --depth; }
+  while (bfTreeLevelIterGetNumPoints(srcLevelIter) != numSrc && depth > top) {
+    bfTreeLevelIterNext(srcLevelIter);
+    --depth;
+  }
+  while (!bfTreeLevelIterCurrentLevelIsInternal(srcLevelIter) && depth > top) {
+    bfTreeLevelIterNext(srcLevelIter);
+    --depth;
+  }
+  while (depth > tgtDepth && depth > top && !ranks_ok(helm, tgtNode, srcLevelIter->levelNodes)) {
+    bfTreeLevelIterNext(srcLevelIter);
+    --depth;
+  }
+  if (depth < tgtDepth || !ranks_ok(helm, tgtNode, srcLevelIter->levelNodes)) return 0;
+  return depth - tgtDepth + 2;
+}
+
+/* the nodes of an iterator's current level as indices into its tree */
+static int32_t *level_indices(BfTreeLevelIter const *it, BfQuadtree const *t, size_t *n) {
+  *n = it->levelNodes->num_elts;
+  int32_t *out = malloc((*n ? *n : 1) * sizeof(int32_t));
+  for (size_t i = 0; i < *n; ++i) {
+    BfQuadtreeNode const *v = it->levelNodes->data[i];
+    if (v < t->nodes || v >= t->nodes + t->numNodes) {
+      free(out);
+      return NULL;
+    }
+    out[i] = (int32_t)(v - t->nodes);
+  }
+  return out;
+}
+
+/* m x n dense complex (row-major) from a column-major block; NULL data is
+ * the identity */
+static BfMat *host_block(const BfComplex *a, int64_t m, int64_t n) {
+  BfMatDenseComplex *D = bfMatDenseComplexNew();
+  bfMatDenseComplexInit(D, (BfSize)m, (BfSize)n);
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t j = 0; j < n; ++j) D->data[i * n + j] = a ? a[j * m + i] : (i == j ? 1.0 : 0.0);
+  return bfMatDenseComplexToMat(D);
+}
+
+/* leaf `leaf` of F as the reference's containers: a BfMatProduct of its
+ * factors, last factor first (fac_helm2.c:693-696), each a BfMatBlockDiag
+ * or BfMatBlockCoo of dense blocks copied from the host copy of the arena */
+static BfMatProduct *host_product(BfMatHipButterfly const *F, const BfComplex *arena, size_t leaf) {
+  const BfHipLeaf *L = &F->leaves.a[leaf];
+  BfMatProduct *P = bfMatProductNew();
+  bfMatProductInit(P);
+  for (int f = L->numFactors - 1; f >= 0; --f) {
+    const BfHipFactor *fa = &F->factors.a[L->firstFactor + f];
+    BfMatBlock *B;
+    bool *owns;
+    BfMat *m;
+    if (fa->isDiag) {
+      BfMatBlockDiag *d = bfMatBlockDiagNew();
+      bfMatBlockDiagInit(d, (BfSize)fa->numBlockRows, (BfSize)fa->numBlockCols);
+      B = &d->super;
+      owns = d->ownsBlock;
+      m = bfMatBlockDiagToMat(d);
+    } else {
+      BfMatBlockCoo *c = bfMatBlockCooNew();
+      bfMatBlockCooInit(c, (BfSize)fa->numBlockRows, (BfSize)fa->numBlockCols, (BfSize)fa->numBlocks);
+      for (int64_t i = 0; i < fa->numBlocks; ++i) {
+        c->rowInd[i] = (BfSize)F->blocks.a[fa->firstBlock + i].rowInd;
+        c->colInd[i] = (BfSize)F->blocks.a[fa->firstBlock + i].colInd;
+      }
+      B = &c->super;
+      owns = c->ownsBlock;
+      m = &c->super.super;
+    }
+    for (int64_t i = 0; i <= fa->numBlockRows; ++i) B->rowOffset[i] = (BfSize)F->offs.a[fa->rowOffStart + i];
+    for (int64_t j = 0; j <= fa->numBlockCols; ++j) B->colOffset[j] = (BfSize)F->offs.a[fa->colOffStart + j];
+    for (int64_t i = 0; i < fa->numBlocks; ++i) {
+      const BfHipBlock *b = &F->blocks.a[fa->firstBlock + i];
+      B->block[i] = host_block(b->dataOff >= 0 ? arena + b->dataOff : NULL, b->m, b->n);
+      owns[i] = true;
+    }
+    m->numRows = B->rowOffset[fa->numBlockRows];
+    m->numCols = B->colOffset[fa->numBlockCols];
+    bfMatProductPostMultiply(P, m);
+  }
+  return P;
+}
+
+BfMatProduct *bfFacHelm2Make(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
+                             BfTreeLevelIter *srcLevelIter, BfTreeLevelIter *tgtLevelIter, BfSize numFactors) {
+  BfFacHelm2Options opts;
+  bfFacHelm2OptionsInitDefault(&opts);
+  if (numFactors < 2 || !srcLevelIter || !tgtLevelIter || !srcLevelIter->aux || !tgtLevelIter->aux ||
+      !problem_ok(helm, srcTree, tgtTree, &opts)) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  if (bfk_init()) {
+    bfSetError(BF_ERROR_RUNTIME_ERROR);
+    return NULL;
+  }
+  /* the levels, walked as the reference walks them: the source iterator
+   * up one level before each middle factor, the target iterator down one
+   * after it (fac_helm2.c:674-686) */
+  const int D = (int)numFactors - 2;
+  int32_t **ls = calloc((size_t)D + 1, sizeof(int32_t *)), **lt = calloc((size_t)D + 1, sizeof(int32_t *));
+  size_t *nls = calloc((size_t)D + 1, sizeof(size_t)), *nlt = calloc((size_t)D + 1, sizeof(size_t));
+  int bad = 0;
+  ls[D] = level_indices(srcLevelIter, srcTree, &nls[D]);
+  lt[0] = level_indices(tgtLevelIter, tgtTree, &nlt[0]);
+  bad = !ls[D] || !lt[0];
+  for (int i = 1; i <= D && !bad; ++i) {
+    bfTreeLevelIterNext(srcLevelIter);
+    ls[D - i] = level_indices(srcLevelIter, srcTree, &nls[D - i]);
+    bfTreeLevelIterNext(tgtLevelIter);
+    lt[i] = level_indices(tgtLevelIter, tgtTree, &nlt[i]);
+    bad = !ls[D - i] || !lt[i];
+  }
+  BfMatHipButterfly *F = NULL;
+  if (!bad && (nls[0] != 1 || nlt[0] != 1)) bad = 1; /* one source node on top, as makeLastFactor asserts */
+  if (!bad) {
+    F = calloc(1, sizeof(BfMatHipButterfly));
+    F->helm = *helm;
+    F->opts = opts;
+    F->numSrc = srcTree->points->size;
+    F->numTgt = tgtTree->points->size;
+    bfMatInitBase(&F->super, &bfMatHipButterflyVtable, F->numTgt, F->numSrc);
+    bad = bfPlanButterfly(F, srcTree, tgtTree, (const int32_t *const *)ls, nls, (const int32_t *const *)lt, nlt, D);
+  }
+  for (int i = 0; i <= D; ++i) {
+    free(ls[i]);
+    free(lt[i]);
+  }
+  free(ls); free(lt); free(nls); free(nlt);
+  if (bad) {
+    if (F) bfMatHipButterflyDelete((BfMat **)&F);
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  const double t0 = bfWallTime();
+  BfMat *m = factor_from_plan(F, tree_coords(srcTree), tree_coords(tgtTree),
+                              uses_src_normals(helm->layerPot) ? tree_normals(srcTree) : NULL,
+                              uses_tgt_normals(helm->layerPot) ? tree_normals(tgtTree) : NULL, t0, t0);
+  if (!m) return NULL;
+  BfComplex *arena = malloc((size_t)(F->arenaEntries ? F->arenaEntries : 1) * sizeof(BfComplex));
+  BfMatProduct *P = NULL;
+  if (arena && !bfk_d2h(arena, F->dArena, (size_t)F->arenaEntries * sizeof(BfComplex)))
+    P = host_product(F, arena, 0);
+  else
+    bfSetError(arena ? BF_ERROR_RUNTIME_ERROR : BF_ERROR_MEMORY_ERROR);
+  free(arena);
+  bfMatDelete(&m);
+  return P;
+}
+
+BfMat *bfFacHelm2MakeSingleLevel(BfHelm2 const *helm, BfQuadtreeNode const *srcNode, BfQuadtreeNode const *tgtNode) {
+  BfQuadtree const *srcTree = bfQuadtreeNodeGetQuadtreeConst(srcNode);
+  BfQuadtree const *tgtTree = bfQuadtreeNodeGetQuadtreeConst(tgtNode);
+  if (!srcTree || !tgtTree || srcNode->depth != tgtNode->depth) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  BfTreeLevelIter si, ti;
+  const BfSize nf = bfFacHelm2Prepare(helm, srcNode, tgtNode, &si, &ti);
+  BfMatProduct *P = NULL;
+  if (nf == 0)
+    bfSetError(BF_ERROR_RUNTIME_ERROR);
+  else
+    P = bfFacHelm2Make(helm, srcTree, tgtTree, &si, &ti, nf);
+  bfTreeLevelIterDeinit(&si);
+  bfTreeLevelIterDeinit(&ti);
+  return bfMatProductToMat(P);
 }
 
 static int ensure_buffers(BfMatHipButterfly *F, size_t nrhs, size_t elemBytes) {

@@ -201,10 +201,11 @@ BfMatDenseComplex *bfMatToMatDenseComplex(BfMat *mat) {
 /* --------------------------------------------- dense kernel matrix (GPU) */
 
 /* helm2.c:282-319 for the single layer, the double layer and the combined
- * field (the last two with the source normals Nsrc): rows are targets,
- * columns sources.  Sampled on the GPU in row panels of 64 targets.  The
- * target-normal layers are not provided (the reference's S' matrix,
- * helm2.c:128-131, has no body). */
+ * field (the last two with the source normals Nsrc) and the normal
+ * derivative S' of the single layer (with the target normals Ntgt,
+ * helm2.c:127-175): rows are targets, columns sources.  Sampled on the GPU
+ * in row panels of 64 targets.  The hypersingular layer (normal derivative
+ * of D) has no case in the reference either: BF_ERROR_NOT_IMPLEMENTED. */
 BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoints2 const *Xtgt,
                               BfVectors2 const *Nsrc, BfVectors2 const *Ntgt) {
   if (!helm || !Xsrc || !Xtgt) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
@@ -220,11 +221,14 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
     layer.ai = cimag(helm->alpha);
     layer.br = creal(helm->beta);
     layer.bi = cimag(helm->beta);
+  } else if (helm->layerPot == BF_LAYER_POTENTIAL_PV_NORMAL_DERIV_SINGLE) {
+    layer.layer = 3; /* S' with the target normals (helm2.c:127-175) */
   } else if (helm->layerPot != BF_LAYER_POTENTIAL_SINGLE) {
     bfSetError(BF_ERROR_NOT_IMPLEMENTED);
     return NULL;
   }
-  if (layer.layer != 0 && !Nsrc) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
+  if ((layer.layer == 1 || layer.layer == 2) && !Nsrc) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
+  if (layer.layer == 3 && !Ntgt) { bfSetError(BF_ERROR_INVALID_ARGUMENTS); return NULL; }
   BfSize m = Xtgt->size, n = Xsrc->size;
   const BfSize panel = 64;
   BfSize np = (m + panel - 1) / panel;
@@ -239,15 +243,18 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
   }
   void *dJobs = bfk_malloc(np * sizeof(BfkKernelJob) + 16);
   void *dS = bfk_malloc(n * 16 + 16), *dT = bfk_malloc(m * 16 + 16), *dOut = bfk_malloc(m * n * 16 + 16);
-  void *dN = layer.layer != 0 ? bfk_malloc(n * 16 + 16) : NULL;
+  void *dN = layer.layer == 1 || layer.layer == 2 ? bfk_malloc(n * 16 + 16) : NULL;
+  void *dNt = layer.layer == 3 ? bfk_malloc(m * 16 + 16) : NULL;
   BfMatDenseComplex *K = NULL;
   BfComplex *tmp = malloc((m * n > 0 ? m * n : 1) * sizeof(BfComplex));
-  int rc = (!dJobs || !dS || !dT || !dOut || !tmp || (layer.layer != 0 && !dN)) ? -2 : 0;
+  int rc = (!dJobs || !dS || !dT || !dOut || !tmp || ((layer.layer == 1 || layer.layer == 2) && !dN) ||
+            (layer.layer == 3 && !dNt)) ? -2 : 0;
   if (!rc && dN) rc = bfk_h2d(dN, Nsrc->data, n * 16);
+  if (!rc && dNt) rc = bfk_h2d(dNt, Ntgt->data, m * 16);
   if (!rc) rc = bfk_h2d(dJobs, jobs, np * sizeof(BfkKernelJob));
   if (!rc) rc = bfk_h2d(dS, Xsrc->data, n * 16);
   if (!rc) rc = bfk_h2d(dT, Xtgt->data, m * 16);
-  if (!rc) rc = bfk_sample(dJobs, (int64_t)np, dS, dT, (const double *)dN, layer, dOut);
+  if (!rc) rc = bfk_sample(dJobs, (int64_t)np, dS, dT, (const double *)dN, (const double *)dNt, layer, dOut);
   if (!rc) rc = bfk_d2h(tmp, dOut, m * n * 16);
   if (!rc) {
     K = bfMatDenseComplexNew();
@@ -264,6 +271,7 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
   bfk_free(dT);
   bfk_free(dOut);
   bfk_free(dN);
+  bfk_free(dNt);
   free(tmp);
   free(jobs);
   if (rc) {

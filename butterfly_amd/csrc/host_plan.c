@@ -735,3 +735,34 @@ int bfPlanBuildAt(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const 
   tree_view_free(&P.tgt);
   return 0;
 }
+
+/* One butterfly between source node S = ls[0][0] and target node
+ * T = lt[0][0] on the given levels (bfFacHelm2Make, fac_helm2.c:653-704):
+ * ls[j] are the source nodes j levels below S, lt[j] the target nodes j
+ * levels below T, j = 0..D, all as node indices of their trees.  The plan
+ * has this one leaf, at the node's rows and columns of the whole trees,
+ * and uses F's wavenumber and options. */
+int bfPlanButterfly(BfMatHipButterfly *F, BfQuadtree const *src, BfQuadtree const *tgt, const int32_t *const *ls,
+                    const size_t *nls, const int32_t *const *lt, const size_t *nlt, int D) {
+  if (D < 0 || nls[0] != 1 || nlt[0] != 1) return -1;
+  Planner P;
+  memset(&P, 0, sizeof(P));
+  P.F = F;
+  P.k = F->helm.k;
+  P.tol = F->opts.tol;
+  P.C = F->opts.rankConst;
+  tree_view(&P.src, src);
+  tree_view(&P.tgt, tgt);
+  Level *Ls = calloc((size_t)D + 1, sizeof(Level)), *Lt = calloc((size_t)D + 1, sizeof(Level));
+  for (int j = 0; j <= D; ++j) {
+    for (size_t i = 0; i < nls[j]; ++i) BF_VEC_PUSH(Ls[j], ls[j][i]);
+    for (size_t i = 0; i < nlt[j]; ++i) BF_VEC_PUSH(Lt[j], lt[j][i]);
+  }
+  BfHipLeaf *l = new_leaf(F, &P, Ls[0].a[0], Lt[0].a[0], 1);
+  make_butterfly(&P, l, Ls, Lt, D, 0);
+  free_levels(Ls, D + 1);
+  free_levels(Lt, D + 1);
+  tree_view_free(&P.src);
+  tree_view_free(&P.tgt);
+  return 0;
+}

@@ -1507,6 +1507,102 @@ BfMat *bfFacGetMat(BfFac const *fac, BfPolicy policy) {
   return m;
 }
 
+/* ------------------------------------- the product as host containers */
+
+static int cmp_i64(const void *a, const void *b) {
+  const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return (x > y) - (x < y);
+}
+
+/* sorted distinct values of v[0..n) in place; returns their count */
+static size_t unique_i64(int64_t *v, size_t n) {
+  qsort(v, n, sizeof(int64_t), cmp_i64);
+  size_t k = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (k == 0 || v[i] != v[k - 1]) v[k++] = v[i];
+  return k;
+}
+
+static size_t find_i64(const int64_t *v, size_t n, int64_t x) {
+  size_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const size_t mid = (lo + hi) / 2;
+    if (v[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+/* one factor as a BfMatBlockCoo on the grid of its blocks' row and column
+ * boundaries: a block over several grid cells becomes one dense block per
+ * cell (identity blocks as dense ones); blocks on one cell are summed by
+ * the container's apply, as overlapping blocks are by the factor's */
+static BfMat *smat_host(const SMat *s) {
+  int64_t *R = malloc((2 * s->nb + 2) * sizeof(int64_t)), *Cb = malloc((2 * s->nb + 2) * sizeof(int64_t));
+  size_t nr = 0, nc = 0;
+  R[nr++] = 0;
+  R[nr++] = s->rows;
+  Cb[nc++] = 0;
+  Cb[nc++] = s->cols;
+  for (size_t i = 0; i < s->nb; ++i) {
+    const SBlock *b = &s->b[i];
+    if (b->m == 0 || b->n == 0) continue;
+    R[nr++] = b->i0;
+    R[nr++] = b->i0 + b->m;
+    Cb[nc++] = b->j0;
+    Cb[nc++] = b->j0 + b->n;
+  }
+  nr = unique_i64(R, nr);
+  nc = unique_i64(Cb, nc);
+  size_t cells = 0;
+  for (size_t i = 0; i < s->nb; ++i) {
+    const SBlock *b = &s->b[i];
+    if (b->m == 0 || b->n == 0) continue;
+    cells += (find_i64(R, nr, b->i0 + b->m) - find_i64(R, nr, b->i0)) *
+             (find_i64(Cb, nc, b->j0 + b->n) - find_i64(Cb, nc, b->j0));
+  }
+  BfMatBlockCoo *c = bfMatBlockCooNew();
+  bfMatBlockCooInit(c, nr - 1, nc - 1, cells);
+  for (size_t i = 0; i < nr; ++i) c->super.rowOffset[i] = (BfSize)R[i];
+  for (size_t j = 0; j < nc; ++j) c->super.colOffset[j] = (BfSize)Cb[j];
+  size_t k = 0;
+  for (size_t i = 0; i < s->nb; ++i) {
+    const SBlock *b = &s->b[i];
+    if (b->m == 0 || b->n == 0) continue;
+    const size_t r0 = find_i64(R, nr, b->i0), r1 = find_i64(R, nr, b->i0 + b->m);
+    const size_t c0 = find_i64(Cb, nc, b->j0), c1 = find_i64(Cb, nc, b->j0 + b->n);
+    for (size_t ri = r0; ri < r1; ++ri)
+      for (size_t ci = c0; ci < c1; ++ci, ++k) {
+        const int64_t m = R[ri + 1] - R[ri], n = Cb[ci + 1] - Cb[ci];
+        const int64_t di = R[ri] - b->i0, dj = Cb[ci] - b->j0;
+        BfMatDenseComplex *D = bfMatDenseComplexNew();
+        bfMatDenseComplexInit(D, (BfSize)m, (BfSize)n);
+        for (int64_t r = 0; r < m; ++r)
+          for (int64_t q = 0; q < n; ++q)
+            D->data[r * n + q] = b->data ? b->data[(dj + q) * b->m + di + r] : (di + r == dj + q ? 1.0 : 0.0);
+        c->rowInd[k] = ri;
+        c->colInd[k] = ci;
+        c->super.block[k] = bfMatDenseComplexToMat(D);
+        c->ownsBlock[k] = true;
+      }
+  }
+  free(R);
+  free(Cb);
+  return bfMatBlockCooToMat(c);
+}
+
+BfMatProduct *bfFacGetMatProduct(BfFac const *fac, BfPolicy policy) {
+  (void)policy;
+  if (!fac) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return NULL;
+  }
+  BfMatProduct *P = bfMatProductNew();
+  bfMatProductInit(P);
+  bfMatProductPostMultiply(P, smat_host(&fac->Psi));
+  for (size_t i = 0; i < fac->numW; ++i) bfMatProductPostMultiply(P, smat_host(&fac->W[i]));
+  return P;
+}
+
 /* the partial factorizations side by side: [F_1 F_2 ...] as one chain,
  * [Psi_1 Psi_2 ...] then block diagonals of the W factors, shorter
  * chains padded with identities (fac_span.c:128-157) */

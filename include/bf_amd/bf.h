@@ -58,8 +58,9 @@ enum BfError {
   BF_ERROR_INCOMPATIBLE_SHAPES,
 };
 
-/* types.h:78-144 (matrix and tree types this path uses; the device
- * factorization is appended as a new type) */
+/* types.h:78-144 with the reference's values (the types this path uses,
+ * BF_TYPE_COUNT and BF_TYPE_NONE); the device types come after
+ * BF_TYPE_NONE, so no reference value changes meaning */
 typedef enum BfTypes {
   BF_TYPE_MAT = 0,
   BF_TYPE_MAT_IDENTITY = 8,
@@ -70,10 +71,11 @@ typedef enum BfTypes {
   BF_TYPE_MAT_DENSE_COMPLEX = 19,
   BF_TYPE_QUADTREE = 34,
   BF_TYPE_QUADTREE_NODE = 39,
-  BF_TYPE_MAT_HIP_BUTTERFLY = 42,
-  BF_TYPE_MAT_HIP_PRODUCT = 43, /* a streamed factorization on the device */
-  BF_TYPE_MAT_HIP_BIE = 44,     /* a boundary integral operator on the device */
-  BF_TYPE_NONE = 1000
+  BF_TYPE_COUNT = 42,
+  BF_TYPE_NONE = 43,
+  BF_TYPE_MAT_HIP_BUTTERFLY = 44, /* a Helmholtz butterfly factorization on the device */
+  BF_TYPE_MAT_HIP_PRODUCT = 45,   /* a streamed factorization on the device */
+  BF_TYPE_MAT_HIP_BIE = 46,       /* a boundary integral operator on the device */
 } BfType;
 
 /* layer_pot.h:28-69 */
@@ -138,6 +140,11 @@ typedef struct BfQuadtreeNode {
   int32_t maxDepthBelow; /* depth of the deepest leaf in this subtree */
   int32_t index;         /* position among the parent's children (0..3) */
 } BfQuadtreeNode;
+
+/* tree_node.h: the tree-generic node is the quadtree node here (the only
+ * tree this path builds), so BfTreeNode pointers are BfQuadtreeNode
+ * pointers */
+typedef BfQuadtreeNode BfTreeNode;
 
 /* tree.h:30-39 */
 typedef struct BfTree {
@@ -228,6 +235,11 @@ void bfQuadtreeDealloc(BfQuadtree **tree);
 void bfQuadtreeDeinitAndDealloc(BfQuadtree **tree);
 BfTree *bfQuadtreeToTree(BfQuadtree *tree);
 BfPerm const *bfTreeGetPermConst(BfTree const *tree);
+/* quadtree_node.h:52: the tree a node belongs to (NULL and
+ * BF_ERROR_INVALID_ARGUMENTS for a node of no live tree) */
+BfQuadtree const *bfQuadtreeNodeGetQuadtreeConst(BfQuadtreeNode const *node);
+/* quadtree_node.h: the node as a tree node (the same pointer here) */
+BfTreeNode *bfQuadtreeNodeToTreeNode(BfQuadtreeNode *node);
 /* quadtree_node.c:321-330, :393-401 */
 BfCircle bfQuadtreeNodeGetBoundingCircle(BfQuadtreeNode const *node);
 bool bfQuadtreeNodesAreSeparated(BfQuadtreeNode const *node1, BfQuadtreeNode const *node2);
@@ -237,11 +249,12 @@ BfSize bfQuadtreeNodeGetNumPoints(BfQuadtreeNode const *node);
 BfSize bfHelm2RankEstForTwoCircles(BfHelm2 const *helm, BfCircle const *circ1,
                                    BfCircle const *circ2, BfReal C, BfReal eps);
 /* helm2.c:282-319: dense kernel matrix, rows = targets, cols = sources, for
- * the single layer, the double layer (PV_DOUBLE, source normals Nsrc) and
- * the combined field alpha S + beta D (helm2.c:38-90).  Sampled on the GPU
- * by the sampling kernel; returns a host matrix.  The target-normal layers
- * return BF_ERROR_NOT_IMPLEMENTED (the reference's S' matrix,
- * helm2.c:128-131, has no body). */
+ * the single layer, the double layer (PV_DOUBLE, source normals Nsrc), the
+ * combined field alpha S + beta D (helm2.c:38-90) and the normal derivative
+ * of the single layer S' (PV_NORMAL_DERIV_SINGLE, target normals Ntgt,
+ * helm2.c:127-175).  Sampled on the GPU by the sampling kernel; returns a
+ * host matrix.  The hypersingular layer (PV_NORMAL_DERIV_DOUBLE), for which
+ * the reference has no case either, returns BF_ERROR_NOT_IMPLEMENTED. */
 BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoints2 const *Xtgt,
                               BfVectors2 const *Nsrc, BfVectors2 const *Ntgt);
 
@@ -253,10 +266,77 @@ BfMat *bfHelm2GetKernelMatrix(BfHelm2 const *helm, BfPoints2 const *Xsrc, BfPoin
  * bfMatMul.  Layer potentials: single, double (PV_DOUBLE) and combined
  * field, the last two with the unit normals of `srcTree` (its proxies use
  * the same layer with the circles' radial normals, layer_pot.h:63-69,
- * fac_helm2.c:83-118); the target-normal layers have no re-expansion
- * (helm2.c:341-342) and return BF_ERROR_NOT_IMPLEMENTED. */
+ * fac_helm2.c:83-118), and S' (PV_NORMAL_DERIV_SINGLE) with the unit normals
+ * of `tgtTree`: its re-expansions use the single layer (BF_PROXY_LAYER_POT,
+ * fac_helm2.c:58-59, :230-231) and the target normals enter only the kernel
+ * blocks of the last factors and the dense leaves (:455-465, :755-771).
+ * PV_NORMAL_DERIV_DOUBLE returns BF_ERROR_NOT_IMPLEMENTED. */
 BfMat *bfFacHelm2MakeMultilevel(BfHelm2 const *helm, BfQuadtree const *srcTree,
                                 BfQuadtree const *tgtTree);
+
+/* tree_traversals.h */
+typedef enum BfTreeTraversals {
+  BF_TREE_TRAVERSAL_UNKNOWN,
+  BF_TREE_TRAVERSAL_LR_LEVEL_ORDER,
+  BF_TREE_TRAVERSAL_LR_REVERSE_LEVEL_ORDER,
+  BF_TREE_TRAVERSAL_POST_ORDER
+} BfTreeTraversal;
+
+/* tree_level_iter.h:7-12: the nodes of the subtree under a node, level by
+ * level; `levelNodes` holds the current level's BfQuadtreeNode pointers */
+typedef struct BfPtrArray BfPtrArray;
+typedef struct BfTreeLevelIter {
+  BfTreeTraversal traversal;
+  struct BfPtrArray {
+    void **data;
+    BfSize capacity;
+    BfSize num_elts;
+    bool isView;
+  } nodes;
+  BfPtrArray *levelNodes;
+  void *aux;
+} BfTreeLevelIter;
+/* tree_level_iter.c:184-330 (LR_LEVEL_ORDER goes down from the node,
+ * LR_REVERSE_LEVEL_ORDER up from the deepest level) */
+void bfTreeLevelIterInit(BfTreeLevelIter *iter, BfTreeTraversal traversal, BfTreeNode *node);
+void bfTreeLevelIterDeinit(BfTreeLevelIter *iter);
+BfSize bfTreeLevelIterCurrentDepth(BfTreeLevelIter const *iter);
+bool bfTreeLevelIterIsDone(BfTreeLevelIter const *iter);
+void bfTreeLevelIterNext(BfTreeLevelIter *iter);
+BfSize bfTreeLevelIterGetNumPoints(BfTreeLevelIter const *iter);
+bool bfTreeLevelIterCurrentLevelIsInternal(BfTreeLevelIter const *iter);
+BfPtrArray *bfTreeLevelIterGetLevelNodes(BfTreeLevelIter *iter);
+
+typedef struct BfMatProduct BfMatProduct;
+
+/* fac_helm2.c:551-651: picks the source level of a butterfly between
+ * srcNode and tgtNode with the reference's rules (the deepest fully
+ * internal target level caps the source depth; the source level must hold
+ * all of srcNode's points, be internal, and every node's rank estimate at
+ * 1e-15 must not exceed its points).  Initializes srcLevelIter (reverse
+ * level order, left on the chosen level) and tgtLevelIter (level order, on
+ * tgtNode) and returns the number of factors, or 0 when the pair cannot be
+ * butterflied; the caller deinitializes the iterators. */
+BfSize bfFacHelm2Prepare(BfHelm2 const *helm, BfQuadtreeNode const *srcNode, BfQuadtreeNode const *tgtNode,
+                         BfTreeLevelIter *srcLevelIter, BfTreeLevelIter *tgtLevelIter);
+/* fac_helm2.c:653-704: builds the numFactors factors from the iterators
+ * prepared by bfFacHelm2Prepare (advancing them as the reference does):
+ * the sampling and re-expansion blocks are computed by the HIP kernels,
+ * then copied into host containers of the reference's shape -- a
+ * BfMatProduct whose factors, applied last first, are the evaluation
+ * factor (BfMatBlockDiag of target-points x proxies kernel blocks), the
+ * re-expansion factors (BfMatBlockCoo) and the first factor
+ * (BfMatBlockDiag), every block a BfMatDenseComplex.  The product maps the
+ * source node's points (tree order) to the target node's points.  srcNode
+ * and tgtNode must be at one depth, as in the reference's multilevel
+ * split.  NULL with BF_ERROR_INVALID_ARGUMENTS when numFactors < 2. */
+BfMatProduct *bfFacHelm2Make(BfHelm2 const *helm, BfQuadtree const *srcTree, BfQuadtree const *tgtTree,
+                             BfTreeLevelIter *srcLevelIter, BfTreeLevelIter *tgtLevelIter, BfSize numFactors);
+/* fac_helm2.c:706-729: Prepare + Make for one pair of nodes (same depth);
+ * the product as a BfMat (BF_TYPE_MAT_PRODUCT).  NULL with
+ * BF_ERROR_RUNTIME_ERROR when the pair cannot be butterflied (the reference
+ * has no result there either). */
+BfMat *bfFacHelm2MakeSingleLevel(BfHelm2 const *helm, BfQuadtreeNode const *srcNode, BfQuadtreeNode const *tgtNode);
 
 /* new: planner options beside the reference rules (no counterpart: the
  * reference hard-codes them, fac_helm2.c:20, :102, :291-296, :523) */
@@ -434,13 +514,7 @@ typedef enum BfPolicies {
 
 /* ------------------------------------------- the block matrix algebra */
 
-/* ptr_array.h:7-40 */
-typedef struct BfPtrArray {
-  void **data;
-  BfSize capacity;
-  BfSize num_elts;
-  bool isView;
-} BfPtrArray;
+/* ptr_array.h:7-40 (the struct is declared with BfTreeLevelIter above) */
 void bfInitPtrArray(BfPtrArray *arr, BfSize capacity);
 void bfInitPtrArrayWithDefaultCapacity(BfPtrArray *arr);
 void bfPtrArrayDeinit(BfPtrArray *arr);
@@ -493,10 +567,10 @@ typedef struct BfMatBlockDense {
 
 /* mat_product.h:28-31: the product F_0 F_1 ... F_{n-1}; applied last
  * factor first */
-typedef struct BfMatProduct {
+struct BfMatProduct {
   BfMat super;
   BfPtrArray factorArr;
-} BfMatProduct;
+};
 
 /* The containers are host objects; bfMatMul on one whose leaves are dense
  * complex matrices (or containers of them) packs the leaves into a device
@@ -623,6 +697,12 @@ BfPerm const *bfFacStreamerGetRowTreeReversePerm(BfFacStreamer const *facStreame
  * (BF_TYPE_MAT_HIP_PRODUCT, applied with bfMatMul; every policy copies) */
 void bfFacDeinitAndDealloc(BfFac **fac);
 BfMat *bfFacGetMat(BfFac const *fac, BfPolicy policy);
+/* fac.c:53-75: the same product as host containers: a BfMatProduct of
+ * Psi, W0, ..., W_L, each a BfMatBlockCoo of BfMatDenseComplex blocks on
+ * the grid of its blocks' row and column boundaries (a block that spans
+ * several grid cells is cut into one block per cell); every policy
+ * copies.  bfMatMul on it packs the blocks into a device product. */
+BfMatProduct *bfFacGetMatProduct(BfFac const *fac, BfPolicy policy);
 BfSize bfFacGetNumBytes(BfFac const *fac);
 /* fac_span.h: the partial factorizations side by side; the span does not
  * own them (they stay with the streamer) */
@@ -644,7 +724,10 @@ int bfFacGetFactorDense(BfFac const *fac, BfSize which, BfComplex *out);
  * corrections of order krOrder (0, 2 or 6; quadrature.c:12-200, applied
  * before the weights as helm2_bie.c:98-103 does) along the boundary order
  * of `points`; `perm` is the tree permutation of fac's quadtree.  Vectors
- * are in tree order. */
+ * are in tree order.  The corrections are values of fac's own layer:
+ * G for S, alpha G for the combined field, S' with the row point's normal
+ * (taken from fac's target tree) for S' (krComplexKernel, helm2.c:372-378);
+ * the double layer takes none (krOrder must be 0). */
 BfMat *bfhipHelm2BieNew(BfMat *fac, BfHelm2 const *helm, BfPoints2 const *points, BfReal const *weights,
                         BfPerm const *perm, BfSize krOrder, BfReal alpha);
 /* new: y = A x on device vectors (n x nrhs, FP64 or FP32; FP32 needs

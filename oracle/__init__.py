@@ -44,8 +44,10 @@ def lib():
         "orc_fac_make": (vp, [d, dp, sz, dp, sz, sz, d, sz, C.c_int]),
         "orc_set_plan_only": (None, [C.c_int]),
         "orc_h1": (None, [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-        "orc_dense_rows_layer": (None, [C.c_double, C.c_int, dp, dp, C.c_size_t, dp, C.c_size_t, dp, dp, dp, dp]),
-        "orc_kernel_matrix_layer": (None, [C.c_double, C.c_int, dp, dp, C.c_size_t, dp, C.c_size_t, dp, dp, dp]),
+        "orc_dense_rows_layer": (None, [C.c_double, C.c_int, dp, dp, C.c_size_t, dp, C.c_size_t, dp, dp, dp, dp,
+                                        dp]),
+        "orc_kernel_matrix_layer": (None, [C.c_double, C.c_int, dp, dp, C.c_size_t, dp, C.c_size_t, dp, dp, dp,
+                                           dp]),
         "orc_set_rank_const": (None, [d]),
         "orc_set_pinv_rcond": (None, [d]),
         "orc_set_impedance": (None, [C.c_int]),
@@ -144,33 +146,44 @@ def tree(points, leaf_size):
 
 
 def _layer_args(layer, alpha, beta):
-    lay = {"single": 0, "double": 1, "combined": 2}[layer]
+    lay = {"single": 0, "double": 1, "combined": 2, "sprime": 3}[layer]
     a = np.array([complex(alpha).real, complex(alpha).imag])
     b = np.array([complex(beta).real, complex(beta).imag])
     return lay, a, b
 
 
-def dense_rows_layer(k, tgt, src, src_normals, x, layer="double", alpha=1.0, beta=1.0):
-    """Rows of K x for the double layer or the combined field alpha S + beta D
-    (helm2.c:38-90), Neumaier-summed."""
+def _normals(nrm):
+    if nrm is None:
+        return None, None
+    nrm = np.ascontiguousarray(nrm, np.float64)
+    return nrm, _dp(nrm)
+
+
+def dense_rows_layer(k, tgt, src, src_normals, x, layer="double", alpha=1.0, beta=1.0, tgt_normals=None):
+    """Rows of K x for the double layer, the combined field alpha S + beta D
+    (source normals) or S' (target normals), helm2.c:38-90, Neumaier-summed."""
     tgt = np.ascontiguousarray(tgt, np.float64)
     src = np.ascontiguousarray(src, np.float64)
-    nrm = np.ascontiguousarray(src_normals, np.float64)
+    sn, psn = _normals(src_normals)
+    tn, ptn = _normals(tgt_normals)
     x = np.ascontiguousarray(x, np.complex128)
     lay, a, b = _layer_args(layer, alpha, beta)
+    assert (lay in (1, 2)) <= (sn is not None) and (lay == 3) <= (tn is not None)
     out = np.zeros(2 * len(tgt))
-    lib().orc_dense_rows_layer(k, lay, _dp(a), _dp(b), len(tgt), _dp(tgt), len(src), _dp(src), _dp(nrm),
+    lib().orc_dense_rows_layer(k, lay, _dp(a), _dp(b), len(tgt), _dp(tgt), len(src), _dp(src), psn, ptn,
                                _dp(x.view(np.float64)), _dp(out))
     return out.view(np.complex128)
 
 
-def kernel_matrix_layer(k, tgt, src, src_normals, layer="double", alpha=1.0, beta=1.0):
+def kernel_matrix_layer(k, tgt, src, src_normals, layer="double", alpha=1.0, beta=1.0, tgt_normals=None):
     tgt = np.ascontiguousarray(tgt, np.float64)
     src = np.ascontiguousarray(src, np.float64)
-    nrm = np.ascontiguousarray(src_normals, np.float64)
+    sn, psn = _normals(src_normals)
+    tn, ptn = _normals(tgt_normals)
     lay, a, b = _layer_args(layer, alpha, beta)
+    assert (lay in (1, 2)) <= (sn is not None) and (lay == 3) <= (tn is not None)
     out = np.zeros(2 * len(tgt) * len(src))
-    lib().orc_kernel_matrix_layer(k, lay, _dp(a), _dp(b), len(tgt), _dp(tgt), len(src), _dp(src), _dp(nrm),
+    lib().orc_kernel_matrix_layer(k, lay, _dp(a), _dp(b), len(tgt), _dp(tgt), len(src), _dp(src), psn, ptn,
                                   _dp(out))
     return out.view(np.complex128).reshape(len(tgt), len(src))
 

@@ -1137,7 +1137,7 @@ void orc_dense_rows(double k, size_t nr, const double *tgt, size_t ns, const dou
   }
 }
 
-/* ------------------------------------------ double layer, combined field */
+/* ------------------------------------ double layer, combined field, S' */
 
 /* H1^(1)(x) = j1 + i y1 from glibc (the reference's bf_H1, bessel.c) */
 void orc_h1(double x, double *re, double *im) {
@@ -1146,13 +1146,19 @@ void orc_h1(double x, double *re, double *im) {
 }
 
 /* kernel value of layer `layer` (0 single, 1 double, 2 combined field
- * alpha S + beta D) for source s with unit normal ns and target t
+ * alpha S + beta D, 3 the normal derivative S' of the single layer) for
+ * source s with unit normal ns and target t with unit normal nt
  * (helm2.c:38-90): S = (i/4) H0(k r) (0 at r = 0); D = (i/4) k H1(k r)
- * ns.(t - s)/r (0 for r < 1e-15) */
+ * ns.(t - s)/r and S' = (i/4) k H1(k r) nt.(t - s)/r (0 for r < 1e-15,
+ * helm2.c:45-53) */
 static cplx layer_value(double k, int layer, cplx alpha, cplx beta, const double *t, const double *s,
-                        const double *ns) {
+                        const double *ns, const double *nt) {
   const double dx = t[0] - s[0], dy = t[1] - s[1];
   const double r = hypot(dx, dy);
+  if (layer == 3) {
+    if (r < 1e-15) return 0;
+    return (I / 4) * k * (j1(k * r) + I * y1(k * r)) * (nt[0] * dx + nt[1] * dy) / r;
+  }
   cplx S = 0, D = 0;
   if (layer != 1) S = r == 0 ? 0 : I * (j0(k * r) + I * y0(k * r)) / 4;
   if (layer != 0 && r >= 1e-15) D = (I / 4) * k * (j1(k * r) + I * y1(k * r)) * (ns[0] * dx + ns[1] * dy) / r;
@@ -1161,16 +1167,18 @@ static cplx layer_value(double k, int layer, cplx alpha, cplx beta, const double
   return alpha * S + beta * D;
 }
 
-/* rows of K x for the layer kernels (Neumaier-summed); srcN: unit normals */
+/* rows of K x for the layer kernels (Neumaier-summed); srcN / tgtN: unit
+ * normals of the sources (D, combined) / targets (S'), NULL if unused */
 void orc_dense_rows_layer(double k, int layer, const double *alpha, const double *beta, size_t nr,
-                          const double *tgt, size_t ns, const double *src, const double *srcN, const double *Xd,
-                          double *Yd) {
+                          const double *tgt, size_t ns, const double *src, const double *srcN, const double *tgtN,
+                          const double *Xd, double *Yd) {
   const cplx *X = (const cplx *)Xd;
   const cplx a = alpha[0] + I * alpha[1], b = beta[0] + I * beta[1];
   for (size_t i = 0; i < nr; ++i) {
     double sr = 0, cr = 0, si = 0, ci = 0;
     for (size_t j = 0; j < ns; ++j) {
-      cplx v = layer_value(k, layer, a, b, &tgt[2 * i], &src[2 * j], &srcN[2 * j]) * X[j];
+      cplx v = layer_value(k, layer, a, b, &tgt[2 * i], &src[2 * j], srcN ? &srcN[2 * j] : NULL,
+                           tgtN ? &tgtN[2 * i] : NULL) * X[j];
       double p = creal(v), q = cimag(v);
       double t = sr + p; cr += fabs(sr) >= fabs(p) ? (sr - t) + p : (p - t) + sr; sr = t;
       t = si + q; ci += fabs(si) >= fabs(q) ? (si - t) + q : (q - t) + si; si = t;
@@ -1182,11 +1190,14 @@ void orc_dense_rows_layer(double k, int layer, const double *alpha, const double
 
 /* the m x n matrix of the layer kernel (row-major complex) */
 void orc_kernel_matrix_layer(double k, int layer, const double *alpha, const double *beta, size_t m,
-                             const double *tgt, size_t n, const double *src, const double *srcN, double *out) {
+                             const double *tgt, size_t n, const double *src, const double *srcN,
+                             const double *tgtN, double *out) {
   const cplx a = alpha[0] + I * alpha[1], b = beta[0] + I * beta[1];
   cplx *Z = (cplx *)out;
   for (size_t i = 0; i < m; ++i)
-    for (size_t j = 0; j < n; ++j) Z[i * n + j] = layer_value(k, layer, a, b, &tgt[2 * i], &src[2 * j], &srcN[2 * j]);
+    for (size_t j = 0; j < n; ++j)
+      Z[i * n + j] = layer_value(k, layer, a, b, &tgt[2 * i], &src[2 * j], srcN ? &srcN[2 * j] : NULL,
+                                 tgtN ? &tgtN[2 * i] : NULL);
 }
 
 /* tree export for parity: per node (BFS order from the root):

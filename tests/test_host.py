@@ -278,3 +278,107 @@ def test_rank_features_closed_form():
     dc, dr = np.sqrt(2), np.sqrt(8)
     want = [np.log(dc), np.log(dr), 10 * dr, np.hypot(2.5, 0.5) / dr, 3, -8]
     assert np.allclose(f, want, rtol=1e-6)
+
+
+def _iter(L, tree, node, traversal):
+    it = bf.TreeLevelIterStruct()
+    L.bfTreeLevelIterInit(C.byref(it), traversal, tree.node_ptr(node))
+    return it
+
+
+def _level_nodes(it, tree):
+    lv = it.levelNodes.contents
+    base = bf.QuadtreeStruct.from_address(tree.ptr).nodes
+    size = C.sizeof(bf.QuadtreeNodeStruct)
+    return [(lv.data[i] - base) // size for i in range(lv.num_elts)]
+
+
+def test_tree_level_iterator_walks_levels_both_ways():
+    """bfTreeLevelIter (tree_level_iter.c:10-330) over a subtree: level order
+    from the node down (IsDone after the deepest level, whose view stays),
+    reverse level order from the deepest level up (the view stays on the
+    node's level); the levels are the subtree's nodes by depth, children in
+    quadrant order; the point counts and the internal test per level."""
+    L = bf.lib()
+    src, _, _ = box_problem(3000)
+    t = bf.Quadtree(src, 1)
+    root = 5  # a depth-2 node
+    nodes, depth = [root], {root: 0}
+    for v in nodes:
+        for c in t.node(v).child:
+            if c >= 0:
+                nodes.append(c)
+                depth[c] = depth[v] + 1
+    D = max(depth.values())
+    levels = [[v for v in nodes if depth[v] == d] for d in range(D + 1)]
+    it = _iter(L, t, root, 1)
+    for d in range(D + 1):
+        assert not L.bfTreeLevelIterIsDone(C.byref(it))
+        assert L.bfTreeLevelIterCurrentDepth(C.byref(it)) == t.node(root).depth + d
+        assert _level_nodes(it, t) == levels[d]
+        npts = sum(t.node(v).offset[4] - t.node(v).offset[0] for v in levels[d])
+        assert L.bfTreeLevelIterGetNumPoints(C.byref(it)) == npts
+        internal = all(any(c >= 0 for c in t.node(v).child) for v in levels[d])
+        assert L.bfTreeLevelIterCurrentLevelIsInternal(C.byref(it)) == internal
+        L.bfTreeLevelIterNext(C.byref(it))
+    assert L.bfTreeLevelIterIsDone(C.byref(it)) and _level_nodes(it, t) == levels[D]
+    L.bfTreeLevelIterDeinit(C.byref(it))
+    it = _iter(L, t, root, 2)
+    for d in range(D, -1, -1):
+        assert _level_nodes(it, t) == levels[d]
+        L.bfTreeLevelIterNext(C.byref(it))
+    assert _level_nodes(it, t) == levels[0]
+    L.bfTreeLevelIterDeinit(C.byref(it))
+    assert L.bfQuadtreeNodeGetQuadtreeConst(t.node_ptr(root)) == t.ptr
+
+
+def test_prepare_gives_the_planners_factor_counts():
+    """bfFacHelm2Prepare (fac_helm2.c:551-651) on every separated pair of the
+    curve -> circle plan at N = 2^14 under the reference rules returns the
+    factor count of the planner's leaf for that pair (the planner is pinned
+    by the census above), and leaves the source iterator on the level the
+    first factor uses."""
+    L = bf.lib()
+    src, tgt, k = curve_problem(1 << 14)
+    ts, tt = bf.Quadtree(src, 1), bf.Quadtree(tgt, 1)
+    helm = bf.Helm2(k)
+    lt, _, _ = bf.FacHelm2.plan_only(helm, ts, tt).plan()
+
+    def index(tree):
+        out = {}
+        for i in range(tree.num_nodes):
+            nd = tree.node(i)
+            out.setdefault((nd.offset[0], nd.offset[4] - nd.offset[0], nd.depth), i)
+        return out
+
+    si, ti = index(ts), index(tt)
+    checked = 0
+    for leaf in lt[lt[:, 0] == 1]:
+        _, row0, col0, m, n, nf = (int(v) for v in leaf)
+        for d in range(2, 30):
+            if (col0, n, d) in si and (row0, m, d) in ti:
+                s, t = si[(col0, n, d)], ti[(row0, m, d)]
+                break
+        a, b = bf.TreeLevelIterStruct(), bf.TreeLevelIterStruct()
+        got = L.bfFacHelm2Prepare(C.byref(helm.s), ts.node_ptr(s), tt.node_ptr(t), C.byref(a), C.byref(b))
+        assert got == nf, (s, t, got, nf)
+        assert L.bfTreeLevelIterCurrentDepth(C.byref(a)) == ts.node(s).depth + nf - 2
+        assert L.bfTreeLevelIterCurrentDepth(C.byref(b)) == tt.node(t).depth
+        L.bfTreeLevelIterDeinit(C.byref(a))
+        L.bfTreeLevelIterDeinit(C.byref(b))
+        checked += 1
+    assert checked == int(lt[:, 0].sum()) and checked > 50
+
+
+def test_c_caller_compiles_against_the_header(tmp_path):
+    """A C11 program written against include/bf_amd/bf.h alone (the reference
+    drivers' calls: quadtree, multilevel and single-level factorization,
+    bfMatMul, the containers' struct fields) compiles warning-free and
+    links against the library; the GPU test runs it."""
+    exe = tmp_path / "bf_caller"
+    r = subprocess.run(["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "c_caller", "bf_caller.c"), "-o", str(exe),
+                        "-L", os.path.join(ROOT, "butterfly_amd"), "-lbutterfly_amd",
+                        "-Wl,-rpath," + os.path.join(ROOT, "butterfly_amd"), "-lm"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -152,6 +152,35 @@ def test_double_layer_is_the_source_normal_derivative():
     assert np.abs(y - CF @ x).max() < 1e-13 * np.abs(y).max()
 
 
+def test_sprime_is_minus_the_target_normal_derivative():
+    """S'(t, s) = (i/4) k H1(k r) n_t.(t - s)/r (helm2.c:45-53, :127-175).
+    Since d/dx H0 = -H1, this is -n_t . grad_t G(t, s): a central difference
+    of the oracle's single-layer entries along the target normal, with the
+    sign the reference's formula carries (a sign slip, or the source normal
+    used in place of the target's, fails it); r = 0 gives 0."""
+    rng = np.random.default_rng(6)
+    k = 9.0
+    src = rng.uniform(-1, 1, (35, 2))
+    tgt = rng.uniform(2, 3, (25, 2))
+    ang = rng.uniform(0, 2 * np.pi, 25)
+    tn = np.stack([np.cos(ang), np.sin(ang)], 1)
+    sn = np.stack([np.cos(ang[:1] + 1.0), np.sin(ang[:1] + 1.0)], 1).repeat(35, 0)
+    h = 1e-5
+    fd = (orc.kernel_matrix(k, tgt + h * tn, src) - orc.kernel_matrix(k, tgt - h * tn, src)) / (2 * h)
+    Sp = orc.kernel_matrix_layer(k, tgt, src, None, "sprime", tgt_normals=tn)
+    assert np.abs(Sp + fd).max() < 1e-7 * np.abs(Sp).max()
+    # the source normals play no part in S'
+    assert np.array_equal(Sp, orc.kernel_matrix_layer(k, tgt, src, sn, "sprime", tgt_normals=tn))
+    # S'(t, s) with n_t is D(s, t) with n_s := n_t and the roles swapped, up to the sign of t - s
+    D = orc.kernel_matrix_layer(k, src, tgt, tn, "double")
+    assert np.abs(Sp + D.T).max() < 1e-15 * np.abs(Sp).max()
+    Z = orc.kernel_matrix_layer(k, tgt[:4], tgt[:4], None, "sprime", tgt_normals=tn[:4])
+    assert np.all(np.diag(Z) == 0)
+    x = rng.standard_normal(35) + 1j * rng.standard_normal(35)
+    y = orc.dense_rows_layer(k, tgt, src, None, x, "sprime", tgt_normals=tn)
+    assert np.abs(y - Sp @ x).max() < 1e-13 * np.abs(y).max()
+
+
 def test_kernel_matrix_closed_form_and_zero_distance():
     k = 7.5
     tgt = np.array([[0.0, 0.0], [1.0, 2.0], [0.5, 0.5]])
