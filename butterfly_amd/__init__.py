@@ -178,6 +178,7 @@ def lib():
         "bfhipDeviceCount": (C.c_int, []),
         "bfhipDeviceCopy": (C.c_int, [vp, vp, sz]),
         "bfhipMatMulDevice": (C.c_int, [vp, vp, vp, sz, C.c_int]),
+        "bfhipMatMulDeviceNoAlloc": (C.c_int, [vp, vp, vp, sz, C.c_int]),
         "bfhipFacPrepareMixed": (C.c_int, [vp, C.c_int]),
         "bfhipFacReleaseFp64": (C.c_int, [vp]),
         "bfhipFacOrthogonalize": (C.c_int, [vp]),
@@ -892,6 +893,15 @@ class DeviceArray:
 def mul_device(fac, dx, dy, nrhs, precision="fp64"):
     """bfhipMatMulDevice: Y = A X on resident device buffers."""
     rc = lib().bfhipMatMulDevice(fac.ptr, dx.ptr, dy.ptr, nrhs, PRECISION[precision])
+    if rc:
+        check()
+        raise BfError(f"apply failed ({rc})")
+
+
+def mul_device_no_alloc(fac, dx, dy, nrhs, precision="fp64"):
+    """bfhipMatMulDeviceNoAlloc: the same apply with the intermediate vectors
+    in the library's fixed device scratch (no allocation)."""
+    rc = lib().bfhipMatMulDeviceNoAlloc(fac.ptr, dx.ptr, dy.ptr, nrhs, PRECISION[precision])
     if rc:
         check()
         raise BfError(f"apply failed ({rc})")

@@ -11,6 +11,7 @@
 /* ---- launchers in bf_kernels.hip */
 int bfk_init(void);
 int bfk_device_count(void);
+int bfk_scratch(void **ptr, size_t *bytes);
 size_t bfk_lds_cap(void);
 void *bfk_stream_handle(void);
 /* rank model and randomized range finder (bf_rank.hip) */

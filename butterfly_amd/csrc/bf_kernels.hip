@@ -57,6 +57,18 @@ static double g_cnt[8];     /* number of measurements since the last reset */
 
 static constexpr double TWO_PI = 6.283185307179586;
 
+/* Fixed device workspace of the allocation-free apply (bfk_scratch): the
+ * intermediate vectors of one apply live here instead of in buffers the
+ * library allocates.  Each device holds its own copy for the life of the
+ * process.  Host code never uses the array's name as an address: the
+ * device address comes from hipGetSymbolAddress, looked up once per device
+ * with that device current (bfk_init for the library's device, or the
+ * first bfk_scratch call on another). */
+#define BFK_SCRATCH_BYTES ((size_t)64 << 20)
+#define BFK_MAX_DEVICES 16
+static __device__ __align__(256) unsigned char scratch[BFK_SCRATCH_BYTES];
+static void *g_scratchPtr[BFK_MAX_DEVICES];
+
 /* ----------------------------------------------------------- helpers */
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -2125,6 +2137,7 @@ int bfk_init(void) {
     return -1;
   }
   BFK_CHECK(hipSetDevice(0));
+  BFK_CHECK(hipGetSymbolAddress(&g_scratchPtr[0], HIP_SYMBOL(scratch)));
   BFK_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   for (int i = 0; i < BFK_SIDE_STREAMS; ++i) {
     BFK_CHECK(hipStreamCreateWithFlags(&g_side[i], hipStreamNonBlocking));
@@ -2160,6 +2173,18 @@ int bfk_init(void) {
              hipFuncSetAttribute((const void *)k_tall_w, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)svd_tall_w_lds(SVD_TALL_MAX_N)) == hipSuccess;
   if (!g_tallOk) (void)hipGetLastError();
+  return 0;
+}
+
+/* the device address and size of the current device's scratch array */
+int bfk_scratch(void **ptr, size_t *bytes) {
+  if (bfk_init()) return -1;
+  int dev = 0;
+  BFK_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= BFK_MAX_DEVICES) return -1;
+  if (!g_scratchPtr[dev]) BFK_CHECK(hipGetSymbolAddress(&g_scratchPtr[dev], HIP_SYMBOL(scratch)));
+  *ptr = g_scratchPtr[dev];
+  *bytes = BFK_SCRATCH_BYTES;
   return 0;
 }
 

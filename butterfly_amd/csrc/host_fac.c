@@ -1118,6 +1118,45 @@ int bfhipMatMulDevice(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, B
   return rc;
 }
 
+/* bfhipMatMulDevice with the intermediate vectors in the library's fixed
+ * device scratch array (two halves, ping and pong) instead of buffers it
+ * allocates: no device memory is allocated or freed.  Every stage writes
+ * all the rows the next stage reads, so nothing needs clearing. */
+int bfhipMatMulDeviceNoAlloc(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable || nrhs == 0 || !dX || !dY) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (!F->dSegs || (prec == BFHIP_PRECISION_FP64 && !F->dArena) ||
+      (prec != BFHIP_PRECISION_FP64 && (!F->dArena32 || (prec == BFHIP_PRECISION_FP32X2 && !F->dArena32lo)))) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  void *ws = NULL;
+  size_t wsBytes = 0;
+  if (bfk_scratch(&ws, &wsBytes)) {
+    bfSetError(BF_ERROR_RUNTIME_ERROR);
+    return -1;
+  }
+  const size_t elem = prec == BFHIP_PRECISION_FP64 ? 16 : 8;
+  const int64_t rows = F->bufRows[0] > F->bufRows[1] ? F->bufRows[0] : F->bufRows[1];
+  const size_t half = (wsBytes / 2) & ~(size_t)255;
+  if ((size_t)rows > half / elem / nrhs) { /* more than a half of the scratch per buffer */
+    bfSetError(BF_ERROR_OUT_OF_RANGE);
+    return -1;
+  }
+  void *ping = ws, *pong = (char *)ws + half;
+  int rc;
+  if (prec == BFHIP_PRECISION_FP64)
+    rc = bfk_apply(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena, dX, ping, pong, dY, (int)nrhs);
+  else
+    rc = bfk_apply_f32(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena32,
+                       prec == BFHIP_PRECISION_FP32X2 ? F->dArena32lo : NULL, dX, ping, pong, dY, (int)nrhs);
+  if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
+  return rc;
+}
+
 int bfhipFacPrepareMixed(BfMat *fac, BfhipPrecision prec) {
   if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);

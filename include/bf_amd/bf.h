@@ -432,6 +432,17 @@ int bfhipDeviceCopy(void const *src, void *dst, BfSize numBytes);
  * Returns 0 on success. */
 int bfhipMatMulDevice(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec);
 
+/* new (no counterpart in the reference): bfhipMatMulDevice without any
+ * device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The intermediate vectors of the apply use a fixed 64 MiB device scratch
+ * array that the library holds on each device until the process ends (two
+ * 32 MiB halves); a factorization whose largest stage needs more than a
+ * half for nrhs right-hand sides (rows x nrhs x 16 B in FP64, 8 B in FP32)
+ * is refused with BF_ERROR_OUT_OF_RANGE.  Calls share that array: calls
+ * that overlap on one device race.  Returns 0 on success. */
+int bfhipMatMulDeviceNoAlloc(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec);
+
 /* new: convert the stored factors for the FP32 / FP32x2 applies (kept
  * beside the FP64 factors until bfMatDelete) */
 int bfhipFacPrepareMixed(BfMat *fac, BfhipPrecision prec);
