@@ -189,6 +189,9 @@ def lib():
         "bfhipResetKernelTimes": (None, []),
         "bfhipSetApplyConfig": (C.c_int, [C.c_int, C.c_int, C.c_int]),
         "bfhipSetApplyCompensation": (C.c_int, [C.c_int]),
+        "bfhipSetApplyGraphs": (C.c_int, [C.c_int]),
+        "bfhipFacTuneApply": (C.c_int, [vp, sz, C.c_int, C.c_char_p]),
+        "bfhipFacGetApplyConfig": (C.c_int, [vp, sz, C.c_int, C.POINTER(C.c_int32)]),
         "bfhipFacPlanSizes": (None, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
         "bfhipFacPlanExport": (None, [vp, i64p, i64p, i64p]),
         "bfhipFacBlockData": (C.c_int, [vp, sz, dp]),
@@ -509,6 +512,28 @@ class Factorization(Mat):
             check()
             raise BfError("orthogonalize failed")
         return lib().bfhipFacOrthogonalizeSeconds(self.ptr)
+
+    def tune_apply(self, nrhs, precision="fp64", cache=None):
+        """bfhipFacTuneApply: time the launch shapes of every stage for nrhs
+        right-hand sides and keep the fastest (cached per stage bucket in
+        the text file `cache`, if given)."""
+        rc = lib().bfhipFacTuneApply(self.ptr, nrhs, PRECISION[precision],
+                                     None if cache is None else str(cache).encode())
+        if rc:
+            check()
+            raise BfError(f"tuning failed ({rc})")
+
+    def apply_config(self, nrhs, precision="fp64"):
+        """The launch shape used by each stage (-1 = the global settings)."""
+        n = lib().bfhipFacGetApplyConfig(self.ptr, nrhs, PRECISION[precision], None)
+        out = (C.c_int32 * max(n, 1))()
+        lib().bfhipFacGetApplyConfig(self.ptr, nrhs, PRECISION[precision], out)
+        return list(out[:n])
+
+
+def set_apply_graphs(on):
+    """bfhipSetApplyGraphs: replay the apply's stage chain as a hipGraph."""
+    lib().bfhipSetApplyGraphs(1 if on else 0)
 
 
 class FacHelm2:

@@ -69,10 +69,22 @@ int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, co
               const size_t *groupBytes, const BfkOrthoItem *dItems, double *dArena, const double *dRin,
               double *dRout);
 int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
-              const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs);
+              const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs,
+              const int32_t *stageCfg);
 int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
                   const float *dArenaHi, const float *dArenaLo, const float *dX, float *dPing, float *dPong,
-                  float *dY, int nrhs);
+                  float *dY, int nrhs, const int32_t *stageCfg);
+int bfk_apply_stage_ms(const BfkSeg *dSegs, const int64_t *stageSeg, int s, const BfkTerm *dTerms,
+                       const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs,
+                       int cfg, int reps, double *ms);
+int bfk_apply_f32_stage_ms(const BfkSeg *dSegs, const int64_t *stageSeg, int s, const BfkTerm *dTerms,
+                           const float *dArenaHi, const float *dArenaLo, const float *dX, float *dPing, float *dPong,
+                           float *dY, int nrhs, int cfg, int reps, double *ms);
+int bfk_graph_begin(void);
+int bfk_graph_end(void **exec);
+int bfk_graph_launch(void *exec, int slot);
+void bfk_graph_destroy(void *exec);
+int bfk_apply_config_stamp(void);
 int bfk_to_f32(const double *dArena, int64_t numEntries, float *dHi, float *dLo);
 int bfk_h0(int64_t n, const double *x, double *out);
 int bfk_copy(const void *src, void *dst, size_t bytes);
@@ -121,6 +133,17 @@ typedef struct {
   int64_t denseOff;            /* arena offset of a dense leaf */
 } BfHipLeaf;
 
+/* right-hand-side classes of the tuned launch shapes: 1, 2-15, 16-31,
+ * 32-63, 64-127, 128-255, 256 and more */
+#define BF_NRHS_CLASSES 7
+static inline int bf_nrhs_class(size_t nrhs) {
+  if (nrhs <= 1) return 0;
+  if (nrhs < 16) return 1;
+  int c = 2;
+  for (size_t v = 32; c < BF_NRHS_CLASSES - 1 && nrhs >= v; v *= 2) ++c;
+  return c;
+}
+
 typedef BF_VEC(BfHipLeaf) BfHipLeafVec;
 typedef BF_VEC(BfHipFactor) BfHipFactorVec;
 typedef BF_VEC(BfHipBlock) BfHipBlockVec;
@@ -164,6 +187,17 @@ typedef struct BfMatHipButterfly {
   BfkTerm *dTerms;
   void *dPing, *dPong;
   size_t bufBytes;
+
+  /* launch shape per stage chosen by the tuner, per precision and
+   * right-hand-side class (NULL: the global settings) */
+  int32_t *tuneCfg[3][BF_NRHS_CLASSES];
+  /* the apply chain captured as a graph, and what it was captured for */
+  void *graphExec;
+  const void *graphX;
+  void *graphY, *graphPing, *graphPong;
+  size_t graphNrhs;
+  int graphPrec, graphStamp;
+  const int32_t *graphCfg;
 
   double stats[16];
   int statsCounted;    /* stats 4-8 and 10 are filled in */

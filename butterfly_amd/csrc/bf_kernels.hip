@@ -2680,11 +2680,50 @@ int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, co
   return 0;
 }
 
-/* one apply: `numStages` launches; stage s covers segments
- * [stageSeg[s], stageSeg[s+1]) */
-int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
-              const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs) {
-  if (bfk_init()) return -1;
+/* The launch shape of stage s: cfg < 0 takes the global settings; for one
+ * right-hand side cfg = waves per block | unroll << 4, for 16 and more
+ * cfg = right-hand-side tiles of 16 per MFMA wave (0 = the vector-unit
+ * kernel); between 2 and 15 right-hand sides it is not used. */
+static int g_capturing = 0; /* launches go into a graph: no timing events */
+
+static int launch_stage_f64(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms, const double *dArena,
+                            const BfkBufs &b, int nrhs, int cfg) {
+  if (n == 0) return 0;
+  if (nrhs == 1) {
+    const int wv = cfg >= 0 ? (cfg & 15) : g_applyWaves;
+    const int unroll = cfg >= 0 ? (cfg >> 4) & 1 : g_applyUnroll;
+    dim3 grid((unsigned)((n + wv - 1) / wv)), block(64 * wv);
+    if (unroll)
+      hipLaunchKernelGGL((k_apply1<double2, false, true>), grid, block, 0, g_stream, segs, n, dTerms,
+                         (const double2 *)dArena, (const double2 *)nullptr, b);
+    else
+      hipLaunchKernelGGL((k_apply1<double2, false, false>), grid, block, 0, g_stream, segs, n, dTerms,
+                         (const double2 *)dArena, (const double2 *)nullptr, b);
+  } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT) > 0) {
+    const int ct = cfg >= 0 ? cfg : g_mfmaCT;
+    const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
+    dim3 grid(mfma_grid(n, chunks));
+    if (ct == 1)
+      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 1, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
+                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
+    else if (ct == 2)
+      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 2, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
+                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
+    else if (ct == 8)
+      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 8, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
+                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
+    else
+      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 4, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
+                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
+  } else {
+    hipLaunchKernelGGL(k_applyN, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream, segs,
+                       dTerms, (const double2 *)dArena, b, nrhs);
+  }
+  BFK_CHECK(hipGetLastError());
+  return 0;
+}
+
+static BfkBufs bufs_f64(const double *dX, double *dPing, double *dPong, double *dY) {
   BfkBufs b;
   b.in[BFK_BUF_X] = (const double2 *)dX;
   b.in[BFK_BUF_PING] = (const double2 *)dPing;
@@ -2694,42 +2733,82 @@ int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const
   b.out[BFK_BUF_PING] = (double2 *)dPing;
   b.out[BFK_BUF_PONG] = (double2 *)dPong;
   b.out[BFK_BUF_Y] = (double2 *)dY;
+  return b;
+}
+
+/* one apply: `numStages` launches; stage s covers segments
+ * [stageSeg[s], stageSeg[s+1]) with launch shape stageCfg[s] (NULL: the
+ * global settings for every stage) */
+int bfk_apply(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
+              const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs,
+              const int32_t *stageCfg) {
+  if (bfk_init()) return -1;
+  const BfkBufs b = bufs_f64(dX, dPing, dPong, dY);
+  if (!g_capturing && time_begin()) return -1;
+  for (int s = 0; s < numStages; ++s)
+    if (launch_stage_f64(dSegs + stageSeg[s], stageSeg[s + 1] - stageSeg[s], dTerms, dArena, b, nrhs,
+                         stageCfg ? stageCfg[s] : -1))
+      return -1;
+  return g_capturing ? 0 : time_end(2);
+}
+
+/* device ms per launch of stage s alone with launch shape cfg, averaged
+ * over `reps` launches after one untimed launch (for the tuner; the
+ * buffers hold whatever the last apply left) */
+int bfk_apply_stage_ms(const BfkSeg *dSegs, const int64_t *stageSeg, int s, const BfkTerm *dTerms,
+                       const double *dArena, const double *dX, double *dPing, double *dPong, double *dY, int nrhs,
+                       int cfg, int reps, double *ms) {
+  if (bfk_init()) return -1;
+  const BfkBufs b = bufs_f64(dX, dPing, dPong, dY);
+  const int64_t n = stageSeg[s + 1] - stageSeg[s];
+  if (launch_stage_f64(dSegs + stageSeg[s], n, dTerms, dArena, b, nrhs, cfg)) return -1;
+  BFK_CHECK(hipEventRecord(g_ev0, g_stream));
+  for (int r = 0; r < reps; ++r)
+    if (launch_stage_f64(dSegs + stageSeg[s], n, dTerms, dArena, b, nrhs, cfg)) return -1;
+  BFK_CHECK(hipEventRecord(g_ev1, g_stream));
+  BFK_CHECK(hipEventSynchronize(g_ev1));
+  float t = 0;
+  BFK_CHECK(hipEventElapsedTime(&t, g_ev0, g_ev1));
+  *ms = (double)t / (reps > 0 ? reps : 1);
+  return 0;
+}
+
+/* graphs of apply chains: begin capturing g_stream, end into an
+ * executable graph, launch one (timed like an apply in `slot`), destroy */
+int bfk_graph_begin(void) {
+  if (bfk_init()) return -1;
+  BFK_CHECK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+  g_capturing = 1;
+  return 0;
+}
+
+int bfk_graph_end(void **exec) {
+  g_capturing = 0;
+  *exec = nullptr;
+  hipGraph_t g = nullptr;
+  BFK_CHECK(hipStreamEndCapture(g_stream, &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  BFK_CHECK(err);
+  *exec = (void *)e;
+  return 0;
+}
+
+int bfk_graph_launch(void *exec, int slot) {
   if (time_begin()) return -1;
-  for (int s = 0; s < numStages; ++s) {
-    int64_t n = stageSeg[s + 1] - stageSeg[s];
-    if (n == 0) continue;
-    if (nrhs == 1) {
-      const int wv = g_applyWaves;
-      dim3 grid((unsigned)((n + wv - 1) / wv)), block(64 * wv);
-      if (g_applyUnroll)
-        hipLaunchKernelGGL((k_apply1<double2, false, true>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b);
-      else
-        hipLaunchKernelGGL((k_apply1<double2, false, false>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b);
-    } else if (nrhs >= 16 && g_mfmaCT > 0) {
-      const int ct = g_mfmaCT;
-      const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
-      dim3 grid(mfma_grid(n, chunks));
-      if (ct == 1)
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 1, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
-      else if (ct == 2)
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 2, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
-      else if (ct == 8)
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 8, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
-      else
-        hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 4, false>), grid, dim3(64), 0, g_stream, dSegs + stageSeg[s],
-                           dTerms, (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
-    } else {
-      hipLaunchKernelGGL(k_applyN, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
-                         dSegs + stageSeg[s], dTerms, (const double2 *)dArena, b, nrhs);
-    }
-    BFK_CHECK(hipGetLastError());
-  }
-  return time_end(2);
+  BFK_CHECK(hipGraphLaunch((hipGraphExec_t)exec, g_stream));
+  return time_end(slot);
+}
+
+void bfk_graph_destroy(void *exec) {
+  if (exec) (void)hipGraphExecDestroy((hipGraphExec_t)exec);
+}
+
+/* the global launch settings as one number: a captured apply is stale once
+ * it changes */
+int bfk_apply_config_stamp(void) {
+  return g_applyWaves | g_applyUnroll << 4 | g_mfmaCT << 5 | g_comp32 << 9 | g_foldMask << 10;
 }
 
 int bfk_copy(const void *src, void *dst, size_t bytes) {
@@ -2881,10 +2960,44 @@ int bfk_to_f32(const double *dArena, int64_t numEntries, float *dHi, float *dLo)
   return 0;
 }
 
-int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
-                  const float *dArenaHi, const float *dArenaLo, const float *dX, float *dPing, float *dPong, float *dY,
-                  int nrhs) {
-  if (bfk_init()) return -1;
+static int launch_stage_f32(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms, const float *dArenaHi,
+                            const float *dArenaLo, const BfkBufsF &b, int nrhs, int cfg) {
+  if (n == 0) return 0;
+  const float2 *H = (const float2 *)dArenaHi, *Lo = (const float2 *)dArenaLo;
+  if (nrhs == 1) {
+    const int wv = cfg >= 0 ? (cfg & 15) : g_applyWaves;
+    const int unroll = cfg >= 0 ? (cfg >> 4) & 1 : g_applyUnroll;
+    dim3 grid((unsigned)((n + wv - 1) / wv)), block(64 * wv);
+    if (g_comp32) {
+      if (Lo) launch_apply1_f32<true, true>(grid, block, segs, n, dTerms, H, Lo, b);
+      else launch_apply1_f32<false, true>(grid, block, segs, n, dTerms, H, Lo, b);
+    } else if (Lo) {
+      launch_apply1_f32<true, false>(grid, block, segs, n, dTerms, H, Lo, b);
+    } else if (unroll) {
+      hipLaunchKernelGGL((k_apply1<float2, false, true>), grid, block, 0, g_stream, segs, n, dTerms, H, Lo, b);
+    } else {
+      launch_apply1_f32<false, false>(grid, block, segs, n, dTerms, H, Lo, b);
+    }
+  } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT) > 0) {
+    const int ct = cfg >= 0 ? cfg : g_mfmaCT;
+    const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
+    dim3 grid(mfma_grid(n, chunks));
+    if (Lo) {
+      if (g_comp32) launch_mfma_f32<true, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+      else launch_mfma_f32<true, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+    } else {
+      if (g_comp32) launch_mfma_f32<false, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+      else launch_mfma_f32<false, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+    }
+  } else {
+    hipLaunchKernelGGL(k_applyN_f32, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream, segs,
+                       dTerms, H, Lo, b, nrhs);
+  }
+  BFK_CHECK(hipGetLastError());
+  return 0;
+}
+
+static BfkBufsF bufs_f32(const float *dX, float *dPing, float *dPong, float *dY) {
   BfkBufsF b;
   b.in[BFK_BUF_X] = (const float2 *)dX;
   b.in[BFK_BUF_PING] = (const float2 *)dPing;
@@ -2894,44 +3007,38 @@ int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, c
   b.out[BFK_BUF_PING] = (float2 *)dPing;
   b.out[BFK_BUF_PONG] = (float2 *)dPong;
   b.out[BFK_BUF_Y] = (float2 *)dY;
-  if (time_begin()) return -1;
-  for (int s = 0; s < numStages; ++s) {
-    int64_t n = stageSeg[s + 1] - stageSeg[s];
-    if (n == 0) continue;
-    if (nrhs == 1) {
-      const int wv = g_applyWaves;
-      dim3 grid((unsigned)((n + wv - 1) / wv)), block(64 * wv);
-      const float2 *H = (const float2 *)dArenaHi, *Lo = (const float2 *)dArenaLo;
-      if (g_comp32) {
-        if (Lo) launch_apply1_f32<true, true>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
-        else launch_apply1_f32<false, true>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
-      } else if (Lo) {
-        launch_apply1_f32<true, false>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
-      } else if (g_applyUnroll) {
-        hipLaunchKernelGGL((k_apply1<float2, false, true>), grid, block, 0, g_stream, dSegs + stageSeg[s], n,
-                           dTerms, H, Lo, b);
-      } else {
-        launch_apply1_f32<false, false>(grid, block, dSegs + stageSeg[s], n, dTerms, H, Lo, b);
-      }
-    } else if (nrhs >= 16 && g_mfmaCT > 0) {
-      const int ct = g_mfmaCT;
-      const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
-      dim3 grid(mfma_grid(n, chunks));
-      const float2 *H = (const float2 *)dArenaHi, *Lo = (const float2 *)dArenaLo;
-      if (Lo) {
-        if (g_comp32) launch_mfma_f32<true, true>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
-        else launch_mfma_f32<true, false>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
-      } else {
-        if (g_comp32) launch_mfma_f32<false, true>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
-        else launch_mfma_f32<false, false>(ct, grid, dSegs + stageSeg[s], dTerms, H, Lo, b, nrhs, chunks, n);
-      }
-    } else {
-      hipLaunchKernelGGL(k_applyN_f32, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream,
-                         dSegs + stageSeg[s], dTerms, (const float2 *)dArenaHi, (const float2 *)dArenaLo, b, nrhs);
-    }
-    BFK_CHECK(hipGetLastError());
-  }
-  return time_end(3);
+  return b;
+}
+
+int bfk_apply_f32(const BfkSeg *dSegs, const int64_t *stageSeg, int numStages, const BfkTerm *dTerms,
+                  const float *dArenaHi, const float *dArenaLo, const float *dX, float *dPing, float *dPong, float *dY,
+                  int nrhs, const int32_t *stageCfg) {
+  if (bfk_init()) return -1;
+  const BfkBufsF b = bufs_f32(dX, dPing, dPong, dY);
+  if (!g_capturing && time_begin()) return -1;
+  for (int s = 0; s < numStages; ++s)
+    if (launch_stage_f32(dSegs + stageSeg[s], stageSeg[s + 1] - stageSeg[s], dTerms, dArenaHi, dArenaLo, b, nrhs,
+                         stageCfg ? stageCfg[s] : -1))
+      return -1;
+  return g_capturing ? 0 : time_end(3);
+}
+
+int bfk_apply_f32_stage_ms(const BfkSeg *dSegs, const int64_t *stageSeg, int s, const BfkTerm *dTerms,
+                           const float *dArenaHi, const float *dArenaLo, const float *dX, float *dPing, float *dPong,
+                           float *dY, int nrhs, int cfg, int reps, double *ms) {
+  if (bfk_init()) return -1;
+  const BfkBufsF b = bufs_f32(dX, dPing, dPong, dY);
+  const int64_t n = stageSeg[s + 1] - stageSeg[s];
+  if (launch_stage_f32(dSegs + stageSeg[s], n, dTerms, dArenaHi, dArenaLo, b, nrhs, cfg)) return -1;
+  BFK_CHECK(hipEventRecord(g_ev0, g_stream));
+  for (int r = 0; r < reps; ++r)
+    if (launch_stage_f32(dSegs + stageSeg[s], n, dTerms, dArenaHi, dArenaLo, b, nrhs, cfg)) return -1;
+  BFK_CHECK(hipEventRecord(g_ev1, g_stream));
+  BFK_CHECK(hipEventSynchronize(g_ev1));
+  float t = 0;
+  BFK_CHECK(hipEventElapsedTime(&t, g_ev0, g_ev1));
+  *ms = (double)t / (reps > 0 ? reps : 1);
+  return 0;
 }
 
 } /* extern "C" */

@@ -54,6 +54,13 @@ static void free_device(BfMatHipButterfly *F) {
   bfk_free(F->dTgtXY);
   bfk_free(F->dSrcN);
   bfk_free(F->dTgtN);
+  bfk_graph_destroy(F->graphExec);
+  F->graphExec = NULL;
+  for (int p = 0; p < 3; ++p)
+    for (int c = 0; c < BF_NRHS_CLASSES; ++c) {
+      free(F->tuneCfg[p][c]);
+      F->tuneCfg[p][c] = NULL;
+    }
   bfk_free(F->dSegs);
   bfk_free(F->dTerms);
   bfk_free(F->dPing);
@@ -1089,33 +1096,195 @@ static int ensure_buffers(BfMatHipButterfly *F, size_t nrhs, size_t elemBytes) {
   return 0;
 }
 
+/* apply graphs: off (0, the default) or on */
+static int g_applyGraphs = 0;
+
+int bfhipSetApplyGraphs(int on) {
+  g_applyGraphs = on ? 1 : 0;
+  return 0;
+}
+
+/* the stage launches of one apply (into a graph while one is captured) */
+static int enqueue_apply(BfMatHipButterfly *F, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec,
+                         const int32_t *cfg) {
+  if (prec == BFHIP_PRECISION_FP64)
+    return bfk_apply(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena, dX, F->dPing, F->dPong, dY,
+                     (int)nrhs, cfg);
+  return bfk_apply_f32(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena32,
+                       prec == BFHIP_PRECISION_FP32X2 ? F->dArena32lo : NULL, dX, F->dPing, F->dPong, dY, (int)nrhs,
+                       cfg);
+}
+
 int bfhipMatMulDevice(BfMat const *fac, void const *dX, void *dY, BfSize nrhs, BfhipPrecision prec) {
-  if (!fac || fac->vtbl != &bfMatHipButterflyVtable || nrhs == 0) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable || nrhs == 0 || (unsigned)prec > 2u) {
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return -1;
   }
   BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
   int rc;
-  if (!F->dSegs || (prec == BFHIP_PRECISION_FP64 && !F->dArena)) {
+  if (!F->dSegs || (prec == BFHIP_PRECISION_FP64 && !F->dArena) ||
+      (prec != BFHIP_PRECISION_FP64 && (!F->dArena32 || (prec == BFHIP_PRECISION_FP32X2 && !F->dArena32lo)))) {
     bfSetError(BF_ERROR_INVALID_ARGUMENTS);
     return -1;
   }
-  if (prec == BFHIP_PRECISION_FP64) {
-    if ((rc = ensure_buffers(F, nrhs, 16))) { bfSetError(BF_ERROR_MEMORY_ERROR); return rc; }
-    rc = bfk_apply(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena, dX, F->dPing, F->dPong, dY,
-                   (int)nrhs);
+  if ((rc = ensure_buffers(F, nrhs, prec == BFHIP_PRECISION_FP64 ? 16 : 8))) {
+    bfSetError(BF_ERROR_MEMORY_ERROR);
+    return rc;
+  }
+  const int32_t *cfg = F->tuneCfg[prec][bf_nrhs_class(nrhs)];
+  if (!g_applyGraphs) {
+    rc = enqueue_apply(F, dX, dY, nrhs, prec, cfg);
   } else {
-    if (!F->dArena32 || (prec == BFHIP_PRECISION_FP32X2 && !F->dArena32lo)) {
-      bfSetError(BF_ERROR_INVALID_ARGUMENTS);
-      return -1;
+    /* one captured chain per factorization, replayed while the vectors,
+     * buffers, precision and launch shapes stay the same */
+    const int stamp = bfk_apply_config_stamp();
+    if (!F->graphExec || F->graphX != dX || F->graphY != dY || F->graphPing != F->dPing ||
+        F->graphPong != F->dPong || F->graphNrhs != nrhs || F->graphPrec != (int)prec || F->graphCfg != cfg ||
+        F->graphStamp != stamp) {
+      bfk_graph_destroy(F->graphExec);
+      F->graphExec = NULL;
+      rc = bfk_graph_begin();
+      if (!rc) {
+        const int rcApply = enqueue_apply(F, dX, dY, nrhs, prec, cfg);
+        rc = bfk_graph_end(&F->graphExec);
+        if (!rc) rc = rcApply;
+      }
+      if (!rc) {
+        F->graphX = dX;
+        F->graphY = dY;
+        F->graphPing = F->dPing;
+        F->graphPong = F->dPong;
+        F->graphNrhs = nrhs;
+        F->graphPrec = (int)prec;
+        F->graphCfg = cfg;
+        F->graphStamp = stamp;
+      }
     }
-    if ((rc = ensure_buffers(F, nrhs, 8))) { bfSetError(BF_ERROR_MEMORY_ERROR); return rc; }
-    rc = bfk_apply_f32(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena32,
-                       prec == BFHIP_PRECISION_FP32X2 ? F->dArena32lo : NULL, dX, F->dPing, F->dPong, dY,
-                       (int)nrhs);
+    if (!rc) rc = bfk_graph_launch(F->graphExec, prec == BFHIP_PRECISION_FP64 ? 2 : 3);
   }
   if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
   return rc;
+}
+
+/* ------------------------------------------------------- the autotuner */
+
+/* A stage's bucket: rounded log2 of its mean segment height and of its
+ * mean columns per segment (the sum over the segment's terms). */
+static void stage_bucket(BfMatHipButterfly const *F, int s, int *hb, int *cb) {
+  const int64_t a = F->stageSeg.a[s], b = F->stageSeg.a[s + 1];
+  double rows = 0, cols = 0;
+  for (int64_t i = a; i < b; ++i) {
+    rows += F->segs.a[i].m;
+    for (int32_t t = F->segs.a[i].t0; t < F->segs.a[i].t1; ++t) cols += F->terms.a[t].aOff < 0 ? 1 : F->terms.a[t].n;
+  }
+  const double n = (double)(b > a ? b - a : 1);
+  *hb = (int)lround(log2(rows / n > 1 ? rows / n : 1));
+  *cb = (int)lround(log2(cols / n > 1 ? cols / n : 1));
+}
+
+typedef struct {
+  int prec, cls, hb, cb, cfg;
+} TuneEntry;
+typedef BF_VEC(TuneEntry) TuneVec;
+
+static void tune_load(TuneVec *v, char const *path) {
+  FILE *fp = path ? fopen(path, "r") : NULL;
+  if (!fp) return;
+  TuneEntry e;
+  while (fscanf(fp, "%d %d %d %d %d", &e.prec, &e.cls, &e.hb, &e.cb, &e.cfg) == 5) BF_VEC_PUSH(*v, e);
+  fclose(fp);
+}
+
+static int tune_save(const TuneVec *v, char const *path) {
+  FILE *fp = path ? fopen(path, "w") : NULL;
+  if (!fp) return path ? -1 : 0;
+  fprintf(fp, "%s", "");
+  for (size_t i = 0; i < v->n; ++i)
+    fprintf(fp, "%d %d %d %d %d\n", v->a[i].prec, v->a[i].cls, v->a[i].hb, v->a[i].cb, v->a[i].cfg);
+  return fclose(fp) ? -1 : 0;
+}
+
+int bfhipFacTuneApply(BfMat *fac, BfSize nrhs, BfhipPrecision prec, char const *cachePath) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable || nrhs == 0 || (unsigned)prec > 2u) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (!F->dSegs || (prec == BFHIP_PRECISION_FP64 && !F->dArena) ||
+      (prec != BFHIP_PRECISION_FP64 && (!F->dArena32 || (prec == BFHIP_PRECISION_FP32X2 && !F->dArena32lo)))) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  const int cls = bf_nrhs_class(nrhs);
+  if (cls == 1) return 0; /* 2-15 right-hand sides: the vector-unit kernel only */
+  if (!cachePath) cachePath = getenv("BF_AMD_TUNE_CACHE");
+  /* candidates: waves per block x lane map (1 RHS); tiles per MFMA wave,
+   * 0 = the vector-unit kernel (16 and more) */
+  static const int one[] = {1, 2, 4, 1 | 16, 2 | 16, 4 | 16}, many[] = {1, 2, 4, 8, 0};
+  const int *cand = cls == 0 ? one : many;
+  const int nc = cls == 0 ? 6 : 5;
+  const size_t elem = prec == BFHIP_PRECISION_FP64 ? 16 : 8;
+  int rc = ensure_buffers(F, nrhs, elem);
+  void *dX = rc ? NULL : bfk_malloc(F->numSrc * nrhs * elem + 16);
+  void *dY = rc ? NULL : bfk_malloc(F->numTgt * nrhs * elem + 16);
+  if (!dX || !dY) rc = -2;
+  if (!rc) rc = bfk_memset(dX, F->numSrc * nrhs * elem + 16);
+  if (!rc) rc = enqueue_apply(F, dX, dY, nrhs, prec, NULL); /* finite values in every buffer */
+  TuneVec cache;
+  memset(&cache, 0, sizeof(cache));
+  tune_load(&cache, cachePath);
+  const size_t cached = cache.n;
+  int32_t *cfg = calloc((size_t)(F->numStages ? F->numStages : 1), sizeof(int32_t));
+  for (int s = 0; s < F->numStages && !rc; ++s) {
+    int hb, cb, found = 0;
+    stage_bucket(F, s, &hb, &cb);
+    for (size_t i = 0; i < cache.n && !found; ++i)
+      if (cache.a[i].prec == (int)prec && cache.a[i].cls == cls && cache.a[i].hb == hb && cache.a[i].cb == cb) {
+        cfg[s] = cache.a[i].cfg;
+        found = 1;
+      }
+    if (found || F->stageSeg.a[s + 1] == F->stageSeg.a[s]) continue;
+    double best = 1e300;
+    for (int c = 0; c < nc && !rc; ++c) {
+      double ms = 0;
+      rc = prec == BFHIP_PRECISION_FP64
+               ? bfk_apply_stage_ms(F->dSegs, F->stageSeg.a, s, F->dTerms, F->dArena, dX, F->dPing, F->dPong, dY,
+                                    (int)nrhs, cand[c], 3, &ms)
+               : bfk_apply_f32_stage_ms(F->dSegs, F->stageSeg.a, s, F->dTerms, F->dArena32,
+                                        prec == BFHIP_PRECISION_FP32X2 ? F->dArena32lo : NULL, dX, F->dPing,
+                                        F->dPong, dY, (int)nrhs, cand[c], 3, &ms);
+      if (!rc && ms < best) {
+        best = ms;
+        cfg[s] = cand[c];
+      }
+    }
+    TuneEntry e = {(int)prec, cls, hb, cb, cfg[s]};
+    if (!rc) BF_VEC_PUSH(cache, e);
+  }
+  if (!rc && cache.n > cached && tune_save(&cache, cachePath)) rc = -3;
+  bfk_free(dX);
+  bfk_free(dY);
+  BF_VEC_FREE(cache);
+  if (rc) {
+    free(cfg);
+    bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : rc == -3 ? BF_ERROR_FILE_ERROR : BF_ERROR_RUNTIME_ERROR);
+    return rc;
+  }
+  free(F->tuneCfg[prec][cls]);
+  F->tuneCfg[prec][cls] = cfg;
+  return 0;
+}
+
+int bfhipFacGetApplyConfig(BfMat const *fac, BfSize nrhs, BfhipPrecision prec, int32_t *stageCfg) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable || (unsigned)prec > 2u) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  const BfMatHipButterfly *F = (const BfMatHipButterfly *)fac;
+  const int32_t *cfg = F->tuneCfg[prec][bf_nrhs_class(nrhs)];
+  if (stageCfg)
+    for (int s = 0; s < F->numStages; ++s) stageCfg[s] = cfg ? cfg[s] : -1;
+  return F->numStages;
 }
 
 /* bfhipMatMulDevice with the intermediate vectors in the library's fixed
@@ -1148,11 +1317,12 @@ int bfhipMatMulDeviceNoAlloc(BfMat const *fac, void const *dX, void *dY, BfSize 
   }
   void *ping = ws, *pong = (char *)ws + half;
   int rc;
+  const int32_t *cfg = F->tuneCfg[prec][bf_nrhs_class(nrhs)];
   if (prec == BFHIP_PRECISION_FP64)
-    rc = bfk_apply(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena, dX, ping, pong, dY, (int)nrhs);
+    rc = bfk_apply(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena, dX, ping, pong, dY, (int)nrhs, cfg);
   else
     rc = bfk_apply_f32(F->dSegs, F->stageSeg.a, F->numStages, F->dTerms, F->dArena32,
-                       prec == BFHIP_PRECISION_FP32X2 ? F->dArena32lo : NULL, dX, ping, pong, dY, (int)nrhs);
+                       prec == BFHIP_PRECISION_FP32X2 ? F->dArena32lo : NULL, dX, ping, pong, dY, (int)nrhs, cfg);
   if (rc) bfSetError(BF_ERROR_RUNTIME_ERROR);
   return rc;
 }

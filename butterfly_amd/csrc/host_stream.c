@@ -1445,7 +1445,7 @@ static BfMat *prod_mul(BfMat const *A, BfMat const *Xm) {
   if (!rc) rc = bfk_h2d(dx, hx, (size_t)(n * nrhs) * 16);
   if (!rc)
     rc = bfk_apply(P->dSegs, P->stageSeg.a, P->numStages, P->dTerms, P->dArena, dx, P->dPing, P->dPong, dy,
-                   (int)nrhs);
+                   (int)nrhs, NULL);
   BfMatDenseComplex *Y = NULL;
   if (!rc) {
     Y = bfMatDenseComplexNew();

@@ -485,6 +485,31 @@ void bfhipResetKernelTimes(void);
  * tiles of 16 per wave on the matrix cores in {2, 4, 8}, or 0 for the
  * vector-unit kernel */
 int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
+/* new (no counterpart): hipGraph replay of the apply's stage chain, off (0,
+ * the default) or on.  When on, bfhipMatMulDevice captures the stage
+ * launches of a factorization once and replays the graph while the input
+ * and output pointers, right-hand-side count, precision, launch shapes
+ * and the library's buffers stay the same (no memset or copy inside the
+ * graph); anything else recaptures.  For callers that apply one operator
+ * many times (iterative solvers, batched serving). */
+int bfhipSetApplyGraphs(int on);
+/* new (no counterpart): the apply's autotuner.  For each apply stage of
+ * `fac` it times the launch shapes that fit `nrhs` (one right-hand side:
+ * 1, 2 or 4 waves per block with either lane map; 16 and more: 1, 2, 4 or
+ * 8 tiles of 16 right-hand sides per matrix-core wave, or the vector-unit
+ * kernel) on that stage alone and keeps the fastest, per bucket of stages
+ * (rounded log2 of the mean segment height and of the mean columns per
+ * segment), for the precision and the class of nrhs (1, 2-15, 16-31,
+ * 32-63, 64-127, 128-255, 256+).  Buckets found in the text file
+ * `cachePath` (or $BF_AMD_TUNE_CACHE when NULL; none when both are
+ * absent) are not timed again; new ones are appended to it.  Later applies
+ * of `fac` in that class use the chosen shapes.  Allocates work vectors
+ * while it runs. */
+int bfhipFacTuneApply(BfMat *fac, BfSize nrhs, BfhipPrecision prec, char const *cachePath);
+/* new: the launch shape of each stage used for nrhs right-hand sides at
+ * `prec` (-1 = the global settings); returns the number of stages */
+int bfhipFacGetApplyConfig(BfMat const *fac, BfSize nrhs, BfhipPrecision prec, int32_t *stageCfg);
+
 /* new: compensated accumulation in the FP32 applies, on (1, the default)
  * or off (0): FP32 products summed in FP64 (1 RHS) or FP32 matrix-core
  * tiles folded into FP64 sums (many RHS), so the FP32 rounding of a stage
