@@ -113,7 +113,7 @@ def gmres_section(bf, n, tol, rank_const, rcond, restart, solve_tol):
     return out
 
 
-def batch_section(bf, count, nrhs, steps, rows):
+def batch_section(bf, count, nrhs, steps, rows, tune_cache=None):
     """Independent problems: sources uniform in [0,1]^2, targets in
     [2,3]x[0,1] (seeded), N log-uniform in [2^12, 2^18], k for 8 points per
     wavelength, leaf size from {32, 64, 128}, tol 1e-7; factored one by one
@@ -183,6 +183,8 @@ def batch_section(bf, count, nrhs, steps, rows):
           f"{bst['apply_plan_s']:.2f}s", file=sys.stderr, flush=True)
     dX = bf.DeviceArray.from_numpy(X)
     dY = bf.DeviceArray(16 * ntgt * nrhs)
+    if tune_cache:
+        B.tune_apply(nrhs, "fp64", tune_cache)
     bf.mul_device(B, dX, dY, nrhs, "fp64")
     bf.lib().bfhipSynchronize()
     t0 = time.perf_counter()
@@ -195,6 +197,8 @@ def batch_section(bf, count, nrhs, steps, rows):
     B.orthogonalize()
     B.prepare_mixed("fp32")
     mixed_prep = time.perf_counter() - t0
+    if tune_cache:
+        B.tune_apply(nrhs, "fp32", tune_cache)
     dX32 = bf.DeviceArray.from_numpy(X.astype(np.complex64))
     dY32 = bf.DeviceArray(8 * ntgt * nrhs)
     bf.mul_device(B, dX32, dY32, nrhs, "fp32")
@@ -228,19 +232,17 @@ def batch_section(bf, count, nrhs, steps, rows):
 
 
 def stream_section(bf, n, tol, oversample, epochs, rows, threads):
-    """Streaming factorization (bfFacStreamer*) of n sources U[0,1]^2 ->
-    n targets U[2,3]x[0,1] (seed 0), k for 8 points per wavelength, leaf 64,
-    tol `tol`, min rows / columns 20, the column blocks sampled on the device
-    beforehand: plain (every truncated SVD in full) against rank-predicted
-    (randomized SVDs of width predicted rank + `oversample` where that is
-    under half the block's smaller side, full SVD as fallback), the model
-    trained on the (geometry, numerical rank) samples the plain run recorded.
-    Times are the feed loops only; errors against exact rows."""
+    """Streaming factorization (bfFacStreamer*) of the headline geometry at
+    n points (n sources on r = 1 + 0.3 cos 5t -> n targets on the circle of
+    radius 3, k for 10 points per wavelength), leaf 64, tol `tol`, row cuts
+    from the second level down, min rows / columns 20, the column blocks sampled on the device beforehand: plain
+    (every truncated SVD in full) against rank-predicted (randomized SVDs of
+    width predicted rank + `oversample` where that is under half the block's
+    smaller side, full SVD as fallback), the model trained on the
+    (geometry, numerical rank) samples the plain run recorded.  Times are
+    the feed loops only; errors against exact rows."""
     rng = np.random.default_rng(0)
-    src = rng.uniform(0, 1, (n, 2))
-    tgt = rng.uniform(0, 1, (n, 2))
-    tgt[:, 0] += 2
-    k = 2 * np.pi * np.sqrt(n) / 8
+    src, tgt, k = curve_problem(n)
     ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
     qs = bf.QuadtreeStruct.from_address(ts.ptr)
     nodes = [(int(a.offset[0]), int(a.offset[4]))
@@ -250,7 +252,7 @@ def stream_section(bf, n, tol, oversample, epochs, rows, threads):
     blocks = {}
 
     def stream(model):
-        S = bf.FacStreamer(tt, ts, tol, init_depth=1, min_rows=20, min_cols=20)
+        S = bf.FacStreamer(tt, ts, tol, init_depth=2, min_rows=20, min_cols=20)
         S.set_rank_model(model, k, oversample)
         t = 0.0
         while not S.done:
@@ -283,18 +285,22 @@ def stream_section(bf, n, tol, oversample, epochs, rows, threads):
           f"trained on its samples in {t_train:.2f}s (MAE {mae:.2f}), rank-predicted {t_model:.2f}s "
           f"({st['sketched']} sketched, {st['fell_back']} fell back); errors {errs[0]} / {errs[1]}",
           file=sys.stderr, flush=True)
-    return {"workload": f"bfFacStreamer: {n} sources U[0,1]^2 -> {n} targets U[2,3]x[0,1], k={k:.1f} (8 ppw), "
-                        f"leaf 64, tol {tol:g}, min rows/cols 20",
+    return {"workload": f"bfFacStreamer: {n} sources on r=1+0.3cos(5t) -> {n} targets on r=3, k={k:.1f} (10 ppw), "
+                        f"leaf 64, tol {tol:g}, row tree from depth 2, min rows/cols 20",
             "N": n, "plain_s": t_plain, "rank_predicted_s": t_model, "speedup": t_plain / t_model,
             "svds": st["svds"], "sketched": st["sketched"], "fell_back": st["fell_back"],
             "train_samples": int(len(y)), "train_s": t_train, "train_mse": mse, "train_mae": mae,
             "plain_rel_err_vs_rows": errs[0], "rank_predicted_rel_err_vs_rows": errs[1]}
 
 
-def cpu_baseline(n_sample, tol, rank_const, rcond, n_full_entries):
-    """Oracle (single-threaded C restatement) factor + apply on the same
-    geometry at n_sample points; the apply rate is scaled to the full size
-    by the ratio of stored entries (the apply streams every entry once)."""
+def cpu_baseline(n_sample, tol, rank_const, rcond, n_full_entries, threads, seconds=4.0):
+    """Oracle (the C restatement) factorization of the same geometry with the
+    same planner at n_sample points (one thread: the factorization is a
+    serial C loop), then its apply timed as throughput on `threads` host
+    threads, each applying its own right-hand side over and over for
+    `seconds` (the C calls release the GIL, so the threads run on separate
+    cores); the rate is scaled to the full size by the ratio of stored
+    entries (the apply streams every entry once)."""
     import oracle as orc
     src, tgt, k = curve_problem(n_sample)
     t0 = time.perf_counter()
@@ -302,24 +308,34 @@ def cpu_baseline(n_sample, tol, rank_const, rcond, n_full_entries):
     t_fac = time.perf_counter() - t0
     lt, ft, bt = O.plan()
     entries = float((bt[:, 2] * bt[:, 3])[bt[:, 4] != 2].sum() + (lt[:, 3] * lt[:, 4])[lt[:, 0] == 0].sum())
-    x = orc.complex_randn(n_sample)
-    reps, t_app = 0, 0.0
+    xs = [orc.complex_randn(n_sample) for _ in range(threads)]
+    O.apply(xs[0])
+    stop = time.perf_counter() + seconds
+
+    def work(x):
+        c = 0
+        while time.perf_counter() < stop:
+            O.apply(x)
+            c += 1
+        return c
+
     t0 = time.perf_counter()
-    while t_app < 3.0:
-        O.apply(x)
-        reps += 1
-        t_app = time.perf_counter() - t0
-    apply_s = t_app / reps
-    scaled_apply_s = apply_s * n_full_entries / entries
+    with ThreadPoolExecutor(threads) as ex:
+        counts = list(ex.map(work, xs))
+    wall = time.perf_counter() - t0
+    rate = sum(counts) / wall  # applies per second at n_sample, all threads
+    scaled = rate * entries / n_full_entries
     return {
-        "value": 1.0 / scaled_apply_s,
+        "value": scaled,
         "unit": "RHS/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle C restatement, same geometry/planner at N={n_sample} "
-                   f"(factor {t_fac:.1f}s, apply {apply_s * 1e3:.2f} ms over {entries:.3g} entries, {reps} reps), "
-                   f"apply time scaled x{n_full_entries / entries:.0f} by stored entries to N=2^20"),
+        "sample": (f"oracle C restatement, same geometry/planner at N={n_sample} (factor {t_fac:.1f}s on 1 thread, "
+                   f"{entries:.3g} stored entries); apply throughput {rate:.1f} RHS/s on {threads} threads over "
+                   f"{wall:.1f}s ({sum(counts)} applies), scaled x{entries / n_full_entries:.2e} by stored entries "
+                   f"to N=2^20"),
         "factor_s_sample": t_fac,
+        "apply_rhs_per_s_sample": rate,
     }
 
 
@@ -347,11 +363,16 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="independent-problems section size (0 = skip)")
     ap.add_argument("--batch-rhs", type=int, default=16)
     ap.add_argument("--stream-n", type=int, default=8192, help="streaming / rank-model section size (0 = skip)")
+    ap.add_argument("--stream-tol", type=float, default=1e-7)
     ap.add_argument("--factor-warmup", type=int, default=1 << 14,
                     help="points of an untimed factorization + apply run first, so the timed one does not carry "
                          "the process's one-time costs (kernel code loading, first touch); 0 = none")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-n", type=int, default=4096)
+    ap.add_argument("--cpu-sample-n", type=int, default=8192)
+    ap.add_argument("--tune", type=int, default=1,
+                    help="run the apply autotuner (bfhipFacTuneApply) before each timed apply shape (untimed)")
+    ap.add_argument("--graphs", type=int, default=1, help="replay the timed applies as hipGraphs")
+    ap.add_argument("--tune-cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "bf_amd_apply_tune.txt"))
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -365,6 +386,7 @@ def main():
     if bf.lib().bfhipDeviceCount() < 1:
         raise SystemExit("no HIP device")
     bf.lib().bfhipSetApplyCompensation(args.comp)
+    bf.set_apply_graphs(args.graphs)
     n = args.n
     src, tgt, k = curve_problem(n)
     print(f"[bench] N={n} k={k:.1f} leaf=64 tol={args.tol:g} C={args.rank_const} rcond={args.pinv_rcond:g}",
@@ -385,6 +407,12 @@ def main():
     print(f"[bench] factor {factor_s:.2f}s (plan {st['plan_s']:.2f}s, sample {st['sample_ms']:.0f} ms, "
           f"compress {st['compress_ms']:.0f} ms) entries {st['entries']:.3e}", file=sys.stderr, flush=True)
 
+    tune_s = {}
+    if args.tune:
+        t0 = time.perf_counter()
+        F.tune_apply(1, "fp64", args.tune_cache)
+        tune_s["fp64_1rhs"] = time.perf_counter() - t0
+    apply_cfg = {"fp64_1rhs": F.apply_config(1)}
     bf.seed(0)
     x = bf.complex_randn(n)
     dx = bf.DeviceArray.from_numpy(x[ts.perm])
@@ -441,6 +469,11 @@ def main():
         Xt = np.ascontiguousarray(X[ts.perm])
         dX = bf.DeviceArray.from_numpy(Xt)
         dY = bf.DeviceArray(16 * n * nr)
+        if args.tune:
+            t0 = time.perf_counter()
+            F.tune_apply(nr, "fp64", args.tune_cache)
+            tune_s[f"fp64_{nr}rhs"] = time.perf_counter() - t0
+        apply_cfg[f"fp64_{nr}rhs"] = F.apply_config(nr)
         bf.mul_device(F, dX, dY, nr, "fp64")
         bf.lib().bfhipSynchronize()
         t0 = time.perf_counter()
@@ -469,6 +502,13 @@ def main():
             ym[tt.perm] = dy.to_numpy(np.complex128, n)
             errm64_rows = float(np.linalg.norm(ym[rows] - yd) / np.linalg.norm(yd))
         bf.lib().bfhipFacReleaseFp64(Fm.ptr)
+        if args.tune:
+            t0 = time.perf_counter()
+            Fm.tune_apply(nr, "fp32", args.tune_cache)
+            Fm.tune_apply(1, "fp32", args.tune_cache)
+            tune_s["fp32"] = time.perf_counter() - t0
+        apply_cfg[f"fp32_{nr}rhs"] = Fm.apply_config(nr, "fp32")
+        apply_cfg["fp32_1rhs"] = Fm.apply_config(1, "fp32")
         dX32 = bf.DeviceArray.from_numpy(Xt.astype(np.complex64))
         dY32 = bf.DeviceArray(8 * n * nr)
         bf.mul_device(Fm, dX32, dY32, nr, "fp32")
@@ -515,15 +555,21 @@ def main():
 
     bt = None
     if args.batch:
-        bt = batch_section(bf, args.batch, args.batch_rhs, max(1, args.mixed_steps), 256 if args.rows else 0)
+        bt = batch_section(bf, args.batch, args.batch_rhs, max(1, args.mixed_steps), 256 if args.rows else 0,
+                           args.tune_cache if args.tune else None)
 
     sm = None
     if args.stream_n:
-        sm = stream_section(bf, args.stream_n, args.tol, 8, 200, 256 if args.rows else 0, args.threads)
+        try:
+            sm = stream_section(bf, args.stream_n, args.stream_tol, 8, 200, 256 if args.rows else 0, args.threads)
+        except bf.BfError as e:  # reported, not fatal: the headline line must still come out
+            sm = {"error": str(e)}
+            print(f"[bench] stream section failed: {e}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.rank_const, args.pinv_rcond, st["entries"])
+        cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.rank_const, args.pinv_rcond, st["entries"],
+                           args.threads)
         print(f"[bench] cpu baseline {cpu['value']:.3f} RHS/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
     line = {
@@ -544,8 +590,10 @@ def main():
                         "leaf 64, rank tol 1e-8, FP64 factor + apply to 1 complex Gaussian RHS",
             "N": n, "k": k, "leaf_size": 64, "tol": args.tol, "rank_const": args.rank_const,
             "pinv_rcond": args.pinv_rcond,
-            "planner": "pass-through", "nrhs": 1,
+            "planner": "pass-through", "nrhs": 1, "graphs": bool(args.graphs), "tuned": bool(args.tune),
         },
+        "tune_s": tune_s,
+        "apply_config": apply_cfg,
         "factor_s": factor_s,
         "factor_warmup": (f"one untimed N={args.factor_warmup} factorization + apply before the timed one"
                           if args.factor_warmup else None),
