@@ -155,3 +155,36 @@ def test_streamer_exact_for_low_rank_and_partial_span(gpu_lib):
     assert F.factor_shape(0)[1] == 6
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     assert rel(F.mat() @ x, A @ x) < 1e-12
+
+
+def test_streamer_cuts_follow_planted_ranks_on_device(gpu_lib):
+    """The planted-rank matrix of the oracle's pin (test_oracle.py): the
+    device streamer makes the same cuts -- final row cut [1, 9..20] with
+    Psi columns 3 + 16 x 12 -- and reproduces the matrix."""
+    bf = gpu_lib
+    g = (np.arange(16) + 0.5) / 16
+    X, Y = np.meshgrid(g, g)
+    pts = np.stack([X.ravel(), Y.ravel()], 1)
+    tt, ts = bf.Quadtree(pts, 16), bf.Quadtree(pts + [2.0, 0.0], 64)
+    rng = np.random.default_rng(12)
+
+    def cg(m, n):
+        return rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+
+    A = np.zeros((256, 256), complex)
+    A[0:64] = cg(64, 3) @ cg(3, 256)
+    for j in range(4):
+        c = slice(64 * j, 64 * (j + 1))
+        A[64:128, c] = cg(64, 3) @ cg(3, 64) if j == 3 else cg(64, 64)
+        A[128:192, c] = cg(64, 2) @ cg(2, 64)
+        A[192:256, c] = cg(64, 20) @ cg(20, 64)
+    S = bf.FacStreamer(tt, ts, 1e-10, init_depth=1, min_rows=1, min_cols=1)
+    cn = _nodes(ts)
+    while not S.done:
+        c = S.current_column_node()
+        assert S.feed(A[:, cn[c].offset[0]:cn[c].offset[4]])
+    F = S.fac()
+    assert list(F.row_nodes()) == [1] + list(range(9, 21))
+    assert F.factor_shape(0)[:2] == (256, 3 + 16 * 12)
+    x = rng.standard_normal(256) + 1j * rng.standard_normal(256)
+    assert rel(F.mat() @ x, A @ x) < 1e-12

@@ -349,6 +349,65 @@ def test_streamer_exact_for_a_low_rank_matrix():
     assert np.linalg.norm(F.dense() - A) / np.linalg.norm(A) < 1e-13
 
 
+def test_streamer_cuts_follow_planted_ranks():
+    """The row cuts of the streamer (fac_streamer.c:316-447, fac.c:509-573,
+    :867-1050) on a matrix whose block ranks are planted, so the cuts follow
+    from the ranks alone.  Rows: a 16 x 16 grid (quadtree leaf 16: root, four
+    64-row nodes 1-4, sixteen 16-row leaves 5-20); columns: another grid,
+    four 64-column leaves.  Row node 1 has rank 3 over all columns; node 2 is
+    random on column leaves 1-3 and of rank 3 on leaf 4; node 3 has rank 2
+    on each column leaf (independent factors); node 4 rank 20 on each.
+
+    Feeding a leaf (from depth 1): a node whose block has rank below its
+    sides keeps that many terms; a full-rank node splits; a full-rank leaf
+    passes its rows through.  Leaves 1-3 cut [1, 9, 10, 11, 12, 3, 4] with
+    3, 16 x 4, 2 and 20 columns of Psi; leaf 4 cuts [1, 2, 3, 4].  Merging
+    the four: the coarsest common cut takes node 2 from leaf 4 over 9-12 from
+    the others; a merged block compresses only if it drops terms and keeps
+    fewer terms than rows, so node 1 keeps 3 terms, node 2 (rank 64 of 64
+    rows) and node 4 (rank 64 of 80 columns, 64 rows) split into leaves that
+    pass through, and node 3 (64 x 8 of full column rank: nothing dropped)
+    splits too.  Final cut [1, 9..20], Psi columns 3 + 16 x 12."""
+    from oracle.streamer import Streamer, eps_cut
+    g = (np.arange(16) + 0.5) / 16
+    X, Y = np.meshgrid(g, g)
+    rows_pts = np.stack([X.ravel(), Y.ravel()], 1)
+    _, tn, _ = orc.tree(rows_pts, 16)
+    _, sn, _ = orc.tree(rows_pts + [2.0, 0.0], 64)
+    assert [tuple(tn[i, 1:3]) for i in range(1, 5)] == [(0, 64), (64, 128), (128, 192), (192, 256)]
+    assert all(tn[i, 4] < 0 for i in range(5, 21)) and len(sn) == 5
+    rng = np.random.default_rng(12)
+
+    def cg(m, n):
+        return rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+
+    A = np.zeros((256, 256), complex)
+    A[0:64] = cg(64, 3) @ cg(3, 256)
+    for j in range(4):
+        c = slice(64 * j, 64 * (j + 1))
+        A[64:128, c] = cg(64, 3) @ cg(3, 64) if j == 3 else cg(64, 64)
+        A[128:192, c] = cg(64, 2) @ cg(2, 64)
+        A[192:256, c] = cg(64, 20) @ cg(20, 64)
+    S = Streamer(tn, sn, 1e-10, init_depth=1, min_rows=1, min_cols=1)
+    order = []
+    for step in range(3):
+        c = S.current_column_node()
+        order.append(c)
+        S.feed(A[:, S.ct.start(c):S.ct.end(c)])
+        f = S.facs[c]
+        assert f.rowNodes == [1, 9, 10, 11, 12, 3, 4], f.rowNodes
+        assert [e - a for a, e in f.colRanges] == [3, 16, 16, 16, 16, 2, 20]
+    c = S.current_column_node()
+    last = eps_cut(S.rt, S.init, 0, A[:, S.ct.start(c):S.ct.end(c)], S.spec, False)
+    assert [nd for nd, _, _ in last] == [1, 2, 3, 4] and [P.shape[1] for _, P, _ in last] == [3, 3, 2, 20]
+    S.feed(A[:, S.ct.start(c):S.ct.end(c)])
+    assert S.done and sorted(order + [c]) == [1, 2, 3, 4]
+    F = S.fac()
+    assert F.rowNodes == [1] + list(range(9, 21)), F.rowNodes
+    assert [e - a for a, e in F.colRanges] == [3] + [16] * 12
+    assert np.linalg.norm(F.dense() - A) / np.linalg.norm(A) < 1e-12
+
+
 def test_streamer_reproduces_a_kernel_matrix_to_its_tolerance():
     """Helmholtz kernel between separated boxes (k = 25): the streamed
     product is within a small multiple of tol of the matrix and stores far
