@@ -1967,6 +1967,20 @@ struct MfmaF64 {
   }
   static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
 };
+/* FP32 storage (factors and vectors), FP64 arithmetic on the matrix cores:
+ * operands widened on load, sums kept in FP64, one rounding to FP32 per
+ * output.  On MI355X the FP64 and FP32 matrix rates differ by 2x only, and
+ * the many-RHS apply is bound by its vector traffic, which FP32 storage
+ * halves either way. */
+struct MfmaF64S32 {
+  typedef double S;
+  typedef float2 V;
+  typedef mf_d4 Acc;
+  static __device__ __forceinline__ Acc mma(S a, S b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
+};
 struct MfmaF32 {
   typedef float S;
   typedef float2 V;
@@ -2104,6 +2118,202 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
   }
 }
 
+/* The same stage with the operand loads software-pipelined.  A wave walks
+ * the 4-column steps of all the segment's block terms as one sequence and
+ * keeps MFMA_RING steps of operands in registers: the loads of the next
+ * MFMA_RING - 1 steps are in flight while a step's MFMAs run, so a wave
+ * waits on memory once per row tile instead of once per step.  Loads
+ * never branch: rows past the segment and columns past nrhs read a
+ * clamped in-bounds address and only feed output entries that are not
+ * stored, and the columns of a block's last step past its width read the
+ * block's last column and are zeroed before use. */
+#define MFMA_RING 3
+template <class M, int CT, bool LO, bool COMP = false>
+__global__ __launch_bounds__(64) void k_applyN_mfma_p(const BfkSeg *__restrict__ segs,
+                                                      const BfkTerm *__restrict__ terms,
+                                                      const typename M::V *__restrict__ hi,
+                                                      const typename M::V *__restrict__ lo,
+                                                      typename BufsOf<typename M::V>::T bufs, int nrhs,
+                                                      int numChunks, int64_t numSegs) {
+  typedef typename M::V V;
+  typedef typename M::Acc Acc;
+  const int64_t slot = blockIdx.x >> 3;
+  const int64_t per = (int64_t)MFMA_SEG_GROUP * numChunks;
+  const int64_t grp = slot / per;
+  const int64_t within = slot - grp * per;
+  const int64_t si = (grp * 8 + (blockIdx.x & 7)) * MFMA_SEG_GROUP + within / numChunks;
+  const int chunk = (int)(within - (within / numChunks) * numChunks);
+  if (si >= numSegs) return;
+  const BfkSeg Sg = segs[si];
+  const int l = threadIdx.x;
+  const int c0 = chunk * 16 * CT;
+  const int m = Sg.m;
+  const int lr = l & 15, lk = l >> 4;
+  V *y = bufs.out[Sg.ybuf];
+  int steps = 0; /* 4-column steps over the block terms */
+  for (int t = Sg.t0; t < Sg.t1; ++t) {
+    const BfkTerm T = terms[t];
+    if (T.aOff >= 0) steps += (T.n + 3) >> 2;
+  }
+  int colc[CT];
+  for (int q = 0; q < CT; ++q) colc[q] = min(c0 + 16 * q + lr, nrhs - 1);
+  for (int rt = 0; rt < m; rt += 16) {
+    Acc re[CT], im[CT], pre[CT], pim[CT];
+    mf_d4 re64[CT], im64[CT];
+    for (int q = 0; q < CT; ++q)
+      for (int r = 0; r < 4; ++r) {
+        re[q][r] = im[q][r] = pre[q][r] = pim[q][r] = 0;
+        re64[q][r] = im64[q][r] = 0;
+      }
+    for (int t = Sg.t0; t < Sg.t1; ++t) { /* identity terms */
+      const BfkTerm T = terms[t];
+      if (T.aOff >= 0) continue;
+      const V *x = bufs.in[T.xbuf];
+      for (int q = 0; q < CT; ++q) {
+        const int col = c0 + 16 * q + lr;
+        for (int r = 0; r < 4; ++r) {
+          const int i = rt + M::row(l, r);
+          if (i < m && col < nrhs) {
+            V v = x[(T.xOff + i) * (int64_t)nrhs + col];
+            if constexpr (COMP) {
+              pre[q][r] += v.x;
+              pim[q][r] += v.y;
+            } else {
+              re[q][r] += v.x;
+              im[q][r] += v.y;
+            }
+          }
+        }
+      }
+      if constexpr (COMP)
+        for (int q = 0; q < CT; ++q) {
+          fold64(re64[q], pre[q]);
+          fold64(im64[q], pim[q]);
+        }
+    }
+    if (steps > 0) {
+      const int arow = min(rt + lr, m - 1);
+      /* the load cursor: term lt, its columns from lk0 */
+      int lt = Sg.t0;
+      while (terms[lt].aOff < 0) ++lt;
+      BfkTerm LT = terms[lt];
+      const V *LX = bufs.in[LT.xbuf];
+      int lk0 = 0, issued = 0;
+      V ra[MFMA_RING], rl[MFMA_RING], rb[MFMA_RING][CT];
+      int sk0[MFMA_RING], sn[MFMA_RING];
+      auto issue = [&](int s) {
+        const int kk = min(lk0 + lk, LT.n - 1);
+        const int64_t ao = LT.aOff + LT.rowOff + arow + (int64_t)kk * LT.lda;
+        ra[s] = hi[ao];
+        if (LO) rl[s] = lo[ao];
+        const V *x = LX + (LT.xOff + kk) * (int64_t)nrhs;
+        for (int q = 0; q < CT; ++q) rb[s][q] = x[colc[q]];
+        sk0[s] = lk0;
+        sn[s] = LT.n;
+        if (++issued < steps) { /* past the last step the cursor stays put */
+          lk0 += 4;
+          if (lk0 >= LT.n) {
+            lk0 = 0;
+            do ++lt;
+            while (terms[lt].aOff < 0);
+            LT = terms[lt];
+            LX = bufs.in[LT.xbuf];
+          }
+        }
+      };
+      auto compute = [&](int s) {
+        V a = ra[s], al = rl[s];
+        V b[CT];
+        for (int q = 0; q < CT; ++q) b[q] = rb[s][q];
+        const int rem = sn[s] - sk0[s];
+        if (rem < 4 && lk >= rem) {
+          a = czero<V>();
+          if (LO) al = czero<V>();
+          for (int q = 0; q < CT; ++q) b[q] = czero<V>();
+        }
+        for (int q = 0; q < CT; ++q) {
+          Acc &dr = COMP ? pre[q] : re[q];
+          Acc &di = COMP ? pim[q] : im[q];
+          dr = M::mma(a.x, b[q].x, dr);
+          dr = M::mma(-a.y, b[q].y, dr);
+          di = M::mma(a.x, b[q].y, di);
+          di = M::mma(a.y, b[q].x, di);
+          if (LO) {
+            dr = M::mma(al.x, b[q].x, dr);
+            dr = M::mma(-al.y, b[q].y, dr);
+            di = M::mma(al.x, b[q].y, di);
+            di = M::mma(al.y, b[q].x, di);
+          }
+        }
+        if constexpr (COMP) {
+          if (((sk0[s] >> 2) & g_foldMaskDev) == g_foldMaskDev || rem <= 4)
+            for (int q = 0; q < CT; ++q) {
+              fold64(re64[q], pre[q]);
+              fold64(im64[q], pim[q]);
+            }
+        }
+      };
+#pragma unroll
+      for (int s = 0; s < MFMA_RING - 1; ++s) issue(s);
+      /* whole turns of the ring, then the last steps (fewer than the
+       * ring holds, already in flight): no exit inside a turn, so the loop
+       * head sees the ring's loads in their steady-state order */
+      const int turns = steps / MFMA_RING, tail = steps - turns * MFMA_RING;
+      for (int u = 0; u < turns; ++u) {
+#pragma unroll
+        for (int s = 0; s < MFMA_RING; ++s) {
+          issue((s + MFMA_RING - 1) % MFMA_RING);
+          compute(s);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < MFMA_RING - 1; ++s)
+        if (s < tail) compute(s);
+    }
+    for (int q = 0; q < CT; ++q) {
+      const int col = c0 + 16 * q + lr;
+      for (int r = 0; r < 4; ++r) {
+        const int i = rt + M::row(l, r);
+        if (i < m && col < nrhs) {
+          V v;
+          if constexpr (COMP) {
+            v.x = (decltype(v.x))re64[q][r];
+            v.y = (decltype(v.y))im64[q][r];
+          } else {
+            v.x = (decltype(v.x))re[q][r];
+            v.y = (decltype(v.y))im[q][r];
+          }
+          y[(Sg.yOff + i) * (int64_t)nrhs + col] = v;
+        }
+      }
+    }
+  }
+}
+
+/* the matrix-core kernel of one many-RHS stage with `ct` tiles of 16
+ * right-hand sides per wave (1, 2, 4 or 8) */
+static int g_mfmaPipe = 1; /* 1: k_applyN_mfma_p, 0: k_applyN_mfma */
+template <class M, bool LO, bool COMP>
+static void launch_mfma(int ct, dim3 grid, const BfkSeg *segs, const BfkTerm *terms, const typename M::V *H,
+                        const typename M::V *Lo, typename BufsOf<typename M::V>::T b, int nrhs, int chunks,
+                        int64_t numSegs) {
+#define BFK_MFMA_LAUNCH(K, C)                                                                                   \
+  hipLaunchKernelGGL((K<M, C, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b, nrhs, chunks, numSegs)
+  if (g_mfmaPipe) {
+    if (ct == 1) BFK_MFMA_LAUNCH(k_applyN_mfma_p, 1);
+    else if (ct == 2) BFK_MFMA_LAUNCH(k_applyN_mfma_p, 2);
+    else if (ct == 8) BFK_MFMA_LAUNCH(k_applyN_mfma_p, 8);
+    else BFK_MFMA_LAUNCH(k_applyN_mfma_p, 4);
+  } else {
+    if (ct == 1) BFK_MFMA_LAUNCH(k_applyN_mfma, 1);
+    else if (ct == 2) BFK_MFMA_LAUNCH(k_applyN_mfma, 2);
+    else if (ct == 8) BFK_MFMA_LAUNCH(k_applyN_mfma, 8);
+    else BFK_MFMA_LAUNCH(k_applyN_mfma, 4);
+  }
+#undef BFK_MFMA_LAUNCH
+}
+
+
 /* ------------------------------------------------------- plain copy */
 
 /* a streaming copy of known size: the reference point for the HBM byte
@@ -2131,6 +2341,7 @@ extern "C" {
 
 int bfk_init(void) {
   if (g_stream) return 0;
+  if (const char *e = getenv("BF_AMD_MFMA_PIPE")) g_mfmaPipe = atoi(e);
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
     fprintf(stderr, "bf_kernels: no HIP device available\n");
@@ -2703,18 +2914,7 @@ static int launch_stage_f64(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
     const int ct = cfg >= 0 ? cfg : g_mfmaCT;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));
-    if (ct == 1)
-      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 1, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
-                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
-    else if (ct == 2)
-      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 2, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
-                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
-    else if (ct == 8)
-      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 8, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
-                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
-    else
-      hipLaunchKernelGGL((k_applyN_mfma<MfmaF64, 4, false>), grid, dim3(64), 0, g_stream, segs, dTerms,
-                         (const double2 *)dArena, (const double2 *)nullptr, b, nrhs, chunks, n);
+    launch_mfma<MfmaF64, false, false>(ct, grid, segs, dTerms, (const double2 *)dArena, nullptr, b, nrhs, chunks, n);
   } else {
     hipLaunchKernelGGL(k_applyN, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream, segs,
                        dTerms, (const double2 *)dArena, b, nrhs);
@@ -2808,7 +3008,7 @@ void bfk_graph_destroy(void *exec) {
 /* the global launch settings as one number: a captured apply is stale once
  * it changes */
 int bfk_apply_config_stamp(void) {
-  return g_applyWaves | g_applyUnroll << 4 | g_mfmaCT << 5 | g_comp32 << 9 | g_foldMask << 10;
+  return g_applyWaves | g_applyUnroll << 4 | g_mfmaCT << 5 | g_comp32 << 9 | g_foldMask << 13;
 }
 
 int bfk_copy(const void *src, void *dst, size_t bytes) {
@@ -2920,29 +3120,19 @@ static void launch_apply1_f32(dim3 grid, dim3 block, const BfkSeg *segs, int64_t
   hipLaunchKernelGGL((k_apply1<float2, LO, false, COMP>), grid, block, 0, g_stream, segs, n, terms, H, Lo, b);
 }
 
-template <bool LO, bool COMP>
-static void launch_mfma_f32(int ct, dim3 grid, const BfkSeg *segs, const BfkTerm *terms, const float2 *H,
-                            const float2 *Lo, BfkBufsF b, int nrhs, int chunks, int64_t numSegs) {
-  if (ct == 1)
-    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 1, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
-                       nrhs, chunks, numSegs);
-  else if (ct == 2)
-    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 2, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
-                       nrhs, chunks, numSegs);
-  else if (ct == 8)
-    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 8, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
-                       nrhs, chunks, numSegs);
-  else
-    hipLaunchKernelGGL((k_applyN_mfma<MfmaF32, 4, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b,
-                       nrhs, chunks, numSegs);
-}
+
 
 extern "C" {
 
+/* accumulation of the FP32 applies: 0 plain FP32; 1 (the default) FP64
+ * sums -- the one-RHS kernel sums FP32 products in FP64, the matrix-core
+ * kernel widens its FP32 operands and runs the FP64 matrix cores; 2 + mask
+ * the matrix-core kernel accumulates FP32 tiles and folds them into FP64
+ * sums every (mask + 1) 4-column steps (mask 0, 1, 3 or 7) */
 int bfk_set_compensation(int on) {
-  g_comp32 = on ? 1 : 0;
-  int mask = on >= 2 ? on - 2 : 0; /* on = 2 + mask selects a longer fold interval */
-  if (mask != 0 && mask != 1 && mask != 3 && mask != 7) return -1;
+  const int mask = on >= 2 ? on - 2 : 0;
+  if (on < 0 || (mask != 0 && mask != 1 && mask != 3 && mask != 7)) return -1;
+  g_comp32 = on;
   if (mask != g_foldMask) {
     if (bfk_init()) return -1;
     BFK_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_foldMaskDev), &mask, sizeof(int)));
@@ -2982,12 +3172,14 @@ static int launch_stage_f32(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
     const int ct = cfg >= 0 ? cfg : g_mfmaCT;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));
-    if (Lo) {
-      if (g_comp32) launch_mfma_f32<true, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
-      else launch_mfma_f32<true, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+    if (g_comp32 == 1 && !Lo) {
+      launch_mfma<MfmaF64S32, false, false>(ct, grid, segs, dTerms, H, nullptr, b, nrhs, chunks, n);
+    } else if (Lo) {
+      if (g_comp32) launch_mfma<MfmaF32, true, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+      else launch_mfma<MfmaF32, true, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
     } else {
-      if (g_comp32) launch_mfma_f32<false, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
-      else launch_mfma_f32<false, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+      if (g_comp32) launch_mfma<MfmaF32, false, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+      else launch_mfma<MfmaF32, false, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
     }
   } else {
     hipLaunchKernelGGL(k_applyN_f32, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream, segs,

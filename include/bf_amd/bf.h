@@ -510,10 +510,13 @@ int bfhipFacTuneApply(BfMat *fac, BfSize nrhs, BfhipPrecision prec, char const *
  * `prec` (-1 = the global settings); returns the number of stages */
 int bfhipFacGetApplyConfig(BfMat const *fac, BfSize nrhs, BfhipPrecision prec, int32_t *stageCfg);
 
-/* new: compensated accumulation in the FP32 applies, on (1, the default)
- * or off (0): FP32 products summed in FP64 (1 RHS) or FP32 matrix-core
- * tiles folded into FP64 sums (many RHS), so the FP32 rounding of a stage
- * stays near one unit in the last place however long its rows are */
+/* new: accumulation in the FP32 applies (factors and vectors stored in
+ * FP32).  0: FP32 throughout.  1 (the default): FP64 sums -- one RHS: FP32
+ * products summed in FP64; many RHS: the FP32 operands widened to FP64 on
+ * load and multiplied on the FP64 matrix cores -- so the rounding of a
+ * stage is one FP32 rounding of its output however long its rows are.
+ * 2 + m (m = 0, 1, 3 or 7): many RHS accumulate FP32 matrix-core tiles and
+ * fold them into FP64 sums every m + 1 four-column steps. */
 int bfhipSetApplyCompensation(int on);
 
 /* new: plan export for parity checks (integers, in the oracle's canonical
