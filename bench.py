@@ -351,9 +351,9 @@ def main():
     ap.add_argument("--mixed-tol", type=float, default=1e-8)
     ap.add_argument("--mixed-rcond", type=float, default=1e-11)
     ap.add_argument("--mixed-rank-const", type=float, default=0.0, help="0 = same as --rank-const")
-    ap.add_argument("--comp", type=int, default=1,
+    ap.add_argument("--comp", type=int, default=2,
                     help="accumulation of the FP32 applies (bfhipSetApplyCompensation: 0 FP32, 1 FP64 sums, "
-                         "2 + m folded FP32 tiles)")
+                         "2 + m FP32 matrix-core tiles folded into FP64 sums every m + 1 steps)")
     ap.add_argument("--rows", type=int, default=4096, help="exact dense rows for the error check (0 = skip)")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--mixed-rhs", type=int, default=256)

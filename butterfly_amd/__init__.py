@@ -139,6 +139,7 @@ def lib():
         "bfPoints2DeinitAndDealloc": (None, [C.POINTER(vp)]),
         "bfQuadtreeNew": (vp, []),
         "bfQuadtreeInitWithLeafSize": (None, [vp, vp, vp, sz]),
+        "bfhipQuadtreeInitWithLeafSize": (None, [vp, vp, vp, sz]),
         "bfQuadtreeDeinitAndDealloc": (None, [C.POINTER(vp)]),
         "bfHelm2RankEstForTwoCircles": (sz, [vp, vp, vp, C.c_double, C.c_double]),
         "bfHelm2GetKernelMatrix": (vp, [vp, vp, vp, vp, vp]),
@@ -326,10 +327,11 @@ class Helm2:
 class Quadtree:
     """bfQuadtreeInit over an (N, 2) array; keeps the array alive."""
 
-    def __init__(self, points, leaf_size=1, normals=None):
+    def __init__(self, points, leaf_size=1, normals=None, device=False):
         """`normals`: unit normals of the points (original order), needed by
         the double layer and the combined field on a source tree and by S' on
-        a target tree."""
+        a target tree.  `device`: build the tree on the GPU
+        (bfhipQuadtreeInitWithLeafSize)."""
         self.points = np.ascontiguousarray(points, dtype=np.float64)
         L = lib()
         self._pts = L.bfPoints2NewViewFromPtr(_dp(self.points), len(self.points))
@@ -340,8 +342,8 @@ class Quadtree:
                 raise ValueError("one unit normal per point")
             self._nrm = Vectors2Struct(self.normals.ctypes.data, len(self.normals), len(self.normals), True)
         self.ptr = L.bfQuadtreeNew()
-        L.bfQuadtreeInitWithLeafSize(self.ptr, self._pts, C.byref(self._nrm) if self._nrm is not None else None,
-                                     leaf_size)
+        init = L.bfhipQuadtreeInitWithLeafSize if device else L.bfQuadtreeInitWithLeafSize
+        init(self.ptr, self._pts, C.byref(self._nrm) if self._nrm is not None else None, leaf_size)
         check()
 
     @property

@@ -14,6 +14,11 @@ int bfk_device_count(void);
 int bfk_scratch(void **ptr, size_t *bytes);
 size_t bfk_lds_cap(void);
 void *bfk_stream_handle(void);
+/* the quadtree on the GPU (bf_tree.hip) */
+int bfk_quadtree(int64_t n, const double *xy, int64_t leafSize,
+                 void (*rootBox)(const double mn[2], const double mx[2], double outMin[2], double outMax[2]),
+                 BfQuadtreeNode **nodesOut, int64_t *numNodesOut, int64_t *levelCount, int *numLevelsOut,
+                 BfSize *perm);
 /* rank model and randomized range finder (bf_rank.hip) */
 int bfk_mlp_nparam(void);
 int bfk_mlp_groups(int B);

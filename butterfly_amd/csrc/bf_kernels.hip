@@ -20,6 +20,7 @@
  * launch.  Everything here is FP64 unless the name says f32.
  */
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -1767,10 +1768,12 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
  * bfk_set_apply_config. */
 static int g_applyWaves = 1;
 static int g_applyUnroll = 0;
-/* right-hand-side tiles of 16 per wave in the MFMA apply (0 = off) */
-static int g_mfmaCT = 1;
-/* compensated accumulation in the FP32 applies (1 = on) */
-static int g_comp32 = 1;
+/* right-hand-side tiles of 16 per wave in the MFMA apply (0 = off, -1 =
+ * 4 in FP64 and 2 in FP32, the shapes the tuner picks at N = 2^20) */
+static int g_mfmaCT = -1;
+/* accumulation in the FP32 applies (bfk_set_compensation): 2 = FP32
+ * matrix-core tiles folded into FP64 sums */
+static int g_comp32 = 2;
 /* the MFMA apply folds its partial FP32 tile into the FP64 sum every
  * (mask + 1) 4-column steps */
 static int g_foldMask = 0;
@@ -1786,12 +1789,15 @@ __constant__ int g_foldMaskDev = 0;
  * factors are an FP32 pair hi + lo.  With COMP (FP32 only) every lane
  * sums its FP32 products in FP64 and the lanes of a row meet in FP64, so
  * the rounding of a stage no longer grows with the length of its rows. */
+#define APPLY1_BATCH 4
 template <typename V, bool LO, bool UNROLL, bool COMP = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
                                                 const BfkTerm *__restrict__ terms, const V *__restrict__ hi,
                                                 const V *__restrict__ lo, typename BufsOf<V>::T bufs) {
   const int lane = threadIdx.x & 63;
-  const int64_t si = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  /* the wave's segment index as a wave-uniform value, so the segment and
+   * term descriptors come through scalar loads */
+  const int64_t si = (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (si >= numSegs) return; /* whole wave; no block barrier below */
   const BfkSeg S = segs[si];
   const int m = S.m;
@@ -1824,25 +1830,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
           const V *A = hi + T.aOff + T.rowOff + i;
           const V *B = lo + T.aOff + T.rowOff + i;
           const int n = T.n;
-          int j = gc;
-          for (; j + Gc < n; j += 2 * Gc) {
-            const V x0 = x[j], x1 = x[j + Gc];
-            V p0 = cmulf(A[(int64_t)j * lda], x0), p1 = cmulf(A[(int64_t)(j + Gc) * lda], x1);
-            if (LO) {
-              p0 = cfmaf(B[(int64_t)j * lda], x0, p0);
-              p1 = cfmaf(B[(int64_t)(j + Gc) * lda], x1, p1);
+          /* a lane's columns j = gc, gc + Gc, ...: APPLY1_BATCH of them
+           * loaded before any is used (past the end a clamped, unused
+           * load), the products alternating between two FP64 chains */
+          for (int j0 = gc; j0 < n; j0 += APPLY1_BATCH * Gc) {
+            V xa[APPLY1_BATCH], aa[APPLY1_BATCH], la[APPLY1_BATCH];
+#pragma unroll
+            for (int u = 0; u < APPLY1_BATCH; ++u) {
+              const int j = min(j0 + u * Gc, n - 1);
+              xa[u] = x[j];
+              aa[u] = A[(int64_t)j * lda];
+              if (LO) la[u] = B[(int64_t)j * lda];
             }
-            sx0 += (double)p0.x;
-            sy0 += (double)p0.y;
-            sx1 += (double)p1.x;
-            sy1 += (double)p1.y;
-          }
-          if (j < n) {
-            const V x0 = x[j];
-            V p0 = cmulf(A[(int64_t)j * lda], x0);
-            if (LO) p0 = cfmaf(B[(int64_t)j * lda], x0, p0);
-            sx0 += (double)p0.x;
-            sy0 += (double)p0.y;
+#pragma unroll
+            for (int u = 0; u < APPLY1_BATCH; ++u) { /* a select, not a branch: the loads stay above */
+              V p = cmulf(aa[u], xa[u]);
+              if (LO) p = cfmaf(la[u], xa[u], p);
+              const bool ok = j0 + u * Gc < n;
+              const double px = ok ? (double)p.x : 0.0, py = ok ? (double)p.y : 0.0;
+              if (u & 1) {
+                sx1 += px;
+                sy1 += py;
+              } else {
+                sx0 += px;
+                sy0 += py;
+              }
+            }
           }
         }
       }
@@ -1884,10 +1897,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         const int64_t lda = T.lda;
         const V *A = hi + T.aOff + T.rowOff + i;
         const int n = T.n;
-        for (int j = gt; j < n; j += Gt) {
-          V xj = x[j];
-          a0 = cfmaT(A[(int64_t)j * lda], xj, a0);
-          if (LO) a0 = cfmaT(lo[T.aOff + T.rowOff + i + (int64_t)j * lda], xj, a0);
+        const V *L = lo + T.aOff + T.rowOff + i;
+        /* a lane's columns j = gt, gt + Gt, ...: APPLY1_BATCH of them
+         * loaded before any is used (past the end a clamped, unused load) */
+        for (int j0 = gt; j0 < n; j0 += APPLY1_BATCH * Gt) {
+          V xa[APPLY1_BATCH], aa[APPLY1_BATCH], la[APPLY1_BATCH];
+#pragma unroll
+          for (int u = 0; u < APPLY1_BATCH; ++u) {
+            const int j = min(j0 + u * Gt, n - 1);
+            xa[u] = x[j];
+            aa[u] = A[(int64_t)j * lda];
+            if (LO) la[u] = L[(int64_t)j * lda];
+          }
+#pragma unroll
+          for (int u = 0; u < APPLY1_BATCH; ++u) { /* a select, not a branch: the loads stay above */
+            V t = cfmaT(aa[u], xa[u], a0);
+            if (LO) t = cfmaT(la[u], xa[u], t);
+            const bool ok = j0 + u * Gt < n;
+            a0.x = ok ? t.x : a0.x;
+            a0.y = ok ? t.y : a0.y;
+          }
         }
       }
     }
@@ -2231,25 +2260,34 @@ __global__ __launch_bounds__(64) void k_applyN_mfma_p(const BfkSeg *__restrict__
           if (LO) al = czero<V>();
           for (int q = 0; q < CT; ++q) b[q] = czero<V>();
         }
-        for (int q = 0; q < CT; ++q) {
-          Acc &dr = COMP ? pre[q] : re[q];
-          Acc &di = COMP ? pim[q] : im[q];
-          dr = M::mma(a.x, b[q].x, dr);
-          dr = M::mma(-a.y, b[q].y, dr);
-          di = M::mma(a.x, b[q].y, di);
-          di = M::mma(a.y, b[q].x, di);
-          if (LO) {
-            dr = M::mma(al.x, b[q].x, dr);
-            dr = M::mma(-al.y, b[q].y, dr);
-            di = M::mma(al.x, b[q].y, di);
-            di = M::mma(al.y, b[q].x, di);
+        /* with COMP a fold group's first step starts its FP32 tiles from
+         * zero instead of clearing them after the fold */
+        auto step = [&](auto freshTag) { /* straight-line MFMAs for either case */
+          constexpr bool FRESH = decltype(freshTag)::value;
+          for (int q = 0; q < CT; ++q) {
+            Acc &dr = COMP ? pre[q] : re[q];
+            Acc &di = COMP ? pim[q] : im[q];
+            dr = M::mma(a.x, b[q].x, FRESH ? Acc{} : dr);
+            di = M::mma(a.x, b[q].y, FRESH ? Acc{} : di);
+            dr = M::mma(-a.y, b[q].y, dr);
+            di = M::mma(a.y, b[q].x, di);
+            if (LO) {
+              dr = M::mma(al.x, b[q].x, dr);
+              dr = M::mma(-al.y, b[q].y, dr);
+              di = M::mma(al.x, b[q].y, di);
+              di = M::mma(al.y, b[q].x, di);
+            }
           }
-        }
+        };
+        if (COMP && ((sk0[s] >> 2) & g_foldMaskDev) == 0)
+          step(std::true_type{});
+        else
+          step(std::false_type{});
         if constexpr (COMP) {
           if (((sk0[s] >> 2) & g_foldMaskDev) == g_foldMaskDev || rem <= 4)
             for (int q = 0; q < CT; ++q) {
-              fold64(re64[q], pre[q]);
-              fold64(im64[q], pim[q]);
+              re64[q] += __builtin_convertvector(pre[q], mf_d4);
+              im64[q] += __builtin_convertvector(pim[q], mf_d4);
             }
         }
       };
@@ -2514,7 +2552,8 @@ void bfk_get_time_sums(double sums[8], double counts[8]) {
 
 int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles) {
   if (wavesPerBlock != 1 && wavesPerBlock != 2 && wavesPerBlock != 4) return -1;
-  if (mfmaTiles != 0 && mfmaTiles != 1 && mfmaTiles != 2 && mfmaTiles != 4 && mfmaTiles != 8) return -1;
+  if (mfmaTiles != -1 && mfmaTiles != 0 && mfmaTiles != 1 && mfmaTiles != 2 && mfmaTiles != 4 && mfmaTiles != 8)
+    return -1;
   g_applyWaves = wavesPerBlock;
   g_applyUnroll = unroll ? 1 : 0;
   g_mfmaCT = mfmaTiles;
@@ -2910,8 +2949,8 @@ static int launch_stage_f64(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
     else
       hipLaunchKernelGGL((k_apply1<double2, false, false>), grid, block, 0, g_stream, segs, n, dTerms,
                          (const double2 *)dArena, (const double2 *)nullptr, b);
-  } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT) > 0) {
-    const int ct = cfg >= 0 ? cfg : g_mfmaCT;
+  } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT != 0)) {
+    const int ct = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : 4;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));
     launch_mfma<MfmaF64, false, false>(ct, grid, segs, dTerms, (const double2 *)dArena, nullptr, b, nrhs, chunks, n);
@@ -3008,7 +3047,7 @@ void bfk_graph_destroy(void *exec) {
 /* the global launch settings as one number: a captured apply is stale once
  * it changes */
 int bfk_apply_config_stamp(void) {
-  return g_applyWaves | g_applyUnroll << 4 | g_mfmaCT << 5 | g_comp32 << 9 | g_foldMask << 13;
+  return g_applyWaves | g_applyUnroll << 4 | (g_mfmaCT & 15) << 5 | g_comp32 << 9 | g_foldMask << 13;
 }
 
 int bfk_copy(const void *src, void *dst, size_t bytes) {
@@ -3124,11 +3163,12 @@ static void launch_apply1_f32(dim3 grid, dim3 block, const BfkSeg *segs, int64_t
 
 extern "C" {
 
-/* accumulation of the FP32 applies: 0 plain FP32; 1 (the default) FP64
- * sums -- the one-RHS kernel sums FP32 products in FP64, the matrix-core
- * kernel widens its FP32 operands and runs the FP64 matrix cores; 2 + mask
- * the matrix-core kernel accumulates FP32 tiles and folds them into FP64
- * sums every (mask + 1) 4-column steps (mask 0, 1, 3 or 7) */
+/* accumulation of the FP32 applies: 0 plain FP32; 1 FP64 sums -- the
+ * one-RHS kernel sums FP32 products in FP64, the matrix-core kernel widens
+ * its FP32 operands and runs the FP64 matrix cores; 2 + mask (2 is the
+ * default) the one-RHS kernel as 1, the matrix-core kernel accumulates
+ * FP32 tiles and folds them into FP64 sums every (mask + 1) 4-column steps
+ * (mask 0, 1, 3 or 7) */
 int bfk_set_compensation(int on) {
   const int mask = on >= 2 ? on - 2 : 0;
   if (on < 0 || (mask != 0 && mask != 1 && mask != 3 && mask != 7)) return -1;
@@ -3168,8 +3208,8 @@ static int launch_stage_f32(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
     } else {
       launch_apply1_f32<false, false>(grid, block, segs, n, dTerms, H, Lo, b);
     }
-  } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT) > 0) {
-    const int ct = cfg >= 0 ? cfg : g_mfmaCT;
+  } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT != 0)) {
+    const int ct = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : 2;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));
     if (g_comp32 == 1 && !Lo) {

@@ -198,6 +198,26 @@ static inline int quadrant_of(const BfReal *pt, const BfReal *split) {
 
 typedef BF_VEC(BfQuadtreeNode) NodeVec;
 
+/* The root box: the points' bounding box (bmn, bmx) stretched to a square
+ * about its centre (quadtree_node.c:284-306, bbox.c:18-30).  Both tree
+ * builders call this, so their boxes agree bit for bit. */
+static void root_square(const BfReal bmn[2], const BfReal bmx[2], BfReal mn[2], BfReal mx[2]) {
+  mn[0] = bmn[0];
+  mn[1] = bmn[1];
+  mx[0] = bmx[0];
+  mx[1] = bmx[1];
+  BfReal w = mx[0] - mn[0], h = mx[1] - mn[1];
+  if (w > h) {
+    BfReal c = (mn[1] + mx[1]) / 2;
+    mn[1] = w * (mn[1] - c) / h + c;
+    mx[1] = w * (mx[1] - c) / h + c;
+  } else {
+    BfReal c = (mn[0] + mx[0]) / 2;
+    mn[0] = h * (mn[0] - c) / w + c;
+    mx[0] = h * (mx[0] - c) / w + c;
+  }
+}
+
 typedef struct {
   NodeVec nodes;
   BfPoint2 const *pts;
@@ -398,24 +418,13 @@ void bfQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points, BfVec
   b.topLevels = n >= ((BfSize)1 << 16) ? 2 : 0;
   for (BfSize i = 0; i < n; ++i) b.perm[i] = i;
 
-  /* root box: bounding box stretched to a square about its centre
-   * (quadtree_node.c:284-306, bbox.c:18-30) */
-  BfReal mn[2] = {INFINITY, INFINITY}, mx[2] = {-INFINITY, -INFINITY};
+  BfReal bmn[2] = {INFINITY, INFINITY}, bmx[2] = {-INFINITY, -INFINITY}, mn[2], mx[2];
   for (BfSize i = 0; i < n; ++i)
     for (int d = 0; d < 2; ++d) {
-      mn[d] = fmin(mn[d], b.pts[i][d]);
-      mx[d] = fmax(mx[d], b.pts[i][d]);
+      bmn[d] = fmin(bmn[d], b.pts[i][d]);
+      bmx[d] = fmax(bmx[d], b.pts[i][d]);
     }
-  BfReal w = mx[0] - mn[0], h = mx[1] - mn[1];
-  if (w > h) {
-    BfReal c = (mn[1] + mx[1]) / 2;
-    mn[1] = w * (mn[1] - c) / h + c;
-    mx[1] = w * (mx[1] - c) / h + c;
-  } else {
-    BfReal c = (mn[0] + mx[0]) / 2;
-    mn[0] = h * (mn[0] - c) / w + c;
-    mx[0] = h * (mx[0] - c) / w + c;
-  }
+  root_square(bmn, bmx, mn, mx);
   BfQuadtreeNode root;
   memset(&root, 0, sizeof(root));
   root.bboxMin[0] = mn[0]; root.bboxMin[1] = mn[1];
@@ -473,6 +482,62 @@ void bfQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points, BfVec
   tree->super.root = nodes;
   tree->super.perm = calloc(1, sizeof(BfPerm));
   tree->super.perm->index = b.perm;
+  tree->super.perm->size = n;
+  tree_register(tree);
+}
+
+/* The same tree built on the GPU (bf_tree.hip: Morton keys, a radix sort
+ * and the node tables level by level).  The nodes, boxes and point ranges
+ * are those of the builder above; the point order inside a leaf is the
+ * key order, which is the reference's order when every leaf holds one
+ * point (leaf size 1). */
+void bfhipQuadtreeInitWithLeafSize(BfQuadtree *tree, BfPoints2 const *points, BfVectors2 const *unitNormals,
+                                   BfSize leafSize) {
+  memset(tree, 0, sizeof(*tree));
+  BfSize n = points ? points->size : 0;
+  if (n < 2 || leafSize < 1 || n > (BfSize)INT32_MAX) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return;
+  }
+  BfSize *perm = malloc(n * sizeof(BfSize));
+  if (!perm) {
+    bfSetError(BF_ERROR_MEMORY_ERROR);
+    return;
+  }
+  BfQuadtreeNode *nodes = NULL;
+  int64_t numNodes = 0, levelCount[64];
+  int numLevels = 0;
+  const int rc = bfk_quadtree((int64_t)n, (const double *)points->data, (int64_t)leafSize, root_square, &nodes,
+                              &numNodes, levelCount, &numLevels, perm);
+  if (rc) {
+    free(perm);
+    bfSetError(rc == -2 ? BF_ERROR_MEMORY_ERROR : rc == -3 ? BF_ERROR_OUT_OF_RANGE : BF_ERROR_RUNTIME_ERROR);
+    return;
+  }
+  for (int64_t i = numNodes; i-- > 0;) { /* children follow their parents */
+    int32_t d = nodes[i].depth;
+    for (int q = 0; q < 4; ++q)
+      if (nodes[i].child[q] >= 0 && nodes[nodes[i].child[q]].maxDepthBelow > d)
+        d = nodes[nodes[i].child[q]].maxDepthBelow;
+    nodes[i].maxDepthBelow = d;
+  }
+  tree->levelOffset = calloc((size_t)numLevels + 1, sizeof(BfSize));
+  if (!tree->levelOffset) {
+    free(perm);
+    free(nodes);
+    bfSetError(BF_ERROR_MEMORY_ERROR);
+    return;
+  }
+  for (int d = 0; d < numLevels; ++d) tree->levelOffset[d + 1] = tree->levelOffset[d] + (BfSize)levelCount[d];
+  tree->numLevels = (BfSize)numLevels;
+  tree->points = points;
+  tree->unitNormals = unitNormals;
+  tree->leafSize = leafSize;
+  tree->nodes = nodes;
+  tree->numNodes = (BfSize)numNodes;
+  tree->super.root = nodes;
+  tree->super.perm = calloc(1, sizeof(BfPerm));
+  tree->super.perm->index = perm;
   tree->super.perm->size = n;
   tree_register(tree);
 }
