@@ -366,6 +366,8 @@ def main():
     ap.add_argument("--batch-rhs", type=int, default=16)
     ap.add_argument("--stream-n", type=int, default=8192, help="streaming / rank-model section size (0 = skip)")
     ap.add_argument("--stream-tol", type=float, default=1e-7)
+    ap.add_argument("--tree", choices=["device", "host"], default="device",
+                    help="build the two quadtrees on the GPU (bfhipQuadtreeInitWithLeafSize) or on the host")
     ap.add_argument("--factor-warmup", type=int, default=1 << 14,
                     help="points of an untimed factorization + apply run first, so the timed one does not carry "
                          "the process's one-time costs (kernel code loading, first touch); 0 = none")
@@ -395,14 +397,16 @@ def main():
           file=sys.stderr, flush=True)
 
     opts = bf.options(tol=args.tol, rank_const=args.rank_const, pass_through=True, pinv_rcond=args.pinv_rcond)
+    dev_tree = args.tree == "device"
     if args.factor_warmup:
         wsrc, wtgt, wk = curve_problem(args.factor_warmup)
-        wts, wtt = bf.Quadtree(wsrc, 64), bf.Quadtree(wtgt, 64)
+        wts, wtt = bf.Quadtree(wsrc, 64, device=dev_tree), bf.Quadtree(wtgt, 64, device=dev_tree)
         Fw = bf.FacHelm2.make_multilevel(bf.Helm2(wk), wts, wtt, opts)
         Fw @ bf.complex_randn(args.factor_warmup)
         del Fw, wts, wtt
     t0 = time.perf_counter()
-    ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
+    ts, tt = bf.Quadtree(src, 64, device=dev_tree), bf.Quadtree(tgt, 64, device=dev_tree)
+    tree_s = time.perf_counter() - t0
     F = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, opts)
     factor_s = time.perf_counter() - t0
     st = F.stats()
@@ -599,7 +603,7 @@ def main():
         "factor_s": factor_s,
         "factor_warmup": (f"one untimed N={args.factor_warmup} factorization + apply before the timed one"
                           if args.factor_warmup else None),
-        "factor_breakdown": {"tree_plan_s": st["plan_s"], "sample_ms": st["sample_ms"],
+        "factor_breakdown": {"trees_s": tree_s, "trees": args.tree, "tree_plan_s": st["plan_s"], "sample_ms": st["sample_ms"],
                              "compress_ms": st["compress_ms"], "apply_plan_s": st["apply_plan_s"],
                              "entries": st["entries"], "lstsq_blocks": st["lstsq_blocks"],
                              "kernel_blocks": st["kernel_blocks"], "h0_evals": st["h0_evals"]},

@@ -140,6 +140,7 @@ def lib():
         "bfQuadtreeNew": (vp, []),
         "bfQuadtreeInitWithLeafSize": (None, [vp, vp, vp, sz]),
         "bfhipQuadtreeInitWithLeafSize": (None, [vp, vp, vp, sz]),
+        "bfhipSvdTallMaxN": (C.c_int, []),
         "bfQuadtreeDeinitAndDealloc": (None, [C.POINTER(vp)]),
         "bfHelm2RankEstForTwoCircles": (sz, [vp, vp, vp, C.c_double, C.c_double]),
         "bfHelm2GetKernelMatrix": (vp, [vp, vp, vp, vp, vp]),

@@ -1768,9 +1768,12 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
  * bfk_set_apply_config. */
 static int g_applyWaves = 1;
 static int g_applyUnroll = 0;
-/* right-hand-side tiles of 16 per wave in the MFMA apply (0 = off, -1 =
- * 4 in FP64 and 2 in FP32, the shapes the tuner picks at N = 2^20) */
+/* the MFMA apply's shape: right-hand-side tiles of 16 per wave | 16 for
+ * two row tiles per wave (0 = the vector-unit kernel, -1 = the defaults
+ * below, the shapes the tuner picks at N = 2^20) */
 static int g_mfmaCT = -1;
+#define MFMA_DEFAULT_F64 4
+#define MFMA_DEFAULT_F32 2
 /* accumulation in the FP32 applies (bfk_set_compensation): 2 = FP32
  * matrix-core tiles folded into FP64 sums */
 static int g_comp32 = 2;
@@ -1781,17 +1784,22 @@ __constant__ int g_foldMaskDev = 0;
 
 /* One stage of the apply for one right-hand side.  Each wave of the block
  * owns one output row segment: its lanes split into R rows x G = 64/R
- * column groups (R = the segment height, at most 64; the compensated FP32
- * form and the UNROLL variant use the next power of two); a lane walks its
- * columns of every term (column-major blocks, so the R lanes of a group
- * read R consecutive entries), then the G partial sums meet through lane
- * reads and the row is written once.  With LO the
- * factors are an FP32 pair hi + lo.  With COMP (FP32 only) every lane
- * sums its FP32 products in FP64 and the lanes of a row meet in FP64, so
- * the rounding of a stage no longer grows with the length of its rows. */
-#define APPLY1_BATCH 4
+ * column groups (R = the segment height, at most 64; UNROLL selects the
+ * next power of two instead); a lane walks its columns of every term
+ * (column-major blocks, so the R lanes of a group read R consecutive
+ * entries), then the G partial sums meet through lane reads and the row is
+ * written once.  The walk is software-pipelined: a lane's columns go in
+ * batches of two, a batch never spanning two terms, and the loads of the
+ * next batch are issued before the products of the current one, so a wave
+ * has two batches of loads in flight across term boundaries.  Loads never
+ * branch: columns past a term's width read its last column and are masked
+ * out by a select.  With LO the factors are an FP32 pair hi + lo.  With
+ * COMP (FP32 only) every lane sums its products in FP64 (two chains) and
+ * the lanes of a row meet in FP64, so the rounding of a stage no longer
+ * grows with the length of its rows; with LO and COMP the pair is summed
+ * and multiplied in FP64. */
 template <typename V, bool LO, bool UNROLL, bool COMP = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_apply1(const BfkSeg *__restrict__ segs, int64_t numSegs,
                                                 const BfkTerm *__restrict__ terms, const V *__restrict__ hi,
                                                 const V *__restrict__ lo, typename BufsOf<V>::T bufs) {
   const int lane = threadIdx.x & 63;
@@ -1801,138 +1809,144 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   if (si >= numSegs) return; /* whole wave; no block barrier below */
   const BfkSeg S = segs[si];
   const int m = S.m;
-  int R = 64;
-  while (R > 1 && (R >> 1) >= m) R >>= 1;
-  const int G = 64 / R;
-  const int r = lane & (R - 1);
-  const int g = lane / R;
+  /* lane map: Rm rows x Gm column groups; the tight map (rows = segment
+   * height) keeps 51 of 64 lanes busy on a 17-row segment where the
+   * power-of-two map keeps 34 (4.4% faster at N = 2^20) */
+  int Rm;
+  if (UNROLL && !COMP) {
+    Rm = 64;
+    while (Rm > 1 && (Rm >> 1) >= m) Rm >>= 1;
+  } else {
+    Rm = m < 64 ? m : 64;
+  }
+  const int Gm = 64 / Rm, rl = lane % Rm, gl = lane / Rm;
+  /* columns per lane per batch: two FP64 batches of two fit seven waves
+   * per SIMD without spills; the compensated FP32 form measured 9.2 ms
+   * with two against 10.5 ms with three at N = 2^20 */
+  constexpr int APPLY1_BATCH = 2;
   V *y = bufs.out[S.ybuf] + S.yOff;
-  if constexpr (COMP) {
-    /* FP32 products summed in FP64 (two chains), reduced across the
-     * column groups in FP64, rounded to FP32 once; the tight lane map
-     * below (rows = segment height) */
-    const int Rc = m < 64 ? m : 64, Gc = 64 / Rc, rc = lane % Rc, gc = lane / Rc;
-    for (int rb = 0; rb < m; rb += Rc) {
-      const int i = rb + rc;
-      double sx0 = 0, sy0 = 0, sx1 = 0, sy1 = 0;
-      if (i < m && gc < Gc) {
-        for (int t = S.t0; t < S.t1; ++t) {
-          const BfkTerm T = terms[t];
-          const V *x = bufs.in[T.xbuf] + T.xOff;
-          if (T.aOff < 0) {
-            if (gc == 0) {
-              sx0 += (double)x[i].x;
-              sy0 += (double)x[i].y;
-            }
-            continue;
-          }
-          const int64_t lda = T.lda;
-          const V *A = hi + T.aOff + T.rowOff + i;
-          const V *B = lo + T.aOff + T.rowOff + i;
-          const int n = T.n;
-          /* a lane's columns j = gc, gc + Gc, ...: APPLY1_BATCH of them
-           * loaded before any is used (past the end a clamped, unused
-           * load), the products alternating between two FP64 chains */
-          for (int j0 = gc; j0 < n; j0 += APPLY1_BATCH * Gc) {
-            V xa[APPLY1_BATCH], aa[APPLY1_BATCH], la[APPLY1_BATCH];
+  /* batches over the block terms: ceil(n / (APPLY1_BATCH Gm)) per term */
+  const int span = APPLY1_BATCH * Gm;
+  int batches = 0;
+  for (int t = S.t0; t < S.t1; ++t) {
+    const BfkTerm T = terms[t];
+    if (T.aOff >= 0) batches += (T.n + span - 1) / span;
+  }
+  for (int rb = 0; rb < m; rb += Rm) {
+    const int i = rb + rl;
+    const bool live = i < m && gl < Gm;
+    const int ic = min(i, m - 1); /* idle lanes load row m - 1, unused */
+    V a0 = czero<V>();
+    double sx0 = 0, sy0 = 0, sx1 = 0, sy1 = 0;
+    for (int t = S.t0; t < S.t1; ++t) { /* identity terms */
+      const BfkTerm T = terms[t];
+      if (T.aOff >= 0 || !live || gl != 0) continue;
+      const V v = bufs.in[T.xbuf][T.xOff + i];
+      if constexpr (COMP) {
+        sx0 += (double)v.x;
+        sy0 += (double)v.y;
+      } else {
+        a0 = cadd(a0, v);
+      }
+    }
+    if (batches > 0) {
+      /* the load cursor: term ct, batch offset cj (uniform) */
+      int ct = S.t0;
+      while (terms[ct].aOff < 0) ++ct;
+      BfkTerm CT = terms[ct];
+      int cj = 0, issued = 0;
+      V xa[2][APPLY1_BATCH], aa[2][APPLY1_BATCH], la[2][APPLY1_BATCH];
+      int sn[2], sj[2];
+      auto issue = [&](int s) {
+        const V *x = bufs.in[CT.xbuf] + CT.xOff;
+        const int64_t ab = CT.aOff + CT.rowOff + ic, lda = CT.lda;
+        const int n = CT.n;
 #pragma unroll
-            for (int u = 0; u < APPLY1_BATCH; ++u) {
-              const int j = min(j0 + u * Gc, n - 1);
-              xa[u] = x[j];
-              aa[u] = A[(int64_t)j * lda];
-              if (LO) la[u] = B[(int64_t)j * lda];
-            }
-#pragma unroll
-            for (int u = 0; u < APPLY1_BATCH; ++u) { /* a select, not a branch: the loads stay above */
-              V p = cmulf(aa[u], xa[u]);
-              if (LO) p = cfmaf(la[u], xa[u], p);
-              const bool ok = j0 + u * Gc < n;
-              const double px = ok ? (double)p.x : 0.0, py = ok ? (double)p.y : 0.0;
-              if (u & 1) {
-                sx1 += px;
-                sy1 += py;
-              } else {
-                sx0 += px;
-                sy0 += py;
-              }
-            }
+        for (int u = 0; u < APPLY1_BATCH; ++u) {
+          const int j = min(cj + gl + u * Gm, n - 1);
+          xa[s][u] = x[j];
+          aa[s][u] = hi[ab + (int64_t)j * lda];
+          if (LO) la[s][u] = lo[ab + (int64_t)j * lda];
+        }
+        sn[s] = n;
+        sj[s] = cj;
+        if (++issued < batches) { /* past the last batch the cursor stays put */
+          cj += span;
+          if (cj >= n) {
+            cj = 0;
+            do ++ct;
+            while (terms[ct].aOff < 0);
+            CT = terms[ct];
           }
         }
+      };
+      auto compute = [&](int s) {
+#pragma unroll
+        for (int u = 0; u < APPLY1_BATCH; ++u) { /* selects, not branches: the loads stay above */
+          const bool ok = live && sj[s] + gl + u * Gm < sn[s];
+          if constexpr (COMP) {
+            double px, py;
+            if constexpr (LO) { /* hi + lo and the product in FP64: the pair's low part survives */
+              const double ar = (double)aa[s][u].x + (double)la[s][u].x;
+              const double ai = (double)aa[s][u].y + (double)la[s][u].y;
+              const double xr = (double)xa[s][u].x, xi = (double)xa[s][u].y;
+              px = fma(ar, xr, -ai * xi);
+              py = fma(ar, xi, ai * xr);
+            } else {
+              const V p = cmulf(aa[s][u], xa[s][u]);
+              px = (double)p.x;
+              py = (double)p.y;
+            }
+            px = ok ? px : 0.0;
+            py = ok ? py : 0.0;
+            if (u & 1) {
+              sx1 += px;
+              sy1 += py;
+            } else {
+              sx0 += px;
+              sy0 += py;
+            }
+          } else {
+            V t = cfmaT(aa[s][u], xa[s][u], a0);
+            if (LO) t = cfmaT(la[s][u], xa[s][u], t);
+            a0.x = ok ? t.x : a0.x;
+            a0.y = ok ? t.y : a0.y;
+          }
+        }
+      };
+      /* whole turns (two batches each), then the last batch if the count
+       * is odd: no exit inside a turn */
+      issue(0);
+      const int turns = batches >> 1;
+      for (int u = 0; u < turns; ++u) {
+        issue(1);
+        compute(0);
+        issue(0);
+        compute(1);
       }
+      if (batches & 1) compute(0);
+    }
+    if constexpr (COMP) {
       const double px = sx0 + sx1, py = sy0 + sy1;
       double ax = px, ay = py;
-      for (int k = 1; k < Gc; ++k) { /* lanes rc + k Rc hold the other groups */
-        ax += __shfl(px, rc + k * Rc, 64);
-        ay += __shfl(py, rc + k * Rc, 64);
+      for (int k = 1; k < Gm; ++k) { /* lanes rl + k Rm hold the other groups */
+        ax += __shfl(px, rl + k * Rm, 64);
+        ay += __shfl(py, rl + k * Rm, 64);
       }
-      if (gc == 0 && i < m) {
+      if (gl == 0 && i < m) {
         V out;
         out.x = (decltype(out.x))ax;
         out.y = (decltype(out.y))ay;
         y[i] = out;
       }
-    }
-    return;
-  }
-  /* the tight lane map: R = m rows (not the next power of two), G = 64 / m
-   * column groups, so a segment of 17 rows keeps 51 lanes busy instead of
-   * 34 (4.4% faster at N = 2^20); the groups meet through lane reads.
-   * UNROLL selects the power-of-two map with xor reductions instead. */
-  constexpr bool TIGHT = !UNROLL;
-  const int Rt = TIGHT ? (m < 64 ? m : 64) : R;
-  const int Gt = TIGHT ? 64 / Rt : G;
-  const int rt = TIGHT ? lane % Rt : r;
-  const int gt = TIGHT ? lane / Rt : g;
-  for (int rb = 0; rb < m; rb += Rt) {
-    const int i = rb + rt;
-    V a0 = czero<V>();
-    if (i < m && gt < Gt) {
-      for (int t = S.t0; t < S.t1; ++t) {
-        const BfkTerm T = terms[t];
-        const V *x = bufs.in[T.xbuf] + T.xOff;
-        if (T.aOff < 0) {
-          if (gt == 0) a0 = cadd(a0, x[i]);
-          continue;
-        }
-        const int64_t lda = T.lda;
-        const V *A = hi + T.aOff + T.rowOff + i;
-        const int n = T.n;
-        const V *L = lo + T.aOff + T.rowOff + i;
-        /* a lane's columns j = gt, gt + Gt, ...: APPLY1_BATCH of them
-         * loaded before any is used (past the end a clamped, unused load) */
-        for (int j0 = gt; j0 < n; j0 += APPLY1_BATCH * Gt) {
-          V xa[APPLY1_BATCH], aa[APPLY1_BATCH], la[APPLY1_BATCH];
-#pragma unroll
-          for (int u = 0; u < APPLY1_BATCH; ++u) {
-            const int j = min(j0 + u * Gt, n - 1);
-            xa[u] = x[j];
-            aa[u] = A[(int64_t)j * lda];
-            if (LO) la[u] = L[(int64_t)j * lda];
-          }
-#pragma unroll
-          for (int u = 0; u < APPLY1_BATCH; ++u) { /* a select, not a branch: the loads stay above */
-            V t = cfmaT(aa[u], xa[u], a0);
-            if (LO) t = cfmaT(la[u], xa[u], t);
-            const bool ok = j0 + u * Gt < n;
-            a0.x = ok ? t.x : a0.x;
-            a0.y = ok ? t.y : a0.y;
-          }
-        }
-      }
-    }
-    V acc = a0;
-    if (TIGHT) {
-      for (int k = 1; k < Gt; ++k) { /* lanes rt + k Rt hold the other groups */
-        acc.x += __shfl(a0.x, rt + k * Rt, 64);
-        acc.y += __shfl(a0.y, rt + k * Rt, 64);
-      }
     } else {
-      for (int o = R; o < 64; o <<= 1) {
-        acc.x += __shfl_xor(acc.x, o, 64);
-        acc.y += __shfl_xor(acc.y, o, 64);
+      V acc = a0;
+      for (int k = 1; k < Gm; ++k) {
+        acc.x += __shfl(a0.x, rl + k * Rm, 64);
+        acc.y += __shfl(a0.y, rl + k * Rm, 64);
       }
+      if (gl == 0 && i < m) y[i] = acc;
     }
-    if (gt == 0 && i < m) y[i] = acc;
   }
 }
 
@@ -2021,22 +2035,33 @@ struct MfmaF32 {
 };
 
 /* One stage of the apply for nrhs right-hand sides on the matrix cores:
- * one wave computes a 16-row x 16*CT-column tile of a segment's output,
- * looping over its row tiles, terms and 4-column steps of each term.  The
- * complex product is four real MFMAs per 16x16x4 step (re*re - im*im,
- * re*im + im*re).  blockIdx.x picks the column chunk so the chunks of one
- * segment share its factor blocks in one L2.  With COMP the matrix cores
- * accumulate a few 4-column steps into a fresh FP32 tile, which is then
- * added to an FP64 running sum, so rounding no longer grows with the
- * length of the rows. */
-template <class M, int CT, bool LO, bool COMP = false>
+ * one wave computes RT x 16 rows x 16*CT columns of a segment's output per
+ * pass.  The complex product is four real MFMAs per 16x16x4 step (re*re -
+ * im*im, re*im + im*re); with RT = 2 every input value a lane loads feeds
+ * both row tiles (segments are mostly 17-22 rows high), which halves the
+ * reads of the input vectors.  blockIdx.x picks the column chunk so the
+ * chunks of one segment share its factor blocks in one L2.
+ *
+ * The operand loads are software-pipelined: a wave walks the 4-column
+ * steps of all the segment's block terms as one sequence and keeps
+ * MFMA_RING steps of operands in registers; the loads of the next
+ * MFMA_RING - 1 steps are in flight while a step's MFMAs run, so a wave
+ * waits on memory once per pass instead of once per step.  Loads never
+ * branch: rows past the segment and columns past nrhs read a clamped
+ * in-bounds address and only feed output entries that are not stored, and
+ * the columns of a block's last step past its width read the block's last
+ * column and are zeroed before use.  With COMP the matrix cores accumulate
+ * FP32 tiles that start from zero at a fold group's first step and are
+ * added into FP64 sums every (mask + 1) steps and at the end of every
+ * term, so rounding no longer grows with the length of the rows. */
+#define MFMA_RING 3
+template <class M, int CT, int RT, bool LO, bool COMP>
 __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ segs,
                                                     const BfkTerm *__restrict__ terms,
                                                     const typename M::V *__restrict__ hi,
                                                     const typename M::V *__restrict__ lo,
                                                     typename BufsOf<typename M::V>::T bufs, int nrhs,
                                                     int numChunks, int64_t numSegs) {
-  typedef typename M::S S;
   typedef typename M::V V;
   typedef typename M::Acc Acc;
   /* workgroups go to the 8 XCDs round-robin by id: ids b with the same
@@ -2057,128 +2082,6 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
   const int m = Sg.m;
   const int lr = l & 15, lk = l >> 4;
   V *y = bufs.out[Sg.ybuf];
-  for (int rt = 0; rt < m; rt += 16) {
-    /* re/im: the tile's sum; with COMP, pre/pim collect up to 16 columns
-     * and cre/cim hold the Kahan compensation of re/im */
-    Acc re[CT], im[CT], pre[CT], pim[CT];
-    mf_d4 re64[CT], im64[CT];
-    for (int q = 0; q < CT; ++q)
-      for (int r = 0; r < 4; ++r) {
-        re[q][r] = im[q][r] = pre[q][r] = pim[q][r] = 0;
-        re64[q][r] = im64[q][r] = 0;
-      }
-    for (int t = Sg.t0; t < Sg.t1; ++t) {
-      const BfkTerm T = terms[t];
-      const V *x = bufs.in[T.xbuf];
-      if (T.aOff < 0) {
-        for (int q = 0; q < CT; ++q) {
-          const int col = c0 + 16 * q + lr;
-          for (int r = 0; r < 4; ++r) {
-            const int i = rt + M::row(l, r);
-            if (i < m && col < nrhs) {
-              V v = x[(T.xOff + i) * (int64_t)nrhs + col];
-              if constexpr (COMP) {
-                pre[q][r] += v.x;
-                pim[q][r] += v.y;
-              } else {
-                re[q][r] += v.x;
-                im[q][r] += v.y;
-              }
-            }
-          }
-        }
-        if constexpr (COMP)
-          for (int q = 0; q < CT; ++q) {
-            fold64(re64[q], pre[q]);
-            fold64(im64[q], pim[q]);
-          }
-        continue;
-      }
-      const int n = T.n;
-      const int64_t lda = T.lda;
-      const int ai = rt + lr;
-      for (int k0 = 0; k0 < n; k0 += 4) {
-        const int k = k0 + lk;
-        const bool aok = ai < m && k < n;
-        V a = aok ? hi[T.aOff + T.rowOff + ai + k * lda] : czero<V>();
-        V al = (LO && aok) ? lo[T.aOff + T.rowOff + ai + k * lda] : czero<V>();
-        for (int q = 0; q < CT; ++q) {
-          const int col = c0 + 16 * q + lr;
-          V b = (k < n && col < nrhs) ? x[(T.xOff + k) * (int64_t)nrhs + col] : czero<V>();
-          Acc &dr = COMP ? pre[q] : re[q];
-          Acc &di = COMP ? pim[q] : im[q];
-          dr = M::mma(a.x, b.x, dr);
-          dr = M::mma(-a.y, b.y, dr);
-          di = M::mma(a.x, b.y, di);
-          di = M::mma(a.y, b.x, di);
-          if (LO) {
-            dr = M::mma(al.x, b.x, dr);
-            dr = M::mma(-al.y, b.y, dr);
-            di = M::mma(al.x, b.y, di);
-            di = M::mma(al.y, b.x, di);
-          }
-        }
-        if constexpr (COMP) {
-          if (((k0 >> 2) & g_foldMaskDev) == g_foldMaskDev || k0 + 4 >= n)
-            for (int q = 0; q < CT; ++q) {
-              fold64(re64[q], pre[q]);
-              fold64(im64[q], pim[q]);
-            }
-        }
-      }
-    }
-    for (int q = 0; q < CT; ++q) {
-      const int col = c0 + 16 * q + lr;
-      for (int r = 0; r < 4; ++r) {
-        const int i = rt + M::row(l, r);
-        if (i < m && col < nrhs) {
-          V v;
-          if constexpr (COMP) {
-            v.x = (S)re64[q][r];
-            v.y = (S)im64[q][r];
-          } else {
-            v.x = (S)re[q][r];
-            v.y = (S)im[q][r];
-          }
-          y[(Sg.yOff + i) * (int64_t)nrhs + col] = v;
-        }
-      }
-    }
-  }
-}
-
-/* The same stage with the operand loads software-pipelined.  A wave walks
- * the 4-column steps of all the segment's block terms as one sequence and
- * keeps MFMA_RING steps of operands in registers: the loads of the next
- * MFMA_RING - 1 steps are in flight while a step's MFMAs run, so a wave
- * waits on memory once per row tile instead of once per step.  Loads
- * never branch: rows past the segment and columns past nrhs read a
- * clamped in-bounds address and only feed output entries that are not
- * stored, and the columns of a block's last step past its width read the
- * block's last column and are zeroed before use. */
-#define MFMA_RING 3
-template <class M, int CT, bool LO, bool COMP = false>
-__global__ __launch_bounds__(64) void k_applyN_mfma_p(const BfkSeg *__restrict__ segs,
-                                                      const BfkTerm *__restrict__ terms,
-                                                      const typename M::V *__restrict__ hi,
-                                                      const typename M::V *__restrict__ lo,
-                                                      typename BufsOf<typename M::V>::T bufs, int nrhs,
-                                                      int numChunks, int64_t numSegs) {
-  typedef typename M::V V;
-  typedef typename M::Acc Acc;
-  const int64_t slot = blockIdx.x >> 3;
-  const int64_t per = (int64_t)MFMA_SEG_GROUP * numChunks;
-  const int64_t grp = slot / per;
-  const int64_t within = slot - grp * per;
-  const int64_t si = (grp * 8 + (blockIdx.x & 7)) * MFMA_SEG_GROUP + within / numChunks;
-  const int chunk = (int)(within - (within / numChunks) * numChunks);
-  if (si >= numSegs) return;
-  const BfkSeg Sg = segs[si];
-  const int l = threadIdx.x;
-  const int c0 = chunk * 16 * CT;
-  const int m = Sg.m;
-  const int lr = l & 15, lk = l >> 4;
-  V *y = bufs.out[Sg.ybuf];
   int steps = 0; /* 4-column steps over the block terms */
   for (int t = Sg.t0; t < Sg.t1; ++t) {
     const BfkTerm T = terms[t];
@@ -2186,55 +2089,63 @@ __global__ __launch_bounds__(64) void k_applyN_mfma_p(const BfkSeg *__restrict__
   }
   int colc[CT];
   for (int q = 0; q < CT; ++q) colc[q] = min(c0 + 16 * q + lr, nrhs - 1);
-  for (int rt = 0; rt < m; rt += 16) {
-    Acc re[CT], im[CT], pre[CT], pim[CT];
-    mf_d4 re64[CT], im64[CT];
-    for (int q = 0; q < CT; ++q)
-      for (int r = 0; r < 4; ++r) {
-        re[q][r] = im[q][r] = pre[q][r] = pim[q][r] = 0;
-        re64[q][r] = im64[q][r] = 0;
-      }
+  for (int rt = 0; rt < m; rt += 16 * RT) {
+    /* row tiles of this pass that hold rows (uniform) */
+    const int live = min(RT, (m - rt + 15) >> 4);
+    Acc re[RT][CT], im[RT][CT], pre[RT][CT], pim[RT][CT];
+    mf_d4 re64[RT][CT], im64[RT][CT];
+    for (int u = 0; u < RT; ++u)
+      for (int q = 0; q < CT; ++q)
+        for (int r = 0; r < 4; ++r) {
+          re[u][q][r] = im[u][q][r] = pre[u][q][r] = pim[u][q][r] = 0;
+          re64[u][q][r] = im64[u][q][r] = 0;
+        }
     for (int t = Sg.t0; t < Sg.t1; ++t) { /* identity terms */
       const BfkTerm T = terms[t];
       if (T.aOff >= 0) continue;
       const V *x = bufs.in[T.xbuf];
-      for (int q = 0; q < CT; ++q) {
-        const int col = c0 + 16 * q + lr;
-        for (int r = 0; r < 4; ++r) {
-          const int i = rt + M::row(l, r);
-          if (i < m && col < nrhs) {
-            V v = x[(T.xOff + i) * (int64_t)nrhs + col];
-            if constexpr (COMP) {
-              pre[q][r] += v.x;
-              pim[q][r] += v.y;
-            } else {
-              re[q][r] += v.x;
-              im[q][r] += v.y;
+      for (int u = 0; u < RT; ++u)
+        for (int q = 0; q < CT; ++q) {
+          const int col = c0 + 16 * q + lr;
+          for (int r = 0; r < 4; ++r) {
+            const int i = rt + 16 * u + M::row(l, r);
+            if (i < m && col < nrhs) {
+              V v = x[(T.xOff + i) * (int64_t)nrhs + col];
+              if constexpr (COMP) {
+                pre[u][q][r] += v.x;
+                pim[u][q][r] += v.y;
+              } else {
+                re[u][q][r] += v.x;
+                im[u][q][r] += v.y;
+              }
             }
           }
         }
-      }
       if constexpr (COMP)
-        for (int q = 0; q < CT; ++q) {
-          fold64(re64[q], pre[q]);
-          fold64(im64[q], pim[q]);
-        }
+        for (int u = 0; u < RT; ++u)
+          for (int q = 0; q < CT; ++q) {
+            fold64(re64[u][q], pre[u][q]);
+            fold64(im64[u][q], pim[u][q]);
+          }
     }
     if (steps > 0) {
-      const int arow = min(rt + lr, m - 1);
+      int arow[RT];
+      for (int u = 0; u < RT; ++u) arow[u] = min(rt + 16 * u + lr, m - 1);
       /* the load cursor: term lt, its columns from lk0 */
       int lt = Sg.t0;
       while (terms[lt].aOff < 0) ++lt;
       BfkTerm LT = terms[lt];
       const V *LX = bufs.in[LT.xbuf];
       int lk0 = 0, issued = 0;
-      V ra[MFMA_RING], rl[MFMA_RING], rb[MFMA_RING][CT];
+      V ra[MFMA_RING][RT], rl[MFMA_RING][RT], rb[MFMA_RING][CT];
       int sk0[MFMA_RING], sn[MFMA_RING];
       auto issue = [&](int s) {
         const int kk = min(lk0 + lk, LT.n - 1);
-        const int64_t ao = LT.aOff + LT.rowOff + arow + (int64_t)kk * LT.lda;
-        ra[s] = hi[ao];
-        if (LO) rl[s] = lo[ao];
+        const int64_t acol = LT.aOff + LT.rowOff + (int64_t)kk * LT.lda;
+        for (int u = 0; u < RT; ++u) {
+          ra[s][u] = hi[acol + arow[u]];
+          if (LO) rl[s][u] = lo[acol + arow[u]];
+        }
         const V *x = LX + (LT.xOff + kk) * (int64_t)nrhs;
         for (int q = 0; q < CT; ++q) rb[s][q] = x[colc[q]];
         sk0[s] = lk0;
@@ -2251,44 +2162,52 @@ __global__ __launch_bounds__(64) void k_applyN_mfma_p(const BfkSeg *__restrict__
         }
       };
       auto compute = [&](int s) {
-        V a = ra[s], al = rl[s];
-        V b[CT];
+        V a[RT], al[RT], b[CT];
+        for (int u = 0; u < RT; ++u) {
+          a[u] = ra[s][u];
+          al[u] = rl[s][u];
+        }
         for (int q = 0; q < CT; ++q) b[q] = rb[s][q];
         const int rem = sn[s] - sk0[s];
         if (rem < 4 && lk >= rem) {
-          a = czero<V>();
-          if (LO) al = czero<V>();
+          for (int u = 0; u < RT; ++u) {
+            a[u] = czero<V>();
+            if (LO) al[u] = czero<V>();
+          }
           for (int q = 0; q < CT; ++q) b[q] = czero<V>();
         }
         /* with COMP a fold group's first step starts its FP32 tiles from
          * zero instead of clearing them after the fold */
-        auto step = [&](auto freshTag) { /* straight-line MFMAs for either case */
+        auto tile = [&](int u, auto freshTag) { /* straight-line MFMAs for either case */
           constexpr bool FRESH = decltype(freshTag)::value;
+          /* with LO the pair hi + lo is summed in the matrix cores'
+           * precision (FP64 for MfmaF64S32) before the products */
+          typedef typename M::S S;
+          const S ar = LO ? (S)a[u].x + (S)al[u].x : (S)a[u].x;
+          const S ai = LO ? (S)a[u].y + (S)al[u].y : (S)a[u].y;
           for (int q = 0; q < CT; ++q) {
-            Acc &dr = COMP ? pre[q] : re[q];
-            Acc &di = COMP ? pim[q] : im[q];
-            dr = M::mma(a.x, b[q].x, FRESH ? Acc{} : dr);
-            di = M::mma(a.x, b[q].y, FRESH ? Acc{} : di);
-            dr = M::mma(-a.y, b[q].y, dr);
-            di = M::mma(a.y, b[q].x, di);
-            if (LO) {
-              dr = M::mma(al.x, b[q].x, dr);
-              dr = M::mma(-al.y, b[q].y, dr);
-              di = M::mma(al.x, b[q].y, di);
-              di = M::mma(al.y, b[q].x, di);
-            }
+            Acc &dr = COMP ? pre[u][q] : re[u][q];
+            Acc &di = COMP ? pim[u][q] : im[u][q];
+            dr = M::mma(ar, b[q].x, FRESH ? Acc{} : dr);
+            di = M::mma(ar, b[q].y, FRESH ? Acc{} : di);
+            dr = M::mma(-ai, b[q].y, dr);
+            di = M::mma(ai, b[q].x, di);
+          }
+          if constexpr (COMP) {
+            if (((sk0[s] >> 2) & g_foldMaskDev) == g_foldMaskDev || rem <= 4)
+              for (int q = 0; q < CT; ++q) {
+                re64[u][q] += __builtin_convertvector(pre[u][q], mf_d4);
+                im64[u][q] += __builtin_convertvector(pim[u][q], mf_d4);
+              }
           }
         };
-        if (COMP && ((sk0[s] >> 2) & g_foldMaskDev) == 0)
-          step(std::true_type{});
-        else
-          step(std::false_type{});
-        if constexpr (COMP) {
-          if (((sk0[s] >> 2) & g_foldMaskDev) == g_foldMaskDev || rem <= 4)
-            for (int q = 0; q < CT; ++q) {
-              re64[q] += __builtin_convertvector(pre[q], mf_d4);
-              im64[q] += __builtin_convertvector(pim[q], mf_d4);
-            }
+        const bool fresh = COMP && ((sk0[s] >> 2) & g_foldMaskDev) == 0;
+        for (int u = 0; u < RT; ++u) {
+          if (u >= live) break; /* uniform: a pass past the segment's rows */
+          if (fresh)
+            tile(u, std::true_type{});
+          else
+            tile(u, std::false_type{});
         }
       };
 #pragma unroll
@@ -2308,45 +2227,47 @@ __global__ __launch_bounds__(64) void k_applyN_mfma_p(const BfkSeg *__restrict__
       for (int s = 0; s < MFMA_RING - 1; ++s)
         if (s < tail) compute(s);
     }
-    for (int q = 0; q < CT; ++q) {
-      const int col = c0 + 16 * q + lr;
-      for (int r = 0; r < 4; ++r) {
-        const int i = rt + M::row(l, r);
-        if (i < m && col < nrhs) {
-          V v;
-          if constexpr (COMP) {
-            v.x = (decltype(v.x))re64[q][r];
-            v.y = (decltype(v.y))im64[q][r];
-          } else {
-            v.x = (decltype(v.x))re[q][r];
-            v.y = (decltype(v.y))im[q][r];
+    for (int u = 0; u < RT; ++u)
+      for (int q = 0; q < CT; ++q) {
+        const int col = c0 + 16 * q + lr;
+        for (int r = 0; r < 4; ++r) {
+          const int i = rt + 16 * u + M::row(l, r);
+          if (i < m && col < nrhs) {
+            V v;
+            if constexpr (COMP) {
+              v.x = (decltype(v.x))re64[u][q][r];
+              v.y = (decltype(v.y))im64[u][q][r];
+            } else {
+              v.x = (decltype(v.x))re[u][q][r];
+              v.y = (decltype(v.y))im[u][q][r];
+            }
+            y[(Sg.yOff + i) * (int64_t)nrhs + col] = v;
           }
-          y[(Sg.yOff + i) * (int64_t)nrhs + col] = v;
         }
       }
-    }
   }
 }
 
-/* the matrix-core kernel of one many-RHS stage with `ct` tiles of 16
- * right-hand sides per wave (1, 2, 4 or 8) */
-static int g_mfmaPipe = 1; /* 1: k_applyN_mfma_p, 0: k_applyN_mfma */
+/* the matrix-core kernel of one many-RHS stage: cfg = tiles of 16
+ * right-hand sides per wave (1, 2, 4 or 8) | 16 for two row tiles per
+ * wave */
 template <class M, bool LO, bool COMP>
-static void launch_mfma(int ct, dim3 grid, const BfkSeg *segs, const BfkTerm *terms, const typename M::V *H,
+static void launch_mfma(int cfg, dim3 grid, const BfkSeg *segs, const BfkTerm *terms, const typename M::V *H,
                         const typename M::V *Lo, typename BufsOf<typename M::V>::T b, int nrhs, int chunks,
                         int64_t numSegs) {
-#define BFK_MFMA_LAUNCH(K, C)                                                                                   \
-  hipLaunchKernelGGL((K<M, C, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b, nrhs, chunks, numSegs)
-  if (g_mfmaPipe) {
-    if (ct == 1) BFK_MFMA_LAUNCH(k_applyN_mfma_p, 1);
-    else if (ct == 2) BFK_MFMA_LAUNCH(k_applyN_mfma_p, 2);
-    else if (ct == 8) BFK_MFMA_LAUNCH(k_applyN_mfma_p, 8);
-    else BFK_MFMA_LAUNCH(k_applyN_mfma_p, 4);
+#define BFK_MFMA_LAUNCH(C, R)                                                                                  \
+  hipLaunchKernelGGL((k_applyN_mfma<M, C, R, LO, COMP>), grid, dim3(64), 0, g_stream, segs, terms, H, Lo, b, nrhs, \
+                     chunks, numSegs)
+  const int ct = cfg & 15;
+  if ((cfg & 16) && ct != 8) {
+    if (ct == 1) BFK_MFMA_LAUNCH(1, 2);
+    else if (ct == 2) BFK_MFMA_LAUNCH(2, 2);
+    else BFK_MFMA_LAUNCH(4, 2);
   } else {
-    if (ct == 1) BFK_MFMA_LAUNCH(k_applyN_mfma, 1);
-    else if (ct == 2) BFK_MFMA_LAUNCH(k_applyN_mfma, 2);
-    else if (ct == 8) BFK_MFMA_LAUNCH(k_applyN_mfma, 8);
-    else BFK_MFMA_LAUNCH(k_applyN_mfma, 4);
+    if (ct == 1) BFK_MFMA_LAUNCH(1, 1);
+    else if (ct == 2) BFK_MFMA_LAUNCH(2, 1);
+    else if (ct == 8) BFK_MFMA_LAUNCH(8, 1);
+    else BFK_MFMA_LAUNCH(4, 1);
   }
 #undef BFK_MFMA_LAUNCH
 }
@@ -2379,7 +2300,6 @@ extern "C" {
 
 int bfk_init(void) {
   if (g_stream) return 0;
-  if (const char *e = getenv("BF_AMD_MFMA_PIPE")) g_mfmaPipe = atoi(e);
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
     fprintf(stderr, "bf_kernels: no HIP device available\n");
@@ -2552,7 +2472,9 @@ void bfk_get_time_sums(double sums[8], double counts[8]) {
 
 int bfk_set_apply_config(int wavesPerBlock, int unroll, int mfmaTiles) {
   if (wavesPerBlock != 1 && wavesPerBlock != 2 && wavesPerBlock != 4) return -1;
-  if (mfmaTiles != -1 && mfmaTiles != 0 && mfmaTiles != 1 && mfmaTiles != 2 && mfmaTiles != 4 && mfmaTiles != 8)
+  const int ct = mfmaTiles & 15;
+  if (mfmaTiles != -1 && mfmaTiles != 0 && ((mfmaTiles & ~31) || (ct != 1 && ct != 2 && ct != 4 && ct != 8) ||
+                                            mfmaTiles == (8 | 16)))
     return -1;
   g_applyWaves = wavesPerBlock;
   g_applyUnroll = unroll ? 1 : 0;
@@ -2932,8 +2854,9 @@ int bfk_ortho(const BfkOrthoJob *dJobs, const int32_t *dOrder, int numGroups, co
 
 /* The launch shape of stage s: cfg < 0 takes the global settings; for one
  * right-hand side cfg = waves per block | unroll << 4, for 16 and more
- * cfg = right-hand-side tiles of 16 per MFMA wave (0 = the vector-unit
- * kernel); between 2 and 15 right-hand sides it is not used. */
+ * cfg = right-hand-side tiles of 16 per MFMA wave | 16 for two row tiles
+ * per wave (0 = the vector-unit kernel); between 2 and 15 right-hand sides
+ * it is not used. */
 static int g_capturing = 0; /* launches go into a graph: no timing events */
 
 static int launch_stage_f64(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms, const double *dArena,
@@ -2950,10 +2873,11 @@ static int launch_stage_f64(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
       hipLaunchKernelGGL((k_apply1<double2, false, false>), grid, block, 0, g_stream, segs, n, dTerms,
                          (const double2 *)dArena, (const double2 *)nullptr, b);
   } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT != 0)) {
-    const int ct = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : 4;
+    const int mc = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : MFMA_DEFAULT_F64;
+    const int ct = mc & 15;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));
-    launch_mfma<MfmaF64, false, false>(ct, grid, segs, dTerms, (const double2 *)dArena, nullptr, b, nrhs, chunks, n);
+    launch_mfma<MfmaF64, false, false>(mc, grid, segs, dTerms, (const double2 *)dArena, nullptr, b, nrhs, chunks, n);
   } else {
     hipLaunchKernelGGL(k_applyN, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream, segs,
                        dTerms, (const double2 *)dArena, b, nrhs);
@@ -3047,7 +2971,7 @@ void bfk_graph_destroy(void *exec) {
 /* the global launch settings as one number: a captured apply is stale once
  * it changes */
 int bfk_apply_config_stamp(void) {
-  return g_applyWaves | g_applyUnroll << 4 | (g_mfmaCT & 15) << 5 | g_comp32 << 9 | g_foldMask << 13;
+  return g_applyWaves | g_applyUnroll << 4 | (g_mfmaCT & 31) << 5 | g_comp32 << 10 | g_foldMask << 14;
 }
 
 int bfk_copy(const void *src, void *dst, size_t bytes) {
@@ -3209,17 +3133,17 @@ static int launch_stage_f32(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
       launch_apply1_f32<false, false>(grid, block, segs, n, dTerms, H, Lo, b);
     }
   } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT != 0)) {
-    const int ct = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : 2;
+    const int mc = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : MFMA_DEFAULT_F32;
+    const int ct = mc & 15;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));
-    if (g_comp32 == 1 && !Lo) {
-      launch_mfma<MfmaF64S32, false, false>(ct, grid, segs, dTerms, H, nullptr, b, nrhs, chunks, n);
-    } else if (Lo) {
-      if (g_comp32) launch_mfma<MfmaF32, true, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
-      else launch_mfma<MfmaF32, true, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+    if (Lo) { /* hi + lo summed in FP64 on load, the FP64 matrix cores */
+      launch_mfma<MfmaF64S32, true, false>(mc, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+    } else if (g_comp32 == 1) {
+      launch_mfma<MfmaF64S32, false, false>(mc, grid, segs, dTerms, H, nullptr, b, nrhs, chunks, n);
     } else {
-      if (g_comp32) launch_mfma<MfmaF32, false, true>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
-      else launch_mfma<MfmaF32, false, false>(ct, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+      if (g_comp32) launch_mfma<MfmaF32, false, true>(mc, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
+      else launch_mfma<MfmaF32, false, false>(mc, grid, segs, dTerms, H, Lo, b, nrhs, chunks, n);
     }
   } else {
     hipLaunchKernelGGL(k_applyN_f32, dim3((unsigned)n, (unsigned)((nrhs + 63) / 64)), dim3(64), 0, g_stream, segs,

@@ -1218,11 +1218,11 @@ int bfhipFacTuneApply(BfMat *fac, BfSize nrhs, BfhipPrecision prec, char const *
   const int cls = bf_nrhs_class(nrhs);
   if (cls == 1) return 0; /* 2-15 right-hand sides: the vector-unit kernel only */
   if (!cachePath) cachePath = getenv("BF_AMD_TUNE_CACHE");
-  /* candidates: waves per block x lane map (1 RHS); tiles per MFMA wave,
-   * 0 = the vector-unit kernel (16 and more) */
-  static const int one[] = {1, 2, 4, 1 | 16, 2 | 16, 4 | 16}, many[] = {1, 2, 4, 8, 0};
+  /* candidates: waves per block x lane map (1 RHS); tiles per MFMA wave
+   * x one or two row tiles, 0 = the vector-unit kernel (16 and more) */
+  static const int one[] = {1, 2, 4, 1 | 16, 2 | 16, 4 | 16}, many[] = {1, 2, 4, 8, 1 | 16, 2 | 16, 4 | 16, 0};
   const int *cand = cls == 0 ? one : many;
-  const int nc = cls == 0 ? 6 : 5;
+  const int nc = cls == 0 ? 6 : 8;
   const size_t elem = prec == BFHIP_PRECISION_FP64 ? 16 : 8;
   int rc = ensure_buffers(F, nrhs, elem);
   void *dX = rc ? NULL : bfk_malloc(F->numSrc * nrhs * elem + 16);

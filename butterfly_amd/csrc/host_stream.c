@@ -359,6 +359,14 @@ static void svd_job_free(SvdJob *J) {
   J->a = J->U = J->W = NULL;
 }
 
+int bfhipSvdTallMaxN(void) {
+  if (bfk_init()) {
+    bfSetError(BF_ERROR_RUNTIME_ERROR);
+    return -1;
+  }
+  return bfk_svd_tall_max_n();
+}
+
 int bfhipSvdBatched(BfSize batch, BfSize const *m, BfSize const *n, BfComplex const *A, BfReal tol, BfSize *k,
                     BfComplex *U, BfComplex *W) {
   if (!m || !n || !A || !k || !U || !W) {

@@ -415,8 +415,11 @@ BfSize bfhipFacLeafNumFactors(BfMat const *fac, BfSize leaf);
 
 /* ------------------------------------------------ device entry points */
 
-/* Precision of the apply: FP64 factors, FP32 factors with FP32
- * accumulation, or FP32 hi+lo factors (two FP32 products summed). */
+/* Precision of the apply: FP64 factors and vectors; FP32 factors and
+ * vectors (accumulation set by bfhipSetApplyCompensation); or FP32 hi+lo
+ * factors with FP32 vectors, the pair summed in FP64 before the product
+ * (FP64 products with compensation on one RHS, the FP64 matrix cores on
+ * many), so only the vectors round to FP32. */
 typedef enum BfhipPrecision {
   BFHIP_PRECISION_FP64 = 0,
   BFHIP_PRECISION_FP32 = 1,
@@ -492,9 +495,9 @@ void bfhipResetKernelTimes(void);
  * block in {1, 2, 4} (one row segment per wave) and, when `unroll` is 1,
  * the power-of-two lane map instead of the default map whose rows match
  * the segment height; the many-RHS apply's right-hand-side
- * tiles of 16 per wave on the matrix cores in {1, 2, 4, 8}, 0 for the
- * vector-unit kernel, or -1 (the default) for 4 tiles in FP64 and 2 in
- * FP32 */
+ * tiles of 16 per wave on the matrix cores in {1, 2, 4, 8}, plus 16 for
+ * two 16-row tiles per wave (tiles 1, 2 or 4), 0 for the vector-unit
+ * kernel, or -1 (the default: the shapes the tuner picks at N = 2^20) */
 int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
 /* new (no counterpart): hipGraph replay of the apply's stage chain, off (0,
  * the default) or on.  When on, bfhipMatMulDevice captures the stage
@@ -507,8 +510,8 @@ int bfhipSetApplyGraphs(int on);
 /* new (no counterpart): the apply's autotuner.  For each apply stage of
  * `fac` it times the launch shapes that fit `nrhs` (one right-hand side:
  * 1, 2 or 4 waves per block with either lane map; 16 and more: 1, 2, 4 or
- * 8 tiles of 16 right-hand sides per matrix-core wave, or the vector-unit
- * kernel) on that stage alone and keeps the fastest, per bucket of stages
+ * 8 tiles of 16 right-hand sides per matrix-core wave, 1, 2 or 4 with two
+ * row tiles per wave, or the vector-unit kernel) on that stage alone and keeps the fastest, per bucket of stages
  * (rounded log2 of the mean segment height and of the mean columns per
  * segment), for the precision and the class of nrhs (1, 2-15, 16-31,
  * 32-63, 64-127, 128-255, 256+).  Buckets found in the text file
@@ -819,6 +822,12 @@ BfMat *bfhipFacLoadFile(char const *path);
  * packed column-major host arrays, U and W sized for full rank min(m,n). */
 int bfhipSvdBatched(BfSize batch, BfSize const *m, BfSize const *n, BfComplex const *A, BfReal tol, BfSize *k,
                     BfComplex *U, BfComplex *W);
+/* new (no counterpart): the widest matrix (columns) that bfhipSvdBatched
+ * sends through its tall path (QR, then Jacobi on the triangle in LDS) --
+ * m >= 2n and at least 2^15 entries -- or 0 when this GPU's LDS cannot
+ * hold it (then such matrices take the one-launch-per-step path); -1
+ * without a device */
+int bfhipSvdTallMaxN(void);
 
 /* new (no counterpart): the same truncated SVDs through a randomized range
  * finder of width width_b (Halko, Martinsson & Tropp 2011, Alg. 4.1 and

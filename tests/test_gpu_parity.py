@@ -231,14 +231,21 @@ def test_device_resident_apply_and_fp32(gpu_lib):
     x32 = x.astype(np.complex64)
     dx32 = bf.DeviceArray.from_numpy(x32)
     dy32 = bf.DeviceArray(8 * len(tgt))
+    # the reference for both modes is the FP64 apply of the FP32-rounded
+    # input, so the input's own rounding is not counted against either
+    y64r = F @ x32.astype(np.complex128)
     errs = {}
     for prec in ("fp32", "fp32x2"):
         bf.mul_device(F, dx32, dy32, 1, prec)
-        errs[prec] = rel(dy32.to_numpy(np.complex64, len(tgt)).astype(complex), y64)
-    print("fp32 apply rel err vs fp64:", errs)
-    # equivalent-source charges cancel strongly, so FP32 vectors lose ~3-4
-    # digits against the FP64 apply (see the mixed-precision notes)
-    assert errs["fp32"] < 1e-3 and errs["fp32x2"] < 1e-3
+        errs[prec] = rel(dy32.to_numpy(np.complex64, len(tgt)).astype(complex), y64r)
+    print("fp32 apply rel err vs fp64 of the same input:", errs)
+    # unorthogonalized factors, whose intermediates cancel: FP32 factors
+    # lose 5.1e-5 here (measured on MI355X); hi + lo factors summed in FP64
+    # before the product leave only the FP32 rounding of the vectors
+    # (2.5e-6 measured): the split form must be several times better
+    assert errs["fp32"] < 2e-4
+    assert errs["fp32x2"] < 1e-5
+    assert errs["fp32x2"] < 0.2 * errs["fp32"]
     X = (rng.standard_normal((len(src), 20)) + 1j * rng.standard_normal((len(src), 20)))
     dX = bf.DeviceArray.from_numpy(X.astype(np.complex64))
     dY = bf.DeviceArray(8 * 20 * len(tgt))

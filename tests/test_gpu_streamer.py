@@ -25,9 +25,13 @@ def rel(a, b):
 # QR then Jacobi in LDS (tall: m >= 2n, n <= 70, >= 2^15 entries), one launch
 # per Jacobi step (large)
 @pytest.mark.parametrize("m,n", [(40, 8), (300, 32), (17, 64), (128, 128), (5, 1), (1000, 20), (600, 300),
-                                 (2048, 64), (700, 50), (64, 1500)])
+                                 (2048, 64), (700, 50), (64, 1500), (600, 70), (700, 67)])
 def test_svd_batched_matches_numpy(gpu_lib, m, n):
     bf = gpu_lib
+    if m >= 2 * n and m * n >= 1 << 15:
+        # the tall path (QR + compact WY, Jacobi on the triangle in LDS) takes
+        # widths up to 70: the widest ones sit at its LDS limit
+        assert bf.lib().bfhipSvdTallMaxN() == 70
     rng = np.random.default_rng(m * 1000 + n)
     mats = []
     for scale in (1.0, 1e-3):
@@ -50,7 +54,7 @@ def test_svd_batched_matches_numpy(gpu_lib, m, n):
             assert err <= (s[k] if k < len(s) else 0) * 1.0001 + 1e-13 * max(1, n / 64) * s[0]
 
 
-@pytest.mark.parametrize("m,n", [(900, 40), (40, 900)])
+@pytest.mark.parametrize("m,n", [(900, 40), (40, 900), (600, 70), (70, 600)])
 def test_svd_batched_rank_deficient_with_zero_columns(gpu_lib, m, n):
     """Tall path (as is, or on the conjugate transpose) with exact rank 10
     and five exactly zero columns (rows): the pivoted QR ends with steps that
