@@ -1772,8 +1772,14 @@ static int g_applyUnroll = 0;
  * two row tiles per wave (0 = the vector-unit kernel, -1 = the defaults
  * below, the shapes the tuner picks at N = 2^20) */
 static int g_mfmaCT = -1;
-#define MFMA_DEFAULT_F64 4
+#define MFMA_DEFAULT_F64 (2 | 16)
 #define MFMA_DEFAULT_F32 2
+/* the default shape with no more column tiles than nrhs fills */
+static inline int mfma_auto(int cfg, int nrhs) {
+  int ct = cfg & 15;
+  while (ct > 1 && 16 * (ct / 2) >= nrhs) ct /= 2;
+  return (cfg & 16) | ct;
+}
 /* accumulation in the FP32 applies (bfk_set_compensation): 2 = FP32
  * matrix-core tiles folded into FP64 sums */
 static int g_comp32 = 2;
@@ -2873,7 +2879,7 @@ static int launch_stage_f64(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
       hipLaunchKernelGGL((k_apply1<double2, false, false>), grid, block, 0, g_stream, segs, n, dTerms,
                          (const double2 *)dArena, (const double2 *)nullptr, b);
   } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT != 0)) {
-    const int mc = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : MFMA_DEFAULT_F64;
+    const int mc = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : mfma_auto(MFMA_DEFAULT_F64, nrhs);
     const int ct = mc & 15;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));
@@ -3133,7 +3139,7 @@ static int launch_stage_f32(const BfkSeg *segs, int64_t n, const BfkTerm *dTerms
       launch_apply1_f32<false, false>(grid, block, segs, n, dTerms, H, Lo, b);
     }
   } else if (nrhs >= 16 && (cfg >= 0 ? cfg : g_mfmaCT != 0)) {
-    const int mc = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : MFMA_DEFAULT_F32;
+    const int mc = cfg >= 0 ? cfg : g_mfmaCT > 0 ? g_mfmaCT : mfma_auto(MFMA_DEFAULT_F32, nrhs);
     const int ct = mc & 15;
     const int chunks = (nrhs + 16 * ct - 1) / (16 * ct);
     dim3 grid(mfma_grid(n, chunks));

@@ -497,7 +497,8 @@ void bfhipResetKernelTimes(void);
  * the segment height; the many-RHS apply's right-hand-side
  * tiles of 16 per wave on the matrix cores in {1, 2, 4, 8}, plus 16 for
  * two 16-row tiles per wave (tiles 1, 2 or 4), 0 for the vector-unit
- * kernel, or -1 (the default: the shapes the tuner picks at N = 2^20) */
+ * kernel, or -1 (the default: the shapes the tuner picks at N = 2^20, 2
+ * tiles x 2 row tiles in FP64 and 2 tiles x 1 row tile in FP32) */
 int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
 /* new (no counterpart): hipGraph replay of the apply's stage chain, off (0,
  * the default) or on.  When on, bfhipMatMulDevice captures the stage
@@ -862,8 +863,8 @@ int bfhipRankModelGetParams(BfhipRankModel const *model, float *params, float no
 int bfhipRankModelSetParams(BfhipRankModel *model, float const *params, float const norm[14]);
 
 /* new (no counterpart): with a model, every truncated SVD of the stream
- * takes the randomized path with width = predicted rank + oversample (the
- * full SVD stays the fallback and the judge); with or without one, the
+ * takes the randomized path with width = predicted rank + oversample (a
+ * rejected sketch falls back to the full SVD); with or without one, the
  * (features, numerical rank) pair of every SVD is recorded for training.
  * `k` is the wavenumber of the kernel behind the stream (0 if none). */
 int bfhipFacStreamerSetRankModel(BfFacStreamer *facStreamer, BfhipRankModel const *model, BfReal k,
