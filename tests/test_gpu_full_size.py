@@ -1,0 +1,116 @@
+"""Full-size workloads on the GPU (the configurations' own sizes, GPU).
+
+Each is checked against exact FP64 dense rows of the kernel matrix (the
+oracle's restatement of the reference kernel, summed on host threads):
+- the box geometry (16384 sources in [0,1]^2, 16384 targets in
+  [2,3]x[0,1], k = 32 pi) with leaf 64 and tolerance 1e-8;
+- the curve -> circle geometry at 2^20 points (leaf 64, tolerance 1e-8,
+  rank constant 1.25, rcond 1e-11, trees built on the GPU): the FP64
+  apply, then the orthogonalized FP32 form on one right-hand side against
+  the exact rows and on 64 right-hand sides against the FP64 apply (the
+  benchmark runs the same kernels and shapes at 256);
+- the second-kind boundary equation I/2 + (S + C)W at 2^18 points on the
+  curve, plane wave at 30 degrees: FP64 GMRES and the mixed solver reach
+  the requested residual and agree.
+"""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import box_problem, curve_problem
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(tol=1e-8, rank_const=1.25, pass_through=True, pinv_rcond=1e-11)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def exact_rows(k, tgt_rows, src, x, threads=16):
+    chunks = np.array_split(np.arange(len(tgt_rows)), threads)
+    out = np.zeros(len(tgt_rows), complex)
+
+    def work(ix):
+        if len(ix):
+            out[ix] = orc.dense_rows(k, tgt_rows[ix], src, x)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, chunks))
+    return out
+
+
+def test_box_geometry_at_full_size_with_leaf_64(gpu_lib):
+    bf = gpu_lib
+    src, tgt, k = box_problem(16384, seed=0)
+    n = len(src)
+    ts, tt = bf.Quadtree(src, 64), bf.Quadtree(tgt, 64)
+    F = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, bf.options(**OPTS))
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = np.empty(n, complex)
+    y[tt.perm] = F @ x[ts.perm]
+    rows = np.sort(rng.choice(n, 512, replace=False))
+    err = rel(y[rows], exact_rows(k, tgt[rows], src, x))
+    print(f"box 16384, leaf 64, tol 1e-8: rel err vs 512 exact rows {err:.2e}")
+    assert err < 1e-7
+
+
+def test_curve_at_2_20_fp64_and_mixed(gpu_lib):
+    bf = gpu_lib
+    n = 1 << 20
+    src, tgt, k = curve_problem(n)
+    ts, tt = bf.Quadtree(src, 64, device=True), bf.Quadtree(tgt, 64, device=True)
+    F = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, bf.options(**OPTS))
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    rows = np.sort(rng.choice(n, 256, replace=False))
+    yd = exact_rows(k, tgt[rows], src, x)
+
+    dx = bf.DeviceArray.from_numpy(x[ts.perm])
+    dy = bf.DeviceArray(16 * n)
+    bf.mul_device(F, dx, dy, 1, "fp64")
+    y = np.empty(n, complex)
+    y[tt.perm] = dy.to_numpy(np.complex128, n)
+    err64 = rel(y[rows], yd)
+
+    F.orthogonalize()
+    F.prepare_mixed("fp32")
+    dx32 = bf.DeviceArray.from_numpy(x[ts.perm].astype(np.complex64))
+    dy32 = bf.DeviceArray(8 * n)
+    bf.mul_device(F, dx32, dy32, 1, "fp32")
+    y32 = np.empty(n, complex)
+    y32[tt.perm] = dy32.to_numpy(np.complex64, n).astype(complex)
+    err32 = rel(y32[rows], yd)
+
+    nr = 64
+    X = (rng.standard_normal((n, nr)) + 1j * rng.standard_normal((n, nr))).astype(np.complex64)
+    dX64 = bf.DeviceArray.from_numpy(X.astype(np.complex128))
+    dY64 = bf.DeviceArray(16 * n * nr)
+    bf.mul_device(F, dX64, dY64, nr, "fp64")
+    Y64 = dY64.to_numpy(np.complex128, (n, nr))
+    del dX64, dY64
+    dX = bf.DeviceArray.from_numpy(X)
+    dY = bf.DeviceArray(8 * n * nr)
+    bf.mul_device(F, dX, dY, nr, "fp32")
+    errN = rel(dY.to_numpy(np.complex64, (n, nr)).astype(complex), Y64)
+    print(f"curve 2^20: FP64 {err64:.2e}, FP32 1 RHS {err32:.2e} vs 256 exact rows; "
+          f"FP32 {nr} RHS vs FP64 {errN:.2e}")
+    assert err64 < 1e-7
+    assert err32 < 1e-6
+    assert errN < 1e-6
+
+
+def test_boundary_equation_at_2_18(gpu_lib):
+    import bench
+    bf = gpu_lib
+    out = bench.gmres_section(bf, 1 << 18, 1e-8, 1.25, 1e-11, 50, 1e-10)
+    print({k: out[k] for k in ("fp64_iterations", "fp64_rel_residual", "mixed_iterations", "mixed_refinements",
+                                "mixed_rel_residual", "mixed_vs_fp64_solution")})
+    assert out["fp64_rel_residual"] < 1e-9
+    assert out["mixed_rel_residual"] < 1e-9
+    assert out["mixed_vs_fp64_solution"] < 1e-8
+    assert out["fp64_iterations"] < 50
