@@ -366,6 +366,8 @@ def main():
     ap.add_argument("--batch-rhs", type=int, default=16)
     ap.add_argument("--stream-n", type=int, default=8192, help="streaming / rank-model section size (0 = skip)")
     ap.add_argument("--stream-tol", type=float, default=1e-7)
+    ap.add_argument("--split-tol", type=float, default=1e-9,
+                    help="tolerance of the hi+lo FP32 (fp32x2) section (0 = skip; needs --rows)")
     ap.add_argument("--tree", choices=["device", "host"], default="device",
                     help="build the two quadtrees on the GPU (bfhipQuadtreeInitWithLeafSize) or on the host")
     ap.add_argument("--factor-warmup", type=int, default=1 << 14,
@@ -554,6 +556,50 @@ def main():
               file=sys.stderr, flush=True)
         del Fm
 
+    # the split form: a tighter factorization (tol 1e-9) stored as hi + lo
+    # FP32 pairs, applied with FP32 vectors (pairs summed in FP64)
+    split = None
+    if args.split_tol and args.rows:
+        F = None  # the FP64 factorization's HBM goes first
+        t0 = time.perf_counter()
+        sopts = bf.options(tol=args.split_tol, rank_const=args.rank_const, pass_through=True,
+                           pinv_rcond=args.pinv_rcond)
+        Fs = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, sopts)
+        fs_s = time.perf_counter() - t0
+        bf.mul_device(Fs, dx, dy, 1, "fp64")
+        bf.lib().bfhipSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bf.mul_device(Fs, dx, dy, 1, "fp64")
+        bf.lib().bfhipSynchronize()
+        ts64 = (time.perf_counter() - t0) / args.steps
+        ys = np.empty(n, complex)
+        ys[tt.perm] = dy.to_numpy(np.complex128, n)
+        errs64 = float(np.linalg.norm(ys[rows] - yd) / np.linalg.norm(yd))
+        sortho = Fs.orthogonalize() if args.mixed_ortho else 0.0
+        Fs.prepare_mixed("fp32x2")
+        bf.lib().bfhipFacReleaseFp64(Fs.ptr)
+        dx1 = bf.DeviceArray.from_numpy(x[ts.perm].astype(np.complex64))
+        dy1 = bf.DeviceArray(8 * n)
+        bf.mul_device(Fs, dx1, dy1, 1, "fp32x2")
+        bf.lib().bfhipSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bf.mul_device(Fs, dx1, dy1, 1, "fp32x2")
+        bf.lib().bfhipSynchronize()
+        tsx = (time.perf_counter() - t0) / args.steps
+        ysx = np.empty(n, complex)
+        ysx[tt.perm] = dy1.to_numpy(np.complex64, n).astype(complex)
+        errsx = float(np.linalg.norm(ysx[rows] - yd) / np.linalg.norm(yd))
+        split = {"tol": args.split_tol, "factor_s": fs_s, "orthogonalize_s": sortho,
+                 "entries": Fs.stats()["entries"], "fp64_1rhs_ms": ts64 * 1e3,
+                 "fp64_rel_err_vs_dense_rows": errs64, "fp32x2_1rhs_ms": tsx * 1e3,
+                 "fp32x2_rel_err_vs_dense_rows": errsx}
+        print(f"[bench] split hi+lo at tol {args.split_tol:g}: factor {fs_s:.2f}s, {split['entries']:.3e} entries; "
+              f"1 RHS fp64 {ts64 * 1e3:.2f} ms at {errs64:.2e}, fp32x2 {tsx * 1e3:.2f} ms at {errsx:.2e} "
+              f"(vs exact rows)", file=sys.stderr, flush=True)
+        del Fs, dx1, dy1
+
     gm = None
     if args.gmres_n:
         gm = gmres_section(bf, args.gmres_n, args.tol, args.rank_const, args.pinv_rcond, args.gmres_restart,
@@ -614,6 +660,7 @@ def main():
                      "kernel": "k_apply1 (all stages of one apply)", "bytes_per_apply": bytes_per_apply,
                      "event_ms_per_apply": apply_ms},
         "mixed": mixed,
+        "split": split,
         "gmres": gm,
         "batch": bt,
         "stream": sm,
