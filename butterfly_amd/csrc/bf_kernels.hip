@@ -1857,8 +1857,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     }
     if (batches > 0) {
       /* the load cursor: term ct, batch offset cj (uniform) */
-      int ct = S.t0;
-      while (terms[ct].aOff < 0) ++ct;
+      int ct = S.t0; /* the first block term with columns (batches > 0: one exists) */
+      while (terms[ct].aOff < 0 || terms[ct].n <= 0) ++ct;
       BfkTerm CT = terms[ct];
       int cj = 0, issued = 0;
       V xa[2][APPLY1_BATCH], aa[2][APPLY1_BATCH], la[2][APPLY1_BATCH];
@@ -1881,7 +1881,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
           if (cj >= n) {
             cj = 0;
             do ++ct;
-            while (terms[ct].aOff < 0);
+            while (terms[ct].aOff < 0 || terms[ct].n <= 0);
             CT = terms[ct];
           }
         }
@@ -2138,8 +2138,8 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
       int arow[RT];
       for (int u = 0; u < RT; ++u) arow[u] = min(rt + 16 * u + lr, m - 1);
       /* the load cursor: term lt, its columns from lk0 */
-      int lt = Sg.t0;
-      while (terms[lt].aOff < 0) ++lt;
+      int lt = Sg.t0; /* the first block term with columns (steps > 0: one exists) */
+      while (terms[lt].aOff < 0 || terms[lt].n <= 0) ++lt;
       BfkTerm LT = terms[lt];
       const V *LX = bufs.in[LT.xbuf];
       int lk0 = 0, issued = 0;
@@ -2161,7 +2161,7 @@ __global__ __launch_bounds__(64) void k_applyN_mfma(const BfkSeg *__restrict__ s
           if (lk0 >= LT.n) {
             lk0 = 0;
             do ++lt;
-            while (terms[lt].aOff < 0);
+            while (terms[lt].aOff < 0 || terms[lt].n <= 0);
             LT = terms[lt];
             LX = bufs.in[LT.xbuf];
           }
