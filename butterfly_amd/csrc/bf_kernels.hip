@@ -130,6 +130,7 @@ __device__ __forceinline__ double2 circle_normal(double cx, double cy, double r,
  * into registers for the whole of a kernel that samples only in its
  * prologue */
 __device__ __attribute__((noinline)) double2 green_call(BfkLayer L, double k, double2 t, double2 s, double2 ns) {
+  if (L.layer == 3) return green_tnormal(k, t, s, ns); /* ns is the target's normal here */
   return green_layer(L, k, t, s, ns);
 }
 
@@ -163,9 +164,11 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void k_sample(const BfkKernelJob *_
     for (int64_t e = threadIdx.x; e < cnt; e += SAMPLE_THREADS) {
       int j = (int)(e / m);
       int i = (int)(e - (int64_t)j * m);
-      out[e] = L.layer == 0   ? green(J.k, tgtXY[J.tgtOff + i], spts[j])
-               : L.layer == 3 ? green_tnormal(J.k, tgtXY[J.tgtOff + i], spts[j], tgtN[J.tgtOff + i])
-                              : green_layer(L, J.k, tgtXY[J.tgtOff + i], spts[j], snrm[j]);
+      const double2 t = tgtXY[J.tgtOff + i];
+      /* the entry through a call: inlined, the Hankel coefficients were
+       * hoisted out of this loop into 428 registers (one wave per SIMD) */
+      out[e] = L.layer == 3 ? green_call(L, J.k, t, spts[j], tgtN[J.tgtOff + i])
+                            : green_call(L, J.k, t, spts[j], L.layer == 0 ? make_double2(0, 0) : snrm[j]);
     }
   }
 }
