@@ -192,6 +192,7 @@ def lib():
         "bfhipSetApplyConfig": (C.c_int, [C.c_int, C.c_int, C.c_int]),
         "bfhipSetApplyCompensation": (C.c_int, [C.c_int]),
         "bfhipSetApplyGraphs": (C.c_int, [C.c_int]),
+        "bfhipSetLstsqWaves": (None, [C.c_int64, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "bfhipFacTuneApply": (C.c_int, [vp, sz, C.c_int, C.c_char_p]),
         "bfhipFacGetApplyConfig": (C.c_int, [vp, sz, C.c_int, C.POINTER(C.c_int32)]),
         "bfhipFacPlanSizes": (None, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
@@ -537,6 +538,15 @@ class Factorization(Mat):
 def set_apply_graphs(on):
     """bfhipSetApplyGraphs: replay the apply's stage chain as a hipGraph."""
     lib().bfhipSetApplyGraphs(1 if on else 0)
+
+
+def set_lstsq_waves(two_above=-1, four_above=-1):
+    """bfhipSetLstsqWaves: re-expansion blocks whose LDS class exceeds
+    two_above (four_above) bytes are solved by 2 (4) waves; -1 keeps a
+    threshold.  Returns the previous (two_above, four_above)."""
+    a, b = C.c_int64(), C.c_int64()
+    lib().bfhipSetLstsqWaves(int(two_above), int(four_above), C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 class FacHelm2:

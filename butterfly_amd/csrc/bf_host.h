@@ -53,6 +53,7 @@ int bfk_lstsq_begin(size_t maxOversizeBytes, int64_t numOversize);
 int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t numJobs, size_t ldsBytes,
                      const double *dSrcXY, const double *dSrcN, BfkLayer layer, double *dArena);
 int bfk_lstsq_end(void);
+void bfk_set_lstsq_waves(int64_t two, int64_t four, int64_t *old2, int64_t *old4);
 int bfk_pair_green(int64_t n, const double *dA, const double *dB, const double *dNA, double k, double *dOut);
 int bfk_scale_rows(int64_t n, int nrhs, const double *dW, const void *dX, void *dU, int fp32);
 int bfk_bie_post(int64_t n, int nrhs, int ncorr, double alpha, const int32_t *dCol, const double *dC,

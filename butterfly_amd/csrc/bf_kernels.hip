@@ -23,6 +23,7 @@
 #include <type_traits>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 
 #include "bf_h0.h"
 #include "bf_plan.h"
@@ -51,6 +52,9 @@ static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 static hipEvent_t g_evL0 = nullptr, g_evL1 = nullptr;
 static int g_lPending = 0;
 static size_t g_ldsCap = 0;
+/* compression: LDS classes above g_lstsq2 bytes per job run 2 waves per
+ * job, above g_lstsq4 4 waves (BF_AMD_LSTSQ_WAVES="a,b" overrides both) */
+static size_t g_lstsq2 = 16384, g_lstsq4 = 40960;
 static bool g_tallOk = false; /* k_tall_jacobi has its LDS */
 static double g_times[8];   /* last measured ms per category */
 static double g_sum[8];     /* accumulated ms per category since the last reset */
@@ -441,6 +445,60 @@ __device__ __forceinline__ void wave_argmax(double &v, int &idx) {
   argmax_step(v, idx, __shfl_xor(v, 32, 64), __shfl_xor(idx, 32, 64));
 }
 
+/* Reductions over a workgroup of G threads (G/64 waves, G a multiple of
+ * 64): inside the wave by lane swaps, then, for G > 64, over the waves
+ * through `red` (2 G/64 doubles of LDS) between barriers; every thread gets
+ * the same result, summed in the same order.  Every thread must call them. */
+template <int G>
+__device__ __forceinline__ double blk_sum(double v, double *red) {
+  v = group_sum(v, 64);
+  if (G == 64) return v;
+  __syncthreads(); /* the previous reduction's reads of red are done */
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  v = red[0];
+  for (int q = 1; q < G / 64; ++q) v += red[q];
+  return v;
+}
+template <int G>
+__device__ __forceinline__ void blk_argmax(double &v, int &idx, double *red) {
+  wave_argmax(v, idx);
+  if (G == 64) return;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = v;
+    red[2 * (threadIdx.x >> 6) + 1] = (double)idx;
+  }
+  __syncthreads();
+  v = red[0];
+  idx = (int)red[1];
+  for (int q = 1; q < G / 64; ++q) argmax_step(v, idx, red[2 * q], (int)red[2 * q + 1]);
+}
+template <int G>
+__device__ __forceinline__ double blk_max(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if (G == 64) return v;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  v = red[0];
+  for (int q = 1; q < G / 64; ++q) v = fmax(v, red[q]);
+  return v;
+}
+/* (through `red`, not __syncthreads_or, whose work-group reduction takes
+ * static LDS and would leave the kernel short of the full 160 KiB) */
+template <int G>
+__device__ __forceinline__ bool blk_any(int v, double *red) {
+  const bool w = __any(v);
+  if (G == 64) return w;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w ? 1.0 : 0.0;
+  __syncthreads();
+  double s = red[0];
+  for (int q = 1; q < G / 64; ++q) s += red[q];
+  return s > 0;
+}
+
 /* round-robin (circle method) partner slot of position pos at step `step`
  * for P (even) rows: position 0 stays, the others rotate */
 __device__ __forceinline__ int rr_row_fast(int pos, int step, int P) {
@@ -495,17 +553,19 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 
 /* One-sided Jacobi on the rows of R (p rows, W columns with the right-hand
  * part), round-robin pairs, LP lanes per pair (a power of two fixed at
- * compile time so the strides fold into the LDS offsets).  A rotation of
+ * compile time so the strides fold into the LDS offsets), on a workgroup
+ * of G threads (one wave, or 2 or 4 waves for the larger blocks; `red` as
+ * for blk_sum).  A rotation of
  * rows i < j with g = r_i . conj(r_j), a = |r_i|^2, b = |r_j|^2:
  *   e = g/|g|, zeta = (b - a)/(2|g|), t = sign(zeta)/(|zeta| + sqrt(1 + zeta^2)),
  *   c = 1/sqrt(1 + t^2), s = c t;  r_i <- c r_i - s e r_j,  r_j <- s r_i + c e r_j. */
-template <int LP>
-__device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int W, int ld) {
+template <int LP, int G>
+__device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int W, int ld, double *red) {
   const int lane = threadIdx.x;
   const double eps = 2.220446049250313e-16;
   const int P = p + (p & 1);
   const int npairs = P / 2;
-  constexpr int pairsPerPass = 64 / LP;
+  constexpr int pairsPerPass = G / LP;
   const int slot = lane / LP;
   const int sub = lane - slot * LP;
   const double thr = 4.0 * eps * sqrt((double)p);
@@ -596,38 +656,44 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
         __syncthreads();
       }
     }
-    if (!__any(rotated)) break;
+    if (!blk_any<G>(rotated, red)) break;
     /* cyclic Jacobi converges quadratically: once every rotation of a sweep
      * was below 1e-8 in |g| / sqrt(a b), the next sweep's would be at the
      * rounding level (below thr), so that sweep, which would only confirm
      * convergence, is skipped */
-    for (int o = 32; o > 0; o >>= 1) worst = fmax(worst, __shfl_xor(worst, o, 64));
-    if (worst < 1e-16) break;
+    if (blk_max<G>(worst, red) < 1e-16) break;
   }
 }
 
-__device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, int n, int ld, double2 *__restrict__ w,
-                                               double2 *__restrict__ v, int *__restrict__ piv, double rcond,
-                                               double2 *__restrict__ out) {
+/* The solve of one re-expansion block by a workgroup of G threads: one
+ * wave for the small blocks, 2 or 4 waves for the larger ones, whose LDS
+ * (20-60 KiB) leaves room for only a few blocks per CU: more waves per
+ * block keep the CU busy where one wave per block would leave it waiting
+ * on LDS and dependent arithmetic.  Every branch below is uniform over the
+ * workgroup (its conditions come from block reductions). */
+template <int G>
+__device__ __forceinline__ void qr_jacobi_group(double2 *__restrict__ M, int p, int n, int ld, double2 *__restrict__ w,
+                                                double2 *__restrict__ v, int *__restrict__ piv, double *__restrict__ red,
+                                                double rcond, double2 *__restrict__ out) {
   const int lane = threadIdx.x;
   const int W = p + n;
   const double eps = 2.220446049250313e-16;
 
-  for (int i = lane; i < p; i += 64) piv[i] = i;
+  for (int i = lane; i < p; i += G) piv[i] = i;
   __syncthreads();
 
   /* ---- 1. pivoted Householder QR (lane = column) */
   for (int j = 0; j < p; ++j) {
     double best = -1.0;
     int bidx = p;
-    for (int c = j + lane; c < p; c += 64) {
+    for (int c = j + lane; c < p; c += G) {
       const double s = col_norm2(M, ld, c, j, p);
       if (s > best) { best = s; bidx = c; }
     }
-    wave_argmax(best, bidx);
+    blk_argmax<G>(best, bidx, red);
     const int q = __builtin_amdgcn_readfirstlane(bidx);
     if (q != j) {
-      for (int r = lane; r < p; r += 64) {
+      for (int r = lane; r < p; r += G) {
         const double2 t = M[r * ld + j];
         M[r * ld + j] = M[r * ld + q];
         M[r * ld + q] = t;
@@ -640,14 +706,14 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
       const double2 x0 = M[j * ld + j];
       const double ax0 = hypot(x0.x, x0.y);
       const double phx = ax0 > 0 ? x0.x / ax0 : 1.0, phy = ax0 > 0 ? x0.y / ax0 : 0.0;
-      for (int r = j + lane; r < p; r += 64) {
+      for (int r = j + lane; r < p; r += G) {
         double2 a = M[r * ld + j];
         if (r == j) a = make_double2(x0.x + phx * sig, x0.y + phy * sig);
         v[r] = a;
       }
       __syncthreads();
       const double inv = 1.0 / (sig * (sig + ax0));
-      for (int c = j + 1 + lane; c < W; c += 64) {
+      for (int c = j + 1 + lane; c < W; c += G) {
         double sr = 0, si = 0, tr = 0, ti = 0; /* v^H y in two chains */
         int r = j;
         for (; r + 1 < p; r += 2) {
@@ -685,7 +751,7 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
         }
       }
       __syncthreads();
-      for (int r = j + lane; r < p; r += 64)
+      for (int r = j + lane; r < p; r += G)
         M[r * ld + j] = (r == j) ? make_double2(-phx * sig, -phy * sig) : make_double2(0, 0);
     }
     __syncthreads();
@@ -698,16 +764,16 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
    * strict lower triangle of M, transposed, with its diagonal in w. */
   {
     double rf = 0;
-    for (int c = lane; c < p; c += 64) rf += col_norm2(M, ld, c, 0, c + 1);
-    rf = group_sum(rf, 64);
-    for (int i = lane; i < p; i += 64) {
+    for (int c = lane; c < p; c += G) rf += col_norm2(M, ld, c, 0, c + 1);
+    rf = blk_sum<G>(rf, red);
+    for (int i = lane; i < p; i += G) {
       const double2 d = M[i * ld + i];
       const double m2 = d.x * d.x + d.y * d.y;
       v[i] = make_double2(d.x / m2, -d.y / m2);
     }
     __syncthreads();
     double zf = 0;
-    for (int j = lane; j < p; j += 64) {
+    for (int j = lane; j < p; j += G) {
       const double2 zjj = v[j];
       w[j] = zjj;
       zf = zjj.x * zjj.x + zjj.y * zjj.y;
@@ -738,7 +804,7 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
         zf = fma(z.x, z.x, fma(z.y, z.y, zf));
       }
     }
-    zf = group_sum(zf, 64);
+    zf = blk_sum<G>(zf, red);
     const double sigMaxUp = sqrt(rf);
     const double cutUp = fmax((double)p * eps * sigMaxUp + eps, rcond * sigMaxUp);
     const bool direct = zf > 0 && zf <= 1.7976931348623157e308 && 1.0 / sqrt(zf) >= cutUp;
@@ -746,8 +812,8 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
     if (direct) {
       const int cnt = p * n;
       int c = lane / p, a = lane - (lane / p) * p;
-      const int dc = 64 / p, da = 64 - dc * p;
-      for (int e = lane; e < cnt; e += 64) {
+      const int dc = G / p, da = G - dc * p;
+      for (int e = lane; e < cnt; e += G) {
         const double2 za = w[a], b0 = M[a * ld + p + c];
         double ax = za.x * b0.x - za.y * b0.y, ay = za.x * b0.y + za.y * b0.x;
         double bx = 0, by = 0;
@@ -773,7 +839,7 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
       return;
     }
     /* back to R: clear the strict lower triangle again */
-    for (int i = lane; i < p; i += 64)
+    for (int i = lane; i < p; i += G)
       for (int k = 0; k < i; ++k) M[i * ld + k] = make_double2(0, 0);
     __syncthreads();
   }
@@ -782,38 +848,38 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
   {
     const int npairs = (p + (p & 1)) / 2;
     int Lp = 64;
-    while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
+    while (Lp > 1 && Lp * npairs > G) Lp >>= 1;
     switch (Lp) {
-      case 8: jacobi_rows<8>(M, p, W, ld); break;
-      case 4: jacobi_rows<4>(M, p, W, ld); break;
-      case 2: jacobi_rows<2>(M, p, W, ld); break;
-      case 1: jacobi_rows<1>(M, p, W, ld); break;
-      case 16: jacobi_rows<16>(M, p, W, ld); break;
-      case 32: jacobi_rows<32>(M, p, W, ld); break;
-      default: jacobi_rows<64>(M, p, W, ld); break;
+      case 8: jacobi_rows<8, G>(M, p, W, ld, red); break;
+      case 4: jacobi_rows<4, G>(M, p, W, ld, red); break;
+      case 2: jacobi_rows<2, G>(M, p, W, ld, red); break;
+      case 1: jacobi_rows<1, G>(M, p, W, ld, red); break;
+      case 16: jacobi_rows<16, G>(M, p, W, ld, red); break;
+      case 32: jacobi_rows<32, G>(M, p, W, ld, red); break;
+      default: jacobi_rows<64, G>(M, p, W, ld, red); break;
     }
   }
 
   /* ---- 3. truncation and X */
   double smax = 0;
-  for (int i = lane; i < p; i += 64) {
+  for (int i = lane; i < p; i += G) {
     const double s = row_norm2(M + i * ld, p);
     w[i].x = s;
     smax = fmax(smax, s);
   }
   int dummy = 0;
-  wave_argmax(smax, dummy);
+  blk_argmax<G>(smax, dummy, red);
   __syncthreads();
   const double tol = fmax((double)p * eps * sqrt(smax) + eps, rcond * sqrt(smax));
-  for (int i = lane; i < p; i += 64) {
+  for (int i = lane; i < p; i += G) {
     const double s = sqrt(w[i].x);
     w[i].y = (s >= tol) ? 1.0 / w[i].x : 0.0;
   }
   __syncthreads();
   const int cnt = p * n;
   int c = lane / p, a = lane - (lane / p) * p;
-  const int dc = 64 / p, da = 64 - dc * p;
-  for (int e = lane; e < cnt; e += 64) {
+  const int dc = G / p, da = G - dc * p;
+  for (int e = lane; e < cnt; e += G) {
     double ax = 0, ay = 0, bx = 0, by = 0;
     int i = 0;
     for (; i + 1 < p; i += 2) {
@@ -831,7 +897,7 @@ __device__ __forceinline__ void qr_jacobi_wave(double2 *__restrict__ M, int p, i
       ay = fma(w0, x0.x * y0.y - x0.y * y0.x, ay);
     }
     out[(size_t)c * p + piv[a]] = make_double2(ax + bx, ay + by);
-    /* next e = e + 64: advance (c, a) without dividing */
+    /* next e = e + G: advance (c, a) without dividing */
     c += dc;
     a += da;
     if (a >= p) { a -= p; ++c; }
@@ -892,8 +958,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_l
     L.M[i * ld + c] = green_call(Lay, J.k, tp[i], s, ns);
   }
   __syncthreads();
-  if (T == 64 && !GLOBAL)
-    qr_jacobi_wave(L.M, p, n, ld, L.w, L.v, L.piv, J.rcond, arena + J.out);
+  if (!GLOBAL)
+    qr_jacobi_group<T>(L.M, p, n, ld, L.w, L.v, L.piv, L.red, J.rcond, arena + J.out);
   else
     qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, J.rcond, arena + J.out);
 }
@@ -914,7 +980,7 @@ __global__ __launch_bounds__(64) void k_lstsq_loaded(const double2 *__restrict__
     L.M[i * ld + c] = c < p ? a[i * p + c] : b[i * n + (c - p)];
   }
   __syncthreads();
-  qr_jacobi_wave(L.M, p, n, ld, L.w, L.v, L.piv, 0.0, X + (size_t)blockIdx.x * p * n);
+  qr_jacobi_group<64>(L.M, p, n, ld, L.w, L.v, L.piv, L.red, 0.0, X + (size_t)blockIdx.x * p * n);
 }
 
 /* ------------------------------------ boundary integral operator, GMRES */
@@ -1395,13 +1461,13 @@ __global__ __launch_bounds__(64) void k_tall_jacobi(const int32_t *__restrict__ 
   int Lp = 64;
   while (Lp > 1 && Lp * npairs > 64) Lp >>= 1;
   switch (Lp) {
-    case 8: jacobi_rows<8>(lds, n, 2 * n, ld); break;
-    case 4: jacobi_rows<4>(lds, n, 2 * n, ld); break;
-    case 2: jacobi_rows<2>(lds, n, 2 * n, ld); break;
-    case 1: jacobi_rows<1>(lds, n, 2 * n, ld); break;
-    case 16: jacobi_rows<16>(lds, n, 2 * n, ld); break;
-    case 32: jacobi_rows<32>(lds, n, 2 * n, ld); break;
-    default: jacobi_rows<64>(lds, n, 2 * n, ld); break;
+    case 8: jacobi_rows<8, 64>(lds, n, 2 * n, ld, nullptr); break;
+    case 4: jacobi_rows<4, 64>(lds, n, 2 * n, ld, nullptr); break;
+    case 2: jacobi_rows<2, 64>(lds, n, 2 * n, ld, nullptr); break;
+    case 1: jacobi_rows<1, 64>(lds, n, 2 * n, ld, nullptr); break;
+    case 16: jacobi_rows<16, 64>(lds, n, 2 * n, ld, nullptr); break;
+    case 32: jacobi_rows<32, 64>(lds, n, 2 * n, ld, nullptr); break;
+    default: jacobi_rows<64, 64>(lds, n, 2 * n, ld, nullptr); break;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < n * ld; e += 64) rg[e] = lds[e];
@@ -2330,10 +2396,19 @@ int bfk_init(void) {
   hipDeviceProp_t prop;
   BFK_CHECK(hipGetDeviceProperties(&prop, 0));
   g_ldsCap = prop.sharedMemPerBlock;
+  if (const char *e = getenv("BF_AMD_LSTSQ_WAVES")) {
+    char *end = nullptr;
+    g_lstsq2 = (size_t)strtoull(e, &end, 10);
+    if (end && *end == ',') g_lstsq4 = (size_t)strtoull(end + 1, nullptr, 10);
+  }
   if (g_ldsCap > 65536) {
     if (hipFuncSetAttribute((const void *)k_lstsq<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)g_ldsCap) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_lstsq_loaded, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_ldsCap) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_lstsq<128, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_ldsCap) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_lstsq<256, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)g_ldsCap) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_ortho, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)g_ldsCap) != hipSuccess) {
@@ -2587,12 +2662,33 @@ int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t nu
   const int64_t maxGrid = 1 << 30;
   for (int64_t j0 = 0; j0 < numJobs; j0 += maxGrid) {
     int64_t nj = numJobs - j0 < maxGrid ? numJobs - j0 : maxGrid;
-    hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), ldsBytes, st, dOrder ? dJobs : dJobs + j0,
-                       dOrder ? dOrder + j0 : (const int32_t *)nullptr, (const double2 *)dSrcXY,
-                       (const double2 *)dSrcN, layer, (double2 *)dArena, (double2 *)nullptr, (int64_t)0);
+    const BfkLstsqJob *jb = dOrder ? dJobs : dJobs + j0;
+    const int32_t *ob = dOrder ? dOrder + j0 : (const int32_t *)nullptr;
+    if (ldsBytes > g_lstsq4)
+      hipLaunchKernelGGL((k_lstsq<256, false>), dim3((unsigned)nj), dim3(256), ldsBytes, st, jb, ob,
+                         (const double2 *)dSrcXY, (const double2 *)dSrcN, layer, (double2 *)dArena,
+                         (double2 *)nullptr, (int64_t)0);
+    else if (ldsBytes > g_lstsq2)
+      hipLaunchKernelGGL((k_lstsq<128, false>), dim3((unsigned)nj), dim3(128), ldsBytes, st, jb, ob,
+                         (const double2 *)dSrcXY, (const double2 *)dSrcN, layer, (double2 *)dArena,
+                         (double2 *)nullptr, (int64_t)0);
+    else
+      hipLaunchKernelGGL((k_lstsq<64, false>), dim3((unsigned)nj), dim3(64), ldsBytes, st, jb, ob,
+                         (const double2 *)dSrcXY, (const double2 *)dSrcN, layer, (double2 *)dArena,
+                         (double2 *)nullptr, (int64_t)0);
     BFK_CHECK(hipGetLastError());
   }
   return 0;
+}
+
+/* the LDS classes (bytes per job) above which the compression runs 2 and
+ * 4 waves per job; a negative value leaves that threshold as it is.  The
+ * previous values go to *old2 / *old4 when given. */
+void bfk_set_lstsq_waves(int64_t two, int64_t four, int64_t *old2, int64_t *old4) {
+  if (old2) *old2 = (int64_t)g_lstsq2;
+  if (old4) *old4 = (int64_t)g_lstsq4;
+  if (two >= 0) g_lstsq2 = (size_t)two;
+  if (four >= 0) g_lstsq4 = (size_t)four;
 }
 
 int bfk_lstsq_end(void) {

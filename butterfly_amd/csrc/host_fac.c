@@ -1096,6 +1096,12 @@ static int ensure_buffers(BfMatHipButterfly *F, size_t nrhs, size_t elemBytes) {
   return 0;
 }
 
+/* compression: the LDS classes (bytes per block) above which a
+ * re-expansion block gets 2 and 4 waves */
+void bfhipSetLstsqWaves(int64_t twoAbove, int64_t fourAbove, int64_t *oldTwo, int64_t *oldFour) {
+  bfk_set_lstsq_waves(twoAbove, fourAbove, oldTwo, oldFour);
+}
+
 /* apply graphs: off (0, the default) or on */
 static int g_applyGraphs = 0;
 

@@ -508,6 +508,16 @@ int bfhipSetApplyConfig(int wavesPerBlock, int unroll, int mfmaTiles);
  * graph); anything else recaptures.  For callers that apply one operator
  * many times (iterative solvers, batched serving). */
 int bfhipSetApplyGraphs(int on);
+/* new (no counterpart): waves per re-expansion block in the compression.
+ * Blocks whose LDS workspace (4-KiB classes) exceeds `twoAbove` bytes are
+ * solved by 2 waves, above `fourAbove` by 4 (defaults 16384 and 40960:
+ * the p >~ 22 blocks, whose LDS leaves room for only a few per CU); the
+ * others by one wave.  Every shape gives the reference's pseudo-inverse
+ * solution; they differ only in the order of rounding.  A negative value
+ * leaves that threshold unchanged; the previous values go to *oldTwo /
+ * *oldFour when not NULL.  Applies to the factorizations made after the
+ * call. */
+void bfhipSetLstsqWaves(int64_t twoAbove, int64_t fourAbove, int64_t *oldTwo, int64_t *oldFour);
 /* new (no counterpart): the apply's autotuner.  For each apply stage of
  * `fac` it times the launch shapes that fit `nrhs` (one right-hand side:
  * 1, 2 or 4 waves per block with either lane map; 16 and more: 1, 2, 4 or

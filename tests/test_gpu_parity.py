@@ -162,6 +162,50 @@ def test_reexpansion_blocks_solve_their_least_squares_problem(gpu_lib, case):
         assert r_dev < 1e-10 and r_dev < 1000 * r_orc + 1e-11, (i, m, n, r_dev, r_orc)
 
 
+def test_compression_with_one_two_and_four_waves_per_block(gpu_lib):
+    """Re-expansion blocks are solved by 1, 2 or 4 waves, chosen by their
+    LDS class (bfhipSetLstsqWaves).  Forcing every block onto each shape
+    must give the same least-squares solutions: operators that match exact
+    rows equally well and agree with each other to that level, and blocks
+    whose residuals are at the oracle's."""
+    bf = gpu_lib
+    src, tgt, k = curve_problem(8192)
+    big = 1 << 40
+    prev = bf.set_lstsq_waves()
+    assert 0 < prev[0] <= prev[1]
+    facs = {}
+    try:
+        for shape, (two, four) in {1: (big, big), 2: (0, big), 4: (0, 0)}.items():
+            bf.set_lstsq_waves(two, four)
+            facs[shape] = _factor_both(src, tgt, k, leaf=64, tol=1e-10, pass_through=True)
+    finally:
+        bf.set_lstsq_waves(*prev)
+    ts, tt = facs[1][1], facs[1][2]
+    orc.seed(7)
+    x = orc.complex_randn(len(src))
+    ys = {s_: _apply_orig(F, ts, tt, x) for s_, (F, _, _) in facs.items()}
+    rows = np.arange(0, len(tgt), 61)
+    yd = orc.dense_rows(k, tgt[rows], src, x)
+    errs = {s_: rel(y[rows], yd) for s_, y in ys.items()}
+    diffs = {s_: rel(ys[s_], ys[1]) for s_ in (2, 4)}
+    print(f"vs exact rows by waves per block {errs}; vs one wave {diffs}")
+    for s_ in (2, 4):
+        assert errs[s_] < 1e-6 and errs[s_] < 2 * errs[1] + 1e-12
+        assert diffs[s_] < 10 * max(errs[1], errs[s_])
+    O = orc.Fac(k, src, tgt, 64, 1e-10, 128 * 128, True)
+    stp = src[ts.perm]
+    for s_ in (2, 4):
+        F = facs[s_][0]
+        lt, ft, bt = F.plan()
+        idx = np.nonzero(bt[:, 4] == 0)[0]
+        for i in np.r_[idx[:: max(1, len(idx) // 30)], idx[np.argsort(bt[idx, 2])[-3:]]]:
+            m, n = bt[i, 2], bt[i, 3]
+            g = F.block_geometry(i)
+            r_dev = _residual(k, g, F.block(i, m, n), stp)
+            r_orc = _residual(k, g, O.block(i, m, n), stp)
+            assert r_dev < 1e-10 and r_dev < 1000 * r_orc + 1e-11, (s_, i, m, n, r_dev, r_orc)
+
+
 def test_apply_ragged_and_self(gpu_lib):
     # unequal sizes, and the self-interaction with coincident points
     orc.seed(5)
