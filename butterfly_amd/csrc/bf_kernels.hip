@@ -53,7 +53,9 @@ static hipEvent_t g_evL0 = nullptr, g_evL1 = nullptr;
 static int g_lPending = 0;
 static size_t g_ldsCap = 0;
 /* compression: LDS classes above g_lstsq2 bytes per job run 2 waves per
- * job, above g_lstsq4 4 waves (BF_AMD_LSTSQ_WAVES="a,b" overrides both) */
+ * job, above g_lstsq4 4 waves (BF_AMD_LSTSQ_WAVES="a,b" overrides both;
+ * measured at 2^20 on the curve: 1.34 s with one wave everywhere, 1.14 s
+ * with these; lower or higher thresholds and 8 waves were slower) */
 static size_t g_lstsq2 = 16384, g_lstsq4 = 40960;
 static bool g_tallOk = false; /* k_tall_jacobi has its LDS */
 static double g_times[8];   /* last measured ms per category */
@@ -193,213 +195,23 @@ __device__ __forceinline__ int rr_row(int pos, int step, int P) {
   return pos == 0 ? 0 : 1 + (pos - 1 + step) % (P - 1);
 }
 
-template <int T>
-__device__ __forceinline__ bool block_any(int v) {
-  if (T == 64) return __any(v);
-  return __syncthreads_or(v);
-}
-
-/* max of v over the T threads of the block, with the index of the first
- * maximal entry (smallest index on ties) */
-template <int T>
-__device__ __forceinline__ void block_argmax(double &v, int &idx, double *red) {
-  for (int o = 32; o > 0; o >>= 1) {
-    double ov = __shfl_xor(v, o, 64);
-    int oi = __shfl_xor(idx, o, 64);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
-  if (T > 64) {
-    const int wv = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) { red[2 * wv] = v; red[2 * wv + 1] = (double)idx; }
-    __syncthreads();
-    v = red[0];
-    idx = (int)red[1];
-    for (int q = 1; q < T / 64; ++q) {
-      double ov = red[2 * q];
-      int oi = (int)red[2 * q + 1];
-      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-    }
-    __syncthreads();
-  }
-}
-
-/* X = Z_equiv^+ Z_orig for M = [Z_equiv | Z_orig] (p x (p+n)):
+/* ---- the re-expansion solve X = Z_equiv^+ Z_orig for M = [Z_equiv | Z_orig]
+ * (p x (p+n)), on the workgroup's LDS slice (or, for the few blocks above
+ * the LDS capacity, a slice of global scratch):
  *  1. Householder QR of Z_equiv with column pivoting, Z_equiv P = Q R,
  *     applied to all of M, giving [R | Q^H Z_orig];
- *  2. one-sided Jacobi on the rows of R (rotations applied to whole rows)
- *     until they are orthogonal: rows become sigma_i v_i^H, the right part
- *     becomes U^H Q^H Z_orig (the QR step makes the graded rows converge
- *     in a few sweeps);
- *  3. keep sigma_i >= p eps sigma_0 + eps (mat_dense_complex.c:1797-1809)
- *     and form X = P sum_i R'_i^H B'_i / sigma_i^2, written column-major. */
-template <int T>
-__device__ void qr_jacobi_solve(double2 *M, int p, int n, int ld, double2 *w, double2 *v, double *red, int *piv, double rcond,
-                                double2 *__restrict__ out) {
-  const int lane = threadIdx.x;
-  const int W = p + n;
-  const double eps = 2.220446049250313e-16;
-
-  for (int i = lane; i < p; i += T) piv[i] = i;
-  __syncthreads();
-
-  /* ---- 1. pivoted Householder QR */
-  for (int j = 0; j < p; ++j) {
-    double best = -1.0;
-    int bidx = p;
-    for (int c = j + lane; c < p; c += T) {
-      double s = 0;
-      for (int r = j; r < p; ++r) {
-        double2 a = M[r * ld + c];
-        s = fma(a.x, a.x, fma(a.y, a.y, s));
-      }
-      if (s > best) { best = s; bidx = c; }
-    }
-    block_argmax<T>(best, bidx, red);
-    const int q = bidx;
-    if (q != j) {
-      for (int r = lane; r < p; r += T) {
-        double2 t = M[r * ld + j];
-        M[r * ld + j] = M[r * ld + q];
-        M[r * ld + q] = t;
-      }
-      if (lane == 0) { int t = piv[j]; piv[j] = piv[q]; piv[q] = t; }
-    }
-    __syncthreads();
-    const double sig = sqrt(best);
-    if (sig > 0) {
-      const double2 x0 = M[j * ld + j];
-      const double ax0 = hypot(x0.x, x0.y);
-      const double phx = ax0 > 0 ? x0.x / ax0 : 1.0, phy = ax0 > 0 ? x0.y / ax0 : 0.0;
-      for (int r = j + lane; r < p; r += T) {
-        double2 a = M[r * ld + j];
-        if (r == j) a = make_double2(x0.x + phx * sig, x0.y + phy * sig);
-        v[r] = a;
-      }
-      __syncthreads();
-      const double inv = 1.0 / (sig * (sig + ax0));
-      for (int c = j + 1 + lane; c < W; c += T) {
-        double sr = 0, si = 0; /* v^H y */
-        for (int r = j; r < p; ++r) {
-          double2 vr = v[r], y = M[r * ld + c];
-          sr = fma(vr.x, y.x, fma(vr.y, y.y, sr));
-          si = fma(vr.x, y.y, fma(-vr.y, y.x, si));
-        }
-        sr *= inv;
-        si *= inv;
-        for (int r = j; r < p; ++r) {
-          double2 vr = v[r], y = M[r * ld + c];
-          y.x -= sr * vr.x - si * vr.y;
-          y.y -= sr * vr.y + si * vr.x;
-          M[r * ld + c] = y;
-        }
-      }
-      __syncthreads();
-      for (int r = j + lane; r < p; r += T)
-        M[r * ld + j] = (r == j) ? make_double2(-phx * sig, -phy * sig) : make_double2(0, 0);
-    }
-    __syncthreads();
-  }
-
-  /* ---- 2. one-sided Jacobi on the rows of R */
-  const int P = p + (p & 1);
-  const int npairs = P / 2;
-  int Lp = 64;
-  while (Lp > 1 && Lp * npairs > T) Lp >>= 1;
-  const int pairsPerPass = T / Lp;
-  const int slot = lane / Lp;
-  const int sub = lane - slot * Lp;
-  const double thr = 4.0 * eps * sqrt((double)p);
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    int rotated = 0;
-    for (int step = 0; step < P - 1; ++step) {
-      for (int pb = 0; pb < npairs; pb += pairsPerPass) {
-        const int pr = pb + slot;
-        int i = -1, j = -1;
-        if (pr < npairs) {
-          int a = rr_row(pr, step, P), b = rr_row(P - 1 - pr, step, P);
-          i = min(a, b);
-          j = max(a, b);
-        }
-        const bool valid = pr < npairs && j < p;
-        double al = 0, be = 0, gx = 0, gy = 0;
-        if (valid) {
-          const double2 *ri = M + i * ld, *rj = M + j * ld;
-          for (int c = sub; c < p; c += Lp) {
-            double2 x = ri[c], y = rj[c];
-            al = fma(x.x, x.x, fma(x.y, x.y, al));
-            be = fma(y.x, y.x, fma(y.y, y.y, be));
-            gx = fma(x.x, y.x, fma(x.y, y.y, gx)); /* x conj(y) */
-            gy = fma(x.y, y.x, fma(-x.x, y.y, gy));
-          }
-        }
-        for (int o = 1; o < Lp; o <<= 1) {
-          al += __shfl_xor(al, o, 64);
-          be += __shfl_xor(be, o, 64);
-          gx += __shfl_xor(gx, o, 64);
-          gy += __shfl_xor(gy, o, 64);
-        }
-        const double ag2 = gx * gx + gy * gy;
-        if (valid && ag2 > 0 && ag2 > thr * thr * al * be) {
-          rotated = 1;
-          const double ag = sqrt(ag2);
-          const double ex = gx / ag, ey = gy / ag;
-          const double zeta = (be - al) / (2 * ag);
-          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
-          const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
-          double2 *ri = M + i * ld, *rj = M + j * ld;
-          for (int c = sub; c < W; c += Lp) {
-            double2 x = ri[c], y = rj[c];
-            double ux = ex * y.x - ey * y.y, uy = ex * y.y + ey * y.x; /* e y */
-            ri[c] = make_double2(cs * x.x - sn * ux, cs * x.y - sn * uy);
-            rj[c] = make_double2(sn * x.x + cs * ux, sn * x.y + cs * uy);
-          }
-        }
-        __syncthreads();
-      }
-    }
-    if (!block_any<T>(rotated)) break;
-  }
-
-  /* ---- 3. truncation and X */
-  double smax = 0;
-  for (int i = lane; i < p; i += T) {
-    const double2 *ri = M + i * ld;
-    double s = 0;
-    for (int c = 0; c < p; ++c) s = fma(ri[c].x, ri[c].x, fma(ri[c].y, ri[c].y, s));
-    w[i].x = s;
-    smax = fmax(smax, s);
-  }
-  int dummy = 0;
-  block_argmax<T>(smax, dummy, red);
-  __syncthreads();
-  const double tol = fmax((double)p * eps * sqrt(smax) + eps, rcond * sqrt(smax));
-  for (int i = lane; i < p; i += T) {
-    double s = sqrt(w[i].x);
-    w[i].y = (s >= tol) ? 1.0 / w[i].x : 0.0;
-  }
-  __syncthreads();
-  const int cnt = p * n;
-  for (int e = lane; e < cnt; e += T) {
-    const int c = e / p;
-    const int a = e - c * p;
-    double ax = 0, ay = 0;
-    for (int i = 0; i < p; ++i) {
-      const double wi = w[i].y;
-      if (wi == 0) continue;
-      double2 x = M[i * ld + a], y = M[i * ld + p + c];
-      ax = fma(wi, x.x * y.x + x.y * y.y, ax); /* conj(x) y */
-      ay = fma(wi, x.x * y.y - x.y * y.x, ay);
-    }
-    out[(size_t)c * p + piv[a]] = make_double2(ax, ay);
-  }
-}
-
-/* ---- the same solve for one wave on LDS, tuned for the small blocks that
- * make up almost all of a factorization (p <= 32): every loop issues its
- * LDS loads in batches before it waits, lane groups reduce with DPP lane
- * swaps instead of LDS shuffles, the round-robin needs no integer modulo,
- * and each rotation costs two FP64 divides instead of five. */
+ *  1b. X = P R^-1 Q^H Z_orig when the QR factor proves that no singular
+ *     value falls below the cut (the pseudo-inverse is then the inverse);
+ *  2. otherwise one-sided Jacobi on the rows of R (rotations applied to
+ *     whole rows) until they are orthogonal: rows become sigma_i v_i^H,
+ *     the right part U^H Q^H Z_orig (the QR step makes the graded rows
+ *     converge in a few sweeps);
+ *  3. keep sigma_i >= max(p eps sigma_0 + eps, rcond sigma_0)
+ *     (mat_dense_complex.c:1797-1809) and form X = P sum_i R'_i^H B'_i /
+ *     sigma_i^2, written column-major.
+ * Every loop issues its loads in batches before it waits, lane groups
+ * reduce with DPP lane swaps, the round-robin needs no integer modulo, and
+ * each rotation costs two FP64 divides. */
 
 /* x from the lane CTRL selects inside the lane's row of 16 (DPP) */
 template <int CTRL>
@@ -958,10 +770,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_l
     L.M[i * ld + c] = green_call(Lay, J.k, tp[i], s, ns);
   }
   __syncthreads();
-  if (!GLOBAL)
-    qr_jacobi_group<T>(L.M, p, n, ld, L.w, L.v, L.piv, L.red, J.rcond, arena + J.out);
-  else
-    qr_jacobi_solve<T>(L.M, p, n, ld, L.w, L.v, L.red, L.piv, J.rcond, arena + J.out);
+  qr_jacobi_group<T>(L.M, p, n, ld, L.w, L.v, L.piv, L.red, J.rcond, arena + J.out);
 }
 
 /* the same solve on matrices given in global memory (row-major A p x p,
@@ -2638,9 +2447,10 @@ int bfk_lstsq_begin(size_t maxOversizeBytes, int64_t numOversize) {
  * dOrder[0..numJobs) indexes dJobs, or with dOrder NULL the jobs are
  * consecutive.  Launches rotate over g_stream and the side streams: a class of a
  * few large blocks runs for milliseconds at one block's latency and low
- * occupancy, and overlaps the classes that hold most of the blocks.  Jobs
- * above the LDS capacity run 256 threads each on the scratch of
- * bfk_lstsq_begin, chunk after chunk on g_stream. */
+ * occupancy, and overlaps the classes that hold most of the blocks.  A job
+ * gets 1, 2 or 4 waves by its class (g_lstsq2, g_lstsq4).  Jobs above the
+ * LDS capacity run 256 threads each on the scratch of bfk_lstsq_begin,
+ * chunk after chunk on g_stream. */
 int bfk_lstsq_launch(const BfkLstsqJob *dJobs, const int32_t *dOrder, int64_t numJobs, size_t ldsBytes,
                      const double *dSrcXY, const double *dSrcN, BfkLayer layer, double *dArena) {
   if (numJobs <= 0) return 0;
