@@ -213,16 +213,19 @@ __device__ __forceinline__ int rr_row(int pos, int step, int P) {
  * reduce with DPP lane swaps, the round-robin needs no integer modulo, and
  * each rotation costs two FP64 divides. */
 
-/* x from the lane CTRL selects inside the lane's row of 16 (DPP) */
+/* x from the lane CTRL selects inside the lane's row of 16 (DPP).  Every
+ * control used here reads a lane of the same row, and the callers run with
+ * all lanes active, so no lane keeps an "old" value: the plain DPP move
+ * needs no destination initialized beforehand. */
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
 }
 /* partner lanes for groups of 2, 4, 8 and 16: quad_perm [1,0,3,2],
  * quad_perm [2,3,0,1], row_half_mirror, row_mirror.  After the k-th swap
@@ -285,17 +288,6 @@ __device__ __forceinline__ void blk_argmax(double &v, int &idx, double *red) {
   v = red[0];
   idx = (int)red[1];
   for (int q = 1; q < G / 64; ++q) argmax_step(v, idx, red[2 * q], (int)red[2 * q + 1]);
-}
-template <int G>
-__device__ __forceinline__ double blk_max(double v, double *red) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  if (G == 64) return v;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  v = red[0];
-  for (int q = 1; q < G / 64; ++q) v = fmax(v, red[q]);
-  return v;
 }
 /* (through `red`, not __syncthreads_or, whose work-group reduction takes
  * static LDS and would leave the kernel short of the full 160 KiB) */
@@ -384,7 +376,7 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
   const double thr2 = thr * thr;
   for (int sweep = 0; sweep < 40; ++sweep) {
     int rotated = 0;
-    double worst = 0; /* largest |g|^2 / (a b) rotated in this sweep */
+    int big = 0; /* a rotation of this sweep had |g|^2 >= 1e-16 a b */
     for (int step = 0; step < P - 1; ++step) {
       for (int pb = 0; pb < npairs; pb += pairsPerPass) {
         const int pr = pb + slot;
@@ -428,7 +420,7 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
         /* rows below 1e-145 in norm lie far under any cut; they are not rotated */
         if (valid && ag2 > 1e-290 && ag2 > thr2 * al * be) {
           rotated = 1;
-          worst = fmax(worst, ag2 / (al * be));
+          big |= ag2 >= 1e-16 * (al * be);
           const double rg = rsqrt_nr(ag2);
           const double ex = gx * rg, ey = gy * rg;
           const double zeta = (be - al) * (0.5 * rg);
@@ -437,8 +429,19 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
           if (az > 1e150) {
             t = 0.5 / zeta;
           } else {
+            /* t only chooses the angle: c = 1/sqrt(1 + t^2) and s = c t,
+             * both to full precision, make the rotation unitary whatever
+             * t's last bits, so t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2))
+             * comes from the hardware estimates and Newton steps (one for
+             * the square root, two for the reciprocal of d >= 1) */
             const double q1 = fma(zeta, zeta, 1.0);
-            t = (zeta >= 0 ? 1.0 : -1.0) / (az + q1 * rsqrt_nr(q1));
+            double y = __builtin_amdgcn_rsq(q1);
+            y = y * fma(-0.5 * q1 * y, y, 1.5);
+            const double d = fma(q1, y, az);
+            double r = __builtin_amdgcn_rcp(d);
+            r = fma(r, fma(-d, r, 1.0), r);
+            r = fma(r, fma(-d, r, 1.0), r);
+            t = zeta >= 0 ? r : -r;
           }
           const double cs = rsqrt_nr(fma(t, t, 1.0)), sn = cs * t;
           const double sx = sn * ex, sy = sn * ey, cx = cs * ex, cy = cs * ey;
@@ -473,7 +476,7 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
      * was below 1e-8 in |g| / sqrt(a b), the next sweep's would be at the
      * rounding level (below thr), so that sweep, which would only confirm
      * convergence, is skipped */
-    if (blk_max<G>(worst, red) < 1e-16) break;
+    if (!blk_any<G>(big, red)) break;
   }
 }
 
