@@ -622,7 +622,36 @@ __device__ __forceinline__ void qr_jacobi_group(double2 *__restrict__ M, int p, 
     zf = blk_sum<G>(zf, red);
     const double sigMaxUp = sqrt(rf);
     const double cutUp = fmax((double)p * eps * sigMaxUp + eps, rcond * sigMaxUp);
-    const bool direct = zf > 0 && zf <= 1.7976931348623157e308 && 1.0 / sqrt(zf) >= cutUp;
+    const bool finiteZ = zf > 0 && zf <= 1.7976931348623157e308;
+    bool direct = finiteZ && 1.0 / sqrt(zf) >= cutUp;
+    if (!direct && finiteZ) {
+      /* a tighter upper bound on sigma_max before giving up: ||R||_F
+       * exceeds it by up to sqrt(p) when several singular values are
+       * comparable, while (sum sigma^4)^(1/4) = ||R^H R||_F^(1/2) never
+       * falls below it and sits much closer.  B = R^H R entry by entry
+       * from R's upper triangle (B_ij = sum_{k <= min(i,j)} conj(R_ki) R_kj),
+       * summing |B_ij|^2 over i <= j, doubled off the diagonal. */
+      double b4 = 0;
+      int i = lane / p, j = lane - (lane / p) * p;
+      const int di = G / p, dj = G - di * p;
+      for (int e = lane; e < p * p; e += G) {
+        if (i <= j) {
+          double bx = 0, by = 0;
+          for (int k = 0; k <= i; ++k) {
+            const double2 a = M[k * ld + i], b = M[k * ld + j];
+            bx = fma(a.x, b.x, fma(a.y, b.y, bx)); /* conj(a) b */
+            by = fma(a.x, b.y, fma(-a.y, b.x, by));
+          }
+          b4 += (i == j ? 1.0 : 2.0) * fma(bx, bx, by * by);
+        }
+        j += dj;
+        i += di;
+        if (j >= p) { j -= p; ++i; }
+      }
+      b4 = blk_sum<G>(b4, red);
+      const double s4 = sqrt(sqrt(b4));
+      direct = 1.0 / sqrt(zf) >= fmax((double)p * eps * s4 + eps, rcond * s4);
+    }
     __syncthreads();
     if (direct) {
       const int cnt = p * n;
