@@ -480,6 +480,85 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
   }
 }
 
+/* Pivoted QR helpers: S lanes per column (S = 1, 2 or 4, an aligned group
+ * inside a quad), each lane over every S-th row from the step's first row,
+ * the S partial sums combined by lane swaps.  S is the widest split whose
+ * S x (columns) lanes fit in the workgroup, so the late steps of a small
+ * block, and the whole QR of a p ~ 16 block, keep most lanes busy. */
+template <int G>
+__device__ __forceinline__ int qr_split(int ncols) {
+  return 4 * ncols <= G ? 4 : 2 * ncols <= G ? 2 : 1;
+}
+
+/* the largest |column c|^2 over rows [j, p) among columns c in [j, p), and
+ * its column (the smallest index among maximal norms) */
+template <int S, int G>
+__device__ __forceinline__ void qr_pivot_norms(const double2 *__restrict__ M, int ld, int j, int p, double &best,
+                                               int &bidx) {
+  const int lane = threadIdx.x;
+  for (int t = lane; t < (p - j) * S; t += G) {
+    const int c = j + t / S;
+    double s0 = 0, s1 = 0;
+    int r = j + (t & (S - 1));
+    for (; r + S < p; r += 2 * S) {
+      const double2 a0 = M[r * ld + c], a1 = M[(r + S) * ld + c];
+      s0 = fma(a0.x, a0.x, fma(a0.y, a0.y, s0));
+      s1 = fma(a1.x, a1.x, fma(a1.y, a1.y, s1));
+    }
+    if (r < p) {
+      const double2 a0 = M[r * ld + c];
+      s0 = fma(a0.x, a0.x, fma(a0.y, a0.y, s0));
+    }
+    const double sn = group_sum(s0 + s1, S);
+    if (sn > best) { best = sn; bidx = c; }
+  }
+}
+
+/* columns j+1 .. W-1 of M minus v (v^H y) inv over rows [j, p) */
+template <int S, int G>
+__device__ __forceinline__ void qr_reflect(double2 *__restrict__ M, const double2 *__restrict__ v, int ld, int j,
+                                           int p, int W, double inv) {
+  const int lane = threadIdx.x;
+  for (int t = lane; t < (W - j - 1) * S; t += G) {
+    const int c = j + 1 + t / S, r0 = j + (t & (S - 1));
+    double sr = 0, si = 0, tr = 0, ti = 0; /* v^H y in two chains */
+    int r = r0;
+    for (; r + S < p; r += 2 * S) {
+      const double2 v0 = v[r], y0 = M[r * ld + c], v1 = v[r + S], y1 = M[(r + S) * ld + c];
+      sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+      si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+      tr = fma(v1.x, y1.x, fma(v1.y, y1.y, tr));
+      ti = fma(v1.x, y1.y, fma(-v1.y, y1.x, ti));
+    }
+    if (r < p) {
+      const double2 v0 = v[r], y0 = M[r * ld + c];
+      sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+      si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+    }
+    sr = group_sum(sr + tr, S) * inv;
+    si = group_sum(si + ti, S) * inv;
+    r = r0;
+    for (; r + 3 * S < p; r += 4 * S) {
+      double2 y[4], vv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) { vv[u] = v[r + u * S]; y[u] = M[(r + u * S) * ld + c]; }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        y[u].x -= sr * vv[u].x - si * vv[u].y;
+        y[u].y -= sr * vv[u].y + si * vv[u].x;
+        M[(r + u * S) * ld + c] = y[u];
+      }
+    }
+    for (; r < p; r += S) {
+      const double2 vr = v[r];
+      double2 y = M[r * ld + c];
+      y.x -= sr * vr.x - si * vr.y;
+      y.y -= sr * vr.y + si * vr.x;
+      M[r * ld + c] = y;
+    }
+  }
+}
+
 /* The solve of one re-expansion block by a workgroup of G threads: one
  * wave for the small blocks, 2 or 4 waves for the larger ones, whose LDS
  * (20-60 KiB) leaves room for only a few blocks per CU: more waves per
@@ -497,13 +576,14 @@ __device__ __forceinline__ void qr_jacobi_group(double2 *__restrict__ M, int p, 
   for (int i = lane; i < p; i += G) piv[i] = i;
   __syncthreads();
 
-  /* ---- 1. pivoted Householder QR (lane = column) */
+  /* ---- 1. pivoted Householder QR (S lanes per column, qr_split) */
   for (int j = 0; j < p; ++j) {
     double best = -1.0;
     int bidx = p;
-    for (int c = j + lane; c < p; c += G) {
-      const double s = col_norm2(M, ld, c, j, p);
-      if (s > best) { best = s; bidx = c; }
+    switch (qr_split<G>(p - j)) {
+      case 4: qr_pivot_norms<4, G>(M, ld, j, p, best, bidx); break;
+      case 2: qr_pivot_norms<2, G>(M, ld, j, p, best, bidx); break;
+      default: qr_pivot_norms<1, G>(M, ld, j, p, best, bidx); break;
     }
     blk_argmax<G>(best, bidx, red);
     const int q = __builtin_amdgcn_readfirstlane(bidx);
@@ -528,42 +608,10 @@ __device__ __forceinline__ void qr_jacobi_group(double2 *__restrict__ M, int p, 
       }
       __syncthreads();
       const double inv = 1.0 / (sig * (sig + ax0));
-      for (int c = j + 1 + lane; c < W; c += G) {
-        double sr = 0, si = 0, tr = 0, ti = 0; /* v^H y in two chains */
-        int r = j;
-        for (; r + 1 < p; r += 2) {
-          const double2 v0 = v[r], y0 = M[r * ld + c], v1 = v[r + 1], y1 = M[(r + 1) * ld + c];
-          sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
-          si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
-          tr = fma(v1.x, y1.x, fma(v1.y, y1.y, tr));
-          ti = fma(v1.x, y1.y, fma(-v1.y, y1.x, ti));
-        }
-        if (r < p) {
-          const double2 v0 = v[r], y0 = M[r * ld + c];
-          sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
-          si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
-        }
-        sr = (sr + tr) * inv;
-        si = (si + ti) * inv;
-        r = j;
-        for (; r + 3 < p; r += 4) {
-          double2 y[4], vv[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) { vv[u] = v[r + u]; y[u] = M[(r + u) * ld + c]; }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            y[u].x -= sr * vv[u].x - si * vv[u].y;
-            y[u].y -= sr * vv[u].y + si * vv[u].x;
-            M[(r + u) * ld + c] = y[u];
-          }
-        }
-        for (; r < p; ++r) {
-          const double2 vr = v[r];
-          double2 y = M[r * ld + c];
-          y.x -= sr * vr.x - si * vr.y;
-          y.y -= sr * vr.y + si * vr.x;
-          M[r * ld + c] = y;
-        }
+      switch (qr_split<G>(W - j - 1)) {
+        case 4: qr_reflect<4, G>(M, v, ld, j, p, W, inv); break;
+        case 2: qr_reflect<2, G>(M, v, ld, j, p, W, inv); break;
+        default: qr_reflect<1, G>(M, v, ld, j, p, W, inv); break;
       }
       __syncthreads();
       for (int r = j + lane; r < p; r += G)
@@ -791,9 +839,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) void k_l
     op[c] = J.origOff >= 0 ? srcXY[J.origOff + c] : circle_pt(J.ocx, J.ocy, J.orr, n, c);
   __syncthreads();
   const int W = p + n;
-  for (int e = lane; e < p * W; e += T) {
-    int i = e / W;
-    int c = e - i * W;
+  /* entry e = i W + c; (i, c) advance by T entries without dividing */
+  int i = lane / W, c = lane - (lane / W) * W;
+  const int di = T / W, dc = T - di * W;
+  for (int e = lane; e < p * W; e += T, i += di, c += dc) {
+    if (c >= W) { c -= W; ++i; }
     double2 s = c < p ? ep[c] : op[c - p];
     double2 ns = make_double2(0, 0);
     if (Lay.layer != 0)
