@@ -1524,44 +1524,84 @@ __host__ __device__ __forceinline__ size_t ortho_units(int rows, int n) {
 
 __device__ __forceinline__ double wave_sum(double v) { return group_sum(v, 64); }
 
-/* y -= v (v^H y) iv over rows [j, rows) of one column (a Householder
- * reflector I - iv v v^H); loads are issued in batches before the waits */
-__device__ __forceinline__ void house_apply(const double2 *__restrict__ v, double2 *__restrict__ y, int j, int rows,
-                                            double iv) {
-  double sr = 0, si = 0, tr = 0, ti = 0;
-  int r = j;
-  for (; r + 1 < rows; r += 2) {
-    const double2 v0 = v[r], y0 = y[r], v1 = v[r + 1], y1 = y[r + 1];
-    sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
-    si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
-    tr = fma(v1.x, y1.x, fma(v1.y, y1.y, tr));
-    ti = fma(v1.x, y1.y, fma(-v1.y, y1.x, ti));
-  }
-  if (r < rows) {
-    const double2 v0 = v[r], y0 = y[r];
-    sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
-    si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
-  }
-  sr = (sr + tr) * iv;
-  si = (si + ti) * iv;
-  r = j;
-  for (; r + 3 < rows; r += 4) {
-    double2 vv[4], yy[4];
+/* Columns j+1 .. n-1 of the column-major M (leading dimension ld) minus
+ * v (v^H y) iv over rows [j, rows), v = column j (a Householder reflector
+ * I - iv v v^H): S lanes per column (qr_split), each over every S-th row,
+ * the partial products combined by lane swaps; loads are issued in
+ * batches before the waits. */
+template <int S>
+__device__ __forceinline__ void ortho_reflect(double2 *__restrict__ M, int ld, int j, int rows, int n, double iv) {
+  const double2 *__restrict__ v = M + j * ld;
+  for (int t = threadIdx.x; t < (n - j - 1) * S; t += 64) {
+    double2 *__restrict__ y = M + (j + 1 + t / S) * ld;
+    const int r0 = j + (t & (S - 1));
+    double sr = 0, si = 0, tr = 0, ti = 0;
+    int r = r0;
+    for (; r + S < rows; r += 2 * S) {
+      const double2 v0 = v[r], y0 = y[r], v1 = v[r + S], y1 = y[r + S];
+      sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+      si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+      tr = fma(v1.x, y1.x, fma(v1.y, y1.y, tr));
+      ti = fma(v1.x, y1.y, fma(-v1.y, y1.x, ti));
+    }
+    if (r < rows) {
+      const double2 v0 = v[r], y0 = y[r];
+      sr = fma(v0.x, y0.x, fma(v0.y, y0.y, sr));
+      si = fma(v0.x, y0.y, fma(-v0.y, y0.x, si));
+    }
+    sr = group_sum(sr + tr, S) * iv;
+    si = group_sum(si + ti, S) * iv;
+    r = r0;
+    for (; r + 3 * S < rows; r += 4 * S) {
+      double2 vv[4], yy[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) { vv[u] = v[r + u]; yy[u] = y[r + u]; }
+      for (int u = 0; u < 4; ++u) { vv[u] = v[r + u * S]; yy[u] = y[r + u * S]; }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      yy[u].x -= sr * vv[u].x - si * vv[u].y;
-      yy[u].y -= sr * vv[u].y + si * vv[u].x;
-      y[r + u] = yy[u];
+      for (int u = 0; u < 4; ++u) {
+        yy[u].x -= sr * vv[u].x - si * vv[u].y;
+        yy[u].y -= sr * vv[u].y + si * vv[u].x;
+        y[r + u * S] = yy[u];
+      }
+    }
+    for (; r < rows; r += S) {
+      const double2 vr = v[r];
+      double2 yr = y[r];
+      yr.x -= sr * vr.x - si * vr.y;
+      yr.y -= sr * vr.y + si * vr.x;
+      y[r] = yr;
     }
   }
-  for (; r < rows; ++r) {
-    const double2 vr = v[r];
-    double2 yr = y[r];
-    yr.x -= sr * vr.x - si * vr.y;
-    yr.y -= sr * vr.y + si * vr.x;
-    y[r] = yr;
+}
+__device__ __forceinline__ void ortho_reflect_any(double2 *M, int ld, int j, int rows, int n, double iv) {
+  switch (qr_split<64>(n - j - 1)) {
+    case 4: ortho_reflect<4>(M, ld, j, rows, n, iv); break;
+    case 2: ortho_reflect<2>(M, ld, j, rows, n, iv); break;
+    default: ortho_reflect<1>(M, ld, j, rows, n, iv); break;
+  }
+}
+
+/* col <- R col in place for the n columns of one block (R upper
+ * triangular m x m, column-major in global memory): row r is
+ * sum_{k >= r} R[r][k] col[k], rows ascending, so each row reads only rows
+ * not yet overwritten; S lanes per column split the sum over k. */
+template <int S>
+__device__ __forceinline__ void ortho_rmul(double2 *__restrict__ M, int ld, int off, int m, int n,
+                                           const double2 *__restrict__ R) {
+  for (int t = threadIdx.x; t < n * S; t += 64) {
+    double2 *col = M + (t / S) * ld + off;
+    const int h = t & (S - 1);
+    for (int r = 0; r < m; ++r) {
+      double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+      int k = r + h;
+      for (; k + S < m; k += 2 * S) {
+        const double2 r0 = R[k * m + r], r1 = R[(k + S) * m + r], c0 = col[k], c1 = col[k + S];
+        a0 = cfma(r0, c0, a0);
+        a1 = cfma(r1, c1, a1);
+      }
+      if (k < m) a0 = cfma(R[k * m + r], col[k], a0);
+      const double sx = group_sum(a0.x + a1.x, S), sy = group_sum(a0.y + a1.y, S);
+      if (h == 0) col[r] = make_double2(sx, sy);
+    }
   }
 }
 
@@ -1593,23 +1633,10 @@ __global__ __launch_bounds__(64) void k_ortho(const BfkOrthoJob *__restrict__ jo
     __syncthreads();
     if (I.rIn >= 0) {
       const double2 *R = rIn + I.rIn;
-      for (int c = lane; c < n; c += 64) {
-        double2 *col = M + c * ld + off;
-        for (int r = 0; r < m; ++r) {
-          double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
-          int k = r;
-          for (; k + 3 < m; k += 4) {
-            const double2 r0 = R[k * m + r], r1 = R[(k + 1) * m + r], r2 = R[(k + 2) * m + r],
-                          r3 = R[(k + 3) * m + r];
-            const double2 c0 = col[k], c1 = col[k + 1], c2 = col[k + 2], c3 = col[k + 3];
-            a0 = cfma(r0, c0, a0);
-            a1 = cfma(r1, c1, a1);
-            a0 = cfma(r2, c2, a0);
-            a1 = cfma(r3, c3, a1);
-          }
-          for (; k < m; ++k) a0 = cfma(R[k * m + r], col[k], a0);
-          col[r] = make_double2(a0.x + a1.x, a0.y + a1.y);
-        }
+      switch (qr_split<64>(n)) {
+        case 4: ortho_rmul<4>(M, ld, off, m, n, R); break;
+        case 2: ortho_rmul<2>(M, ld, off, m, n, R); break;
+        default: ortho_rmul<1>(M, ld, off, m, n, R); break;
       }
       __syncthreads();
     }
@@ -1637,8 +1664,7 @@ __global__ __launch_bounds__(64) void k_ortho(const BfkOrthoJob *__restrict__ jo
         inv[j] = iv;
       }
       __syncthreads();
-      if (iv != 0)
-        for (int c = j + 1 + lane; c < n; c += 64) house_apply(M + j * ld, M + c * ld, j, rows, iv);
+      if (iv != 0) ortho_reflect_any(M, ld, j, rows, n, iv);
       __syncthreads();
     }
     double2 *Ro = rOut + J.rOut;
@@ -1654,8 +1680,7 @@ __global__ __launch_bounds__(64) void k_ortho(const BfkOrthoJob *__restrict__ jo
     __syncthreads();
     for (int j = n - 1; j >= 0; --j) {
       const double iv = inv[j];
-      if (iv != 0)
-        for (int c = j + 1 + lane; c < n; c += 64) house_apply(M + j * ld, M + c * ld, j, rows, iv);
+      if (iv != 0) ortho_reflect_any(M, ld, j, rows, n, iv);
       __syncthreads();
       const double2 v0 = M[j * ld + j];
       __syncthreads();
