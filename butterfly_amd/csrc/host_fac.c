@@ -1404,24 +1404,55 @@ static void ortho_push(OrthoJobVec *jobs, OrthoItemVec *items, const BfkOrthoIte
   BF_VEC_PUSH(*jobs, j);
 }
 
-int bfhipFacOrthogonalize(BfMat *fac) {
-  if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
-    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
-    return -1;
-  }
-  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
-  if (!F->dArena) {
-    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
-    return -1;
-  }
-  if (F->orthogonal) return 0;
-  double t0 = bfWallTime();
-  const int Fmax = F->numStages;
-  const size_t cap = bfk_lds_cap() / 16;
-  OrthoJobVec *jobs = calloc((size_t)Fmax, sizeof(OrthoJobVec));
+/* The orthogonalization's jobs for leaves [l0, l1): per stage g the jobs
+ * (item indices and R offsets local to this range), the items, and the R
+ * entries each stage writes.  Ranges of leaves are independent: a leaf's
+ * triangular factors pass only between its own stages. */
+typedef struct {
+  const BfMatHipButterfly *F;
+  size_t l0, l1;
+  int Fmax;
+  size_t cap;
+  OrthoJobVec *jobs; /* [Fmax] */
   OrthoItemVec items;
-  memset(&items, 0, sizeof(items));
-  int64_t *rFill = calloc((size_t)Fmax, sizeof(int64_t));
+  int64_t *rFill;    /* [Fmax] */
+  /* the merge: this range's first item, first job and first R entry per
+   * stage in the concatenated tables, and those tables */
+  int32_t itemBase;
+  int64_t *jobBase, *rBase; /* [Fmax + 1] */
+  OrthoJobVec *outJobs;     /* [Fmax] */
+  BfkOrthoItem *outItems;
+} OrthoPlanTask;
+
+/* copy one range's jobs and items into the concatenated tables, rebasing
+ * item indices and R offsets (the items of a stage-g job read the R that
+ * stage g + 1 writes) */
+static void *ortho_merge_range(void *arg) {
+  OrthoPlanTask *t = arg;
+  for (int g = 0; g < t->Fmax; ++g) {
+    const OrthoJobVec *src = &t->jobs[g];
+    BfkOrthoJob *dst = t->outJobs[g].a + t->jobBase[g];
+    for (size_t i = 0; i < src->n; ++i) {
+      BfkOrthoJob j = src->a[i];
+      for (int32_t it = j.i0; it < j.i1; ++it) {
+        BfkOrthoItem itm = t->items.a[it];
+        if (itm.rIn >= 0) itm.rIn += t->rBase[g + 1];
+        t->outItems[t->itemBase + it] = itm;
+      }
+      j.i0 += t->itemBase;
+      j.i1 += t->itemBase;
+      if (j.rOut >= 0) j.rOut += t->rBase[g];
+      dst[i] = j;
+    }
+  }
+  return NULL;
+}
+
+static void *ortho_plan_range(void *arg) {
+  OrthoPlanTask *t = arg;
+  const BfMatHipButterfly *F = t->F;
+  const int Fmax = t->Fmax;
+  const size_t cap = t->cap;
   BF_VEC(BfkOrthoItem) col;
   memset(&col, 0, sizeof(col));
   BF_VEC(int64_t) pend, npend, cstart, corder;
@@ -1431,8 +1462,7 @@ int bfhipFacOrthogonalize(BfMat *fac) {
   memset(&corder, 0, sizeof(corder));
   BF_VEC(char) rowOK;
   memset(&rowOK, 0, sizeof(rowOK));
-
-  for (size_t l = 0; l < F->leaves.n; ++l) {
+  for (size_t l = t->l0; l < t->l1; ++l) {
     BfHipLeaf *lf = &F->leaves.a[l];
     if (!lf->isButterfly) continue;
     const int nf = lf->numFactors;
@@ -1484,9 +1514,9 @@ int bfhipFacOrthogonalize(BfMat *fac) {
         if (col.n == 0) continue;
         const int fits = bfk_ortho_units(rows, n) <= cap;
         if (s >= 1 && rowOK.a[c] && allData && rows >= n && fits) {
-          const int64_t rOut = rFill[g];
-          rFill[g] += (int64_t)n * n;
-          ortho_push(&jobs[g], &items, col.a, (int)col.n, n, rOut);
+          const int64_t rOut = t->rFill[g];
+          t->rFill[g] += (int64_t)n * n;
+          ortho_push(&t->jobs[g], &t->items, col.a, (int)col.n, n, rOut);
           npend.a[c] = rOut;
         } else if (anyPend) {
           /* only the blocks whose rows carry a triangular factor change */
@@ -1494,9 +1524,9 @@ int bfhipFacOrthogonalize(BfMat *fac) {
           int prow = 0;
           for (size_t q = 0; q < col.n; ++q)
             if (col.a[q].rIn >= 0) { col.a[k++] = col.a[q]; prow += col.a[q].m; }
-          if (bfk_ortho_units(prow, n) <= cap) ortho_push(&jobs[g], &items, col.a, (int)k, n, -1);
+          if (bfk_ortho_units(prow, n) <= cap) ortho_push(&t->jobs[g], &t->items, col.a, (int)k, n, -1);
           else
-            for (size_t q = 0; q < k; ++q) ortho_push(&jobs[g], &items, &col.a[q], 1, n, -1);
+            for (size_t q = 0; q < k; ++q) ortho_push(&t->jobs[g], &t->items, &col.a[q], 1, n, -1);
         }
       }
       /* the next (earlier) factor's rows are this factor's columns */
@@ -1511,6 +1541,118 @@ int bfhipFacOrthogonalize(BfMat *fac) {
   BF_VEC_FREE(cstart);
   BF_VEC_FREE(corder);
   BF_VEC_FREE(rowOK);
+  return NULL;
+}
+
+int bfhipFacOrthogonalize(BfMat *fac) {
+  if (!fac || fac->vtbl != &bfMatHipButterflyVtable) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  BfMatHipButterfly *F = (BfMatHipButterfly *)fac;
+  if (!F->dArena) {
+    bfSetError(BF_ERROR_INVALID_ARGUMENTS);
+    return -1;
+  }
+  if (F->orthogonal) return 0;
+  double t0 = bfWallTime();
+  const int Fmax = F->numStages;
+  const size_t cap = bfk_lds_cap() / 16;
+  OrthoJobVec *jobs = calloc((size_t)Fmax, sizeof(OrthoJobVec));
+  OrthoItemVec items;
+  memset(&items, 0, sizeof(items));
+  int64_t *rFill = calloc((size_t)Fmax, sizeof(int64_t));
+
+  /* plan on threads over ranges of leaves of about equal block counts,
+   * then concatenate the ranges in order, rebasing item indices and R
+   * offsets: the same jobs, items and offsets as one pass over all leaves */
+  enum { ORTHO_NT = 16 };
+  int64_t totalBlocks = 0;
+  for (size_t l = 0; l < F->leaves.n; ++l) {
+    const BfHipLeaf *lf = &F->leaves.a[l];
+    if (!lf->isButterfly) continue;
+    for (int s = 0; s < lf->numFactors; ++s) totalBlocks += F->factors.a[lf->firstFactor + s].numBlocks;
+  }
+  const int nt = totalBlocks < (1 << 16) ? 1 : ORTHO_NT;
+  OrthoPlanTask tasks[ORTHO_NT];
+  memset(tasks, 0, sizeof(tasks));
+  {
+    size_t l = 0;
+    int64_t acc = 0;
+    for (int q = 0; q < nt; ++q) {
+      tasks[q].F = F;
+      tasks[q].Fmax = Fmax;
+      tasks[q].cap = cap;
+      tasks[q].l0 = l;
+      const int64_t target = totalBlocks * (q + 1) / nt;
+      while (l < F->leaves.n && (q == nt - 1 || acc < target)) {
+        const BfHipLeaf *lf = &F->leaves.a[l];
+        if (lf->isButterfly)
+          for (int s = 0; s < lf->numFactors; ++s) acc += F->factors.a[lf->firstFactor + s].numBlocks;
+        ++l;
+      }
+      tasks[q].l1 = l;
+      tasks[q].jobs = calloc((size_t)Fmax, sizeof(OrthoJobVec));
+      tasks[q].rFill = calloc((size_t)Fmax, sizeof(int64_t));
+    }
+  }
+  {
+    pthread_t th[ORTHO_NT];
+    int ok[ORTHO_NT] = {0};
+    for (int q = 1; q < nt; ++q) {
+      ok[q] = pthread_create(&th[q], NULL, ortho_plan_range, &tasks[q]) == 0;
+      if (!ok[q]) ortho_plan_range(&tasks[q]);
+    }
+    ortho_plan_range(&tasks[0]);
+    for (int q = 1; q < nt; ++q)
+      if (ok[q]) pthread_join(th[q], NULL);
+  }
+  {
+    int64_t *bases = calloc((size_t)nt * 2 * (Fmax + 1), sizeof(int64_t));
+    size_t nItems = 0;
+    for (int q = 0; q < nt; ++q) {
+      tasks[q].itemBase = (int32_t)nItems;
+      nItems += tasks[q].items.n;
+      tasks[q].jobBase = bases + (size_t)q * 2 * (Fmax + 1);
+      tasks[q].rBase = tasks[q].jobBase + (Fmax + 1);
+      tasks[q].outJobs = jobs;
+    }
+    items.a = malloc((nItems ? nItems : 1) * sizeof(BfkOrthoItem));
+    bfHugePages(items.a, nItems * sizeof(BfkOrthoItem));
+    items.cap = nItems ? nItems : 1;
+    items.n = nItems;
+    for (int g = 0; g < Fmax; ++g) {
+      size_t nj = 0;
+      for (int q = 0; q < nt; ++q) {
+        tasks[q].jobBase[g] = (int64_t)nj;
+        tasks[q].rBase[g] = rFill[g];
+        rFill[g] += tasks[q].rFill[g];
+        nj += tasks[q].jobs[g].n;
+      }
+      jobs[g].a = malloc((nj ? nj : 1) * sizeof(BfkOrthoJob));
+      bfHugePages(jobs[g].a, nj * sizeof(BfkOrthoJob));
+      jobs[g].cap = nj ? nj : 1;
+      jobs[g].n = nj;
+    }
+    for (int q = 0; q < nt; ++q) tasks[q].outItems = items.a;
+    pthread_t th[ORTHO_NT];
+    int ok[ORTHO_NT] = {0};
+    for (int q = 1; q < nt; ++q) {
+      ok[q] = pthread_create(&th[q], NULL, ortho_merge_range, &tasks[q]) == 0;
+      if (!ok[q]) ortho_merge_range(&tasks[q]);
+    }
+    ortho_merge_range(&tasks[0]);
+    for (int q = 1; q < nt; ++q)
+      if (ok[q]) pthread_join(th[q], NULL);
+    for (int q = 0; q < nt; ++q) {
+      for (int g = 0; g < Fmax; ++g) BF_VEC_FREE(tasks[q].jobs[g]);
+      free(tasks[q].jobs);
+      free(tasks[q].rFill);
+      BF_VEC_FREE(tasks[q].items);
+    }
+    free(bases);
+  }
+  bfTrace("ortho: jobs planned", t0);
 
   /* two R buffers, alternating by stage parity: stage g writes buffer
    * g % 2, stage g - 1 reads it */
@@ -1553,6 +1695,7 @@ int bfhipFacOrthogonalize(BfMat *fac) {
     if (!rc) rc = bfk_ortho(dJobs, dOrder, (int)gb.n, gs.a, gb.a, dItems, F->dArena, dR[(g + 1) % 2], dR[g % 2]);
     if (!rc) rc = bfk_sync();
   }
+  bfTrace("ortho: stages done", t0);
   BF_VEC_FREE(gs);
   BF_VEC_FREE(gb);
   free(cls);
