@@ -2370,18 +2370,72 @@ void *bfk_stream_handle(void) { return (void *)g_stream; }
 
 size_t bfk_lstsq_lds_bytes(int p, int n) { return lstsq_units(p, n) * 16; }
 
+/* One freed block of 16 GiB or more is kept for the next request it can
+ * serve (one that needs at least half of it): a factorization's arena is
+ * tens of GB, and a hipMalloc of that size right after the last one was
+ * freed waited up to a second while the driver cleared the freed memory
+ * (0.5-1.1 s at N = 2^20, against 0.4 ms).  A request that hipMalloc
+ * cannot serve releases the kept block and tries again.  Every kernel
+ * writes the arena entries it later reads, so a reused block needs no
+ * clearing; where zeros matter the library writes them (bfk_memset). */
+#define BFK_KEEP_MIN ((size_t)16 << 30)
+#define BFK_MAX_BIG 16
+struct BfkBigBlock {
+  void *p;
+  size_t bytes;
+};
+static BfkBigBlock g_big[BFK_MAX_BIG]; /* live blocks of BFK_KEEP_MIN and up */
+static int g_numBig = 0;
+static void *g_kept = nullptr;
+static size_t g_keptBytes = 0;
+
+static void big_track(void *p, size_t bytes) {
+  if (g_numBig < BFK_MAX_BIG) g_big[g_numBig++] = BfkBigBlock{p, bytes};
+}
+
 void *bfk_malloc(size_t bytes) {
   if (bfk_init()) return nullptr;
+  if (bytes >= BFK_KEEP_MIN && g_kept && g_keptBytes >= bytes && bytes >= g_keptBytes / 2) {
+    void *p = g_kept;
+    big_track(p, g_keptBytes);
+    g_kept = nullptr;
+    g_keptBytes = 0;
+    return p;
+  }
   void *p = nullptr;
   if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) {
     (void)hipGetLastError();
-    return nullptr;
+    if (!g_kept) return nullptr;
+    (void)hipFree(g_kept);
+    g_kept = nullptr;
+    g_keptBytes = 0;
+    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
   }
+  if (bytes >= BFK_KEEP_MIN) big_track(p, bytes);
   return p;
 }
 
+/* hipFree, or for a large block: wait for the device (as hipFree would),
+ * then keep it in place of the block kept before */
+static void release(void *p) {
+  for (int i = 0; i < g_numBig; ++i)
+    if (g_big[i].p == p) {
+      const size_t b = g_big[i].bytes;
+      g_big[i] = g_big[--g_numBig];
+      (void)hipDeviceSynchronize();
+      if (g_kept) (void)hipFree(g_kept);
+      g_kept = p;
+      g_keptBytes = b;
+      return;
+    }
+  (void)hipFree(p);
+}
+
 void bfk_free(void *p) {
-  if (p) (void)hipFree(p);
+  if (p) release(p);
 }
 
 /* released at the next bfk_sync, after the work queued before it is done;
@@ -2393,13 +2447,13 @@ void bfk_free_async(void *p) {
   if (!p) return;
   if (g_numDeferred == BFK_MAX_DEFERRED) {
     (void)hipStreamSynchronize(g_stream);
-    for (int i = 0; i < g_numDeferred; ++i) (void)hipFree(g_deferred[i]);
+    for (int i = 0; i < g_numDeferred; ++i) release(g_deferred[i]);
     g_numDeferred = 0;
   }
   g_deferred[g_numDeferred++] = p;
 }
 static void free_deferred(void) {
-  for (int i = 0; i < g_numDeferred; ++i) (void)hipFree(g_deferred[i]);
+  for (int i = 0; i < g_numDeferred; ++i) release(g_deferred[i]);
   g_numDeferred = 0;
 }
 
