@@ -11,7 +11,9 @@ oracle's restatement of the reference kernel, summed on host threads):
   benchmark runs the same kernels and shapes at 256);
 - the second-kind boundary equation I/2 + (S + C)W at 2^18 points on the
   curve, plane wave at 30 degrees: FP64 GMRES and the mixed solver reach
-  the requested residual and agree.
+  the requested residual and agree;
+- two factorizations of the curve at 2^19 in a row, the second on the
+  first one's freed arena: their applies agree bit for bit.
 """
 from concurrent.futures import ThreadPoolExecutor
 
@@ -114,3 +116,26 @@ def test_boundary_equation_at_2_18(gpu_lib):
     assert out["mixed_rel_residual"] < 1e-9
     assert out["mixed_vs_fp64_solution"] < 1e-8
     assert out["fp64_iterations"] < 50
+
+
+def test_refactoring_reuses_the_freed_arena(gpu_lib):
+    """A factorization's arena (tens of GB) is kept when the factorization
+    is deleted and serves the next one of similar size.  The kernels write
+    every arena entry they read, so the second factorization of the same
+    problem must apply bit for bit like the first."""
+    bf = gpu_lib
+    n = 1 << 19
+    src, tgt, k = curve_problem(n)
+    ts, tt = bf.Quadtree(src, 64, device=True), bf.Quadtree(tgt, 64, device=True)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dx = bf.DeviceArray.from_numpy(x[ts.perm])
+    dy = bf.DeviceArray(16 * n)
+    ys = []
+    for _ in range(2):
+        F = bf.FacHelm2.make_multilevel(bf.Helm2(k), ts, tt, bf.options(**OPTS))
+        assert F.stats()["entries"] * 16 >= 16 << 30  # arena-sized: the kept block serves it
+        bf.mul_device(F, dx, dy, 1, "fp64")
+        ys.append(dy.to_numpy(np.complex128, n))
+        del F
+    assert np.array_equal(ys[0], ys[1])
