@@ -376,7 +376,7 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
   const double thr2 = thr * thr;
   for (int sweep = 0; sweep < 40; ++sweep) {
     int rotated = 0;
-    int big = 0; /* a rotation of this sweep had |g|^2 >= 1e-16 a b */
+    int big = 0; /* a rotation of this sweep had |g|^2 >= 1e-12 a b */
     for (int step = 0; step < P - 1; ++step) {
       for (int pb = 0; pb < npairs; pb += pairsPerPass) {
         const int pr = pb + slot;
@@ -420,7 +420,7 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
         /* rows below 1e-145 in norm lie far under any cut; they are not rotated */
         if (valid && ag2 > 1e-290 && ag2 > thr2 * al * be) {
           rotated = 1;
-          big |= ag2 >= 1e-16 * (al * be);
+          big |= ag2 >= 1e-12 * (al * be);
           const double rg = rsqrt_nr(ag2);
           const double ex = gx * rg, ey = gy * rg;
           const double zeta = (be - al) * (0.5 * rg);
@@ -473,9 +473,10 @@ __device__ __forceinline__ void jacobi_rows(double2 *__restrict__ M, int p, int 
     }
     if (!blk_any<G>(rotated, red)) break;
     /* cyclic Jacobi converges quadratically: once every rotation of a sweep
-     * was below 1e-8 in |g| / sqrt(a b), the next sweep's would be at the
-     * rounding level (below thr), so that sweep, which would only confirm
-     * convergence, is skipped */
+     * was below 1e-6 in |g| / sqrt(a b), the rows it leaves are orthogonal
+     * to about 1e-12 relative, an order under the 1e-11 singular-value cut,
+     * so the next sweep, which would move singular values and vectors by
+     * that much, is skipped */
     if (!blk_any<G>(big, red)) break;
   }
 }
