@@ -653,8 +653,12 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   if (!rc) rc = bfk_lstsq_begin(bigBytes, numBig);
   const int begun = rc == 0;
   /* jobs per upload: small at first so that the kernels start early, then
-   * doubling up to 4M */
-  size_t chunk = (size_t)1 << 18;
+   * doubling up to 4M.  The uploads end at these bounds and at the end of
+   * the first sort pass, a class boundary, so a class is cut into launches
+   * at the same points whether or not the sort ran in two passes (the
+   * launches take the streams in turn: one more cut would move every later
+   * launch to another stream) */
+  size_t chunk = (size_t)1 << 18, bound = chunk;
   size_t up = 0, launched = 0, g = 0;
   int sortDone = 0;
   while (!rc && up < nj) {
@@ -665,8 +669,11 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
       avail = nj;
       bfTrace("factor: jobs sorted by class", tk);
     }
-    const size_t n = avail - up < chunk ? avail - up : chunk;
-    if (chunk < ((size_t)1 << 22)) chunk *= 2;
+    if (up == bound) {
+      if (chunk < ((size_t)1 << 22)) chunk *= 2;
+      bound += chunk;
+    }
+    const size_t n = (bound < avail ? bound : avail) - up;
     rc = bfk_h2d_side(dL + up, sorted + up, n * sizeof(BfkLstsqJob));
     up += n;
     /* what is uploaded: the rest of the current class, the classes that
