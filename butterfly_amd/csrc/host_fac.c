@@ -454,9 +454,7 @@ static void *apply_plan_task(void *arg) {
  * of NT threads classes a contiguous range and counts it, the class starts
  * are summed per thread, then each thread scatters its range.  Within a
  * class the jobs keep plan order, as group_by_class(..., 1) with a gather
- * would give.  The scatter runs in two passes: the largest classes (about
- * 1/64 of the jobs), which run longest and go out first, then the rest,
- * which the caller sorts while the first launches run. */
+ * would give. */
 enum { CLASS_NT = 16 };
 
 typedef struct {
@@ -465,7 +463,6 @@ typedef struct {
   uint32_t *cls;
   size_t i0, i1;
   uint32_t maxCls;
-  uint32_t lo, hi; /* phase 3 scatters the jobs of classes [lo, hi) */
   size_t *pos; /* per class: the next slot of this thread (phase 3), counts (phase 2) */
   int phase;
 } ClassTask;
@@ -483,10 +480,7 @@ static void *class_task(void *arg) {
   } else if (t->phase == 2) {
     for (size_t i = t->i0; i < t->i1; ++i) ++t->pos[t->cls[i]];
   } else {
-    for (size_t i = t->i0; i < t->i1; ++i) {
-      const uint32_t c = t->cls[i];
-      if (c >= t->lo && c < t->hi) t->dst[t->pos[c]++] = t->src[i];
-    }
+    for (size_t i = t->i0; i < t->i1; ++i) t->dst[t->pos[t->cls[i]]++] = t->src[i];
   }
   return NULL;
 }
@@ -504,33 +498,19 @@ static void class_phase(ClassTask *t, int nt, int phase) {
     if (ok[q]) pthread_join(th[q], NULL);
 }
 
-typedef struct {
+/* dst gets src's jobs class by class (largest first); gs gets each
+ * non-empty class's first position plus a final nj, gb its byte size */
+static int class_sort_jobs(const BfkLstsqJob *src, size_t nj, BfkLstsqJob *dst, BfI64Vec *gs, BfSizeVec *gb) {
   ClassTask t[CLASS_NT];
-  int nt;
-  uint32_t *cls;
-  size_t *pos;
-  uint32_t top; /* the first pass scattered the classes >= top */
-} ClassSort;
-
-/* Classes and counts src's jobs and scatters the largest classes into dst
- * (largest first, dst[0 .. *nFirst)); gs gets each non-empty class's first
- * position plus a final nj, gb its byte size.  Classes above capBytes (run
- * on shared scratch, launched together) all go in the first pass.
- * class_sort_finish scatters the rest. */
-static int class_sort_begin(ClassSort *S, const BfkLstsqJob *src, size_t nj, BfkLstsqJob *dst, size_t capBytes,
-                            BfI64Vec *gs, BfSizeVec *gb, size_t *nFirst) {
-  ClassTask *t = S->t;
   const int nt = nj < ((size_t)1 << 16) ? 1 : CLASS_NT;
-  S->nt = nt;
-  S->pos = NULL;
-  S->cls = malloc((nj ? nj : 1) * sizeof(uint32_t));
-  if (!S->cls) return -2;
-  bfHugePages(S->cls, nj * sizeof(uint32_t));
-  memset(t, 0, sizeof(S->t));
+  uint32_t *cls = malloc((nj ? nj : 1) * sizeof(uint32_t));
+  if (!cls) return -2;
+  bfHugePages(cls, nj * sizeof(uint32_t));
+  memset(t, 0, sizeof(t));
   for (int q = 0; q < nt; ++q) {
     t[q].src = src;
     t[q].dst = dst;
-    t[q].cls = S->cls;
+    t[q].cls = cls;
     t[q].i0 = nj * (size_t)q / (size_t)nt;
     t[q].i1 = nj * (size_t)(q + 1) / (size_t)nt;
   }
@@ -538,20 +518,16 @@ static int class_sort_begin(ClassSort *S, const BfkLstsqJob *src, size_t nj, Bfk
   uint32_t maxCls = 0;
   for (int q = 0; q < nt; ++q)
     if (t[q].maxCls > maxCls) maxCls = t[q].maxCls;
-  S->pos = calloc((size_t)nt * (maxCls + 1), sizeof(size_t));
-  if (!S->pos) {
-    free(S->cls);
-    S->cls = NULL;
+  size_t *pos = calloc((size_t)nt * (maxCls + 1), sizeof(size_t));
+  if (!pos) {
+    free(cls);
     return -2;
   }
-  for (int q = 0; q < nt; ++q) t[q].pos = S->pos + (size_t)q * (maxCls + 1);
+  for (int q = 0; q < nt; ++q) t[q].pos = pos + (size_t)q * (maxCls + 1);
   class_phase(t, nt, 2);
-  /* counts -> each thread's first slot per class, classes descending; the
-   * first pass takes classes from the top until it holds nj / 64 jobs and
-   * every class above capBytes (one pass when the jobs are few) */
+  /* counts -> each thread's first slot per class, classes descending */
   gs->n = gb->n = 0;
   size_t run = 0;
-  uint32_t top = nt == 1 ? 0 : maxCls + 1;
   for (int64_t c = maxCls; c >= 0; --c) {
     const size_t first = run;
     for (int q = 0; q < nt; ++q) {
@@ -563,35 +539,12 @@ static int class_sort_begin(ClassSort *S, const BfkLstsqJob *src, size_t nj, Bfk
       BF_VEC_PUSH(*gs, (int64_t)first);
       BF_VEC_PUSH(*gb, (size_t)c << 12);
     }
-    if (top == (uint32_t)c + 1 && ((first < nj / 64 && run - first <= nj / 8) || ((size_t)c << 12) > capBytes))
-      top = (uint32_t)c;
   }
   BF_VEC_PUSH(*gs, (int64_t)nj);
-  S->top = top;
-  *nFirst = 0;
-  for (size_t g = 0; g + 1 < gs->n; ++g)
-    if ((gb->a[g] >> 12) >= top) *nFirst = (size_t)gs->a[g + 1];
-  for (int q = 0; q < nt; ++q) {
-    t[q].lo = top;
-    t[q].hi = maxCls + 1;
-  }
   class_phase(t, nt, 3);
+  free(pos);
+  free(cls);
   return 0;
-}
-
-/* scatters the classes below S->top and frees the sort's tables */
-static void class_sort_finish(ClassSort *S) {
-  if (S->top > 0 && S->pos) {
-    for (int q = 0; q < S->nt; ++q) {
-      S->t[q].lo = 0;
-      S->t[q].hi = S->top;
-    }
-    class_phase(S->t, S->nt, 3);
-  }
-  free(S->pos);
-  free(S->cls);
-  S->pos = NULL;
-  S->cls = NULL;
 }
 
 /* Sampling, then the compression: the re-expansion jobs are grouped by
@@ -622,15 +575,13 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   memset(&gb, 0, sizeof(gb));
   BfkLstsqJob *sorted = malloc((nj ? nj : 1) * sizeof(BfkLstsqJob));
   bfHugePages(sorted, nj * sizeof(BfkLstsqJob));
-  ClassSort cs;
-  size_t avail = 0; /* jobs of sorted[] in place */
-  if (!sorted || class_sort_begin(&cs, F->lstsqJobs.a, nj, sorted, bfk_lds_cap(), &gs, &gb, &avail)) {
+  if (!sorted || class_sort_jobs(F->lstsqJobs.a, nj, sorted, &gs, &gb)) {
     free(sorted);
     BF_VEC_FREE(gs);
     BF_VEC_FREE(gb);
     return -2;
   }
-  bfTrace("factor: largest classes sorted", tk);
+  bfTrace("factor: jobs sorted by class", tk);
   /* the apply plan is host work: build it on a helper thread while this
    * thread uploads the jobs and the kernels run */
   ApplyPlanTask task = {F, 0, 0};
@@ -653,27 +604,12 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
   if (!rc) rc = bfk_lstsq_begin(bigBytes, numBig);
   const int begun = rc == 0;
   /* jobs per upload: small at first so that the kernels start early, then
-   * doubling up to 4M.  The uploads end at these bounds and at the end of
-   * the first sort pass, a class boundary, so a class is cut into launches
-   * at the same points whether or not the sort ran in two passes (the
-   * launches take the streams in turn: one more cut would move every later
-   * launch to another stream) */
-  size_t chunk = (size_t)1 << 18, bound = chunk;
+   * doubling up to 4M */
+  size_t chunk = (size_t)1 << 18;
   size_t up = 0, launched = 0, g = 0;
-  int sortDone = 0;
   while (!rc && up < nj) {
-    if (up == avail) {
-      /* the largest classes are launched: sort the rest while they run */
-      class_sort_finish(&cs);
-      sortDone = 1;
-      avail = nj;
-      bfTrace("factor: jobs sorted by class", tk);
-    }
-    if (up == bound) {
-      if (chunk < ((size_t)1 << 22)) chunk *= 2;
-      bound += chunk;
-    }
-    const size_t n = (bound < avail ? bound : avail) - up;
+    const size_t n = nj - up < chunk ? nj - up : chunk;
+    if (chunk < ((size_t)1 << 22)) chunk *= 2;
     rc = bfk_h2d_side(dL + up, sorted + up, n * sizeof(BfkLstsqJob));
     up += n;
     /* what is uploaded: the rest of the current class, the classes that
@@ -696,7 +632,6 @@ static int run_factor_kernels(BfMatHipButterfly *F, double *tApplyPlan) {
       if (launched == classEnd) g = gEnd;
     }
   }
-  if (!sortDone) class_sort_finish(&cs);
   bfTrace("factor: compression launched", tk);
   /* close the launch sequence even after a failed launch, so the side
    * streams are joined and the scratch is released */
